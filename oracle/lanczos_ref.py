@@ -1,0 +1,36 @@
+"""FP64 Lanczos restatement used as the eigenvalue oracle (test infrastructure).
+
+Follows the reference's `lanczos_p` loop (src/blanczosmain.f90:1582-1753):
+uniform pivot 1/sqrt(D) (src/blanczoslib.f90:825-831); w = H v_i;
+alpha_i = <v_i|w>, w -= alpha_i v_i (br_remove_prev_overlap,
+src/breorthog.f90:683-711); classical Gram-Schmidt against every stored vector,
+all dot products first and then one subtraction (br_orthogonalize, :831-891);
+beta_i = ||w||, v_{i+1} = w / beta_i (br_normalize, :758-783); Ritz values from
+the tridiagonal matrix (find_lanczos_eigenvalues, src/blanczoslib.f90:143-376).
+Everything is FP64 here (the reference keeps vectors in FP32 by default).
+"""
+import numpy as np
+
+
+def lanczos(apply, dim, niter, nkeep):
+    niter = min(niter, dim)
+    V = np.zeros((niter + 1, dim))
+    v = np.full(dim, 1.0 / np.sqrt(dim))
+    V[0] = v
+    alpha = np.zeros(niter)
+    beta = np.zeros(niter)
+    for i in range(niter):
+        w = apply(V[i])
+        a = float(V[i] @ w)
+        w = w - a * V[i]
+        d = V[: i + 1] @ w
+        w = w - d @ V[: i + 1]
+        b = float(np.linalg.norm(w))
+        alpha[i] = a
+        beta[i] = b
+        if i + 1 < niter + 1:
+            V[i + 1] = w / b if b > 0 else 0.0
+    T = np.diag(alpha) + np.diag(beta[:-1], 1) + np.diag(beta[:-1], -1)
+    e, y = np.linalg.eigh(T)
+    vecs = y[:, :nkeep].T @ V[:niter]
+    return alpha, beta, e[:nkeep], vecs

@@ -1,0 +1,221 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference
+BIGSTICK executable built by oracle/Makefile (oracle/_ref/bigstick.x).
+
+Runs only in the CPU container that has the reference build; the fixtures it
+writes are data (inputs' seeds, basis masks, Hamiltonian nonzeros, Lanczos
+coefficients, energies), committed so that tests never need the reference.
+
+    python tests/golden/make_golden.py [case ...]
+
+The reference is driven through its own stdin menu (src/bmenu.f90:11) and its
+own writers: `wh` -> ham.dat (src/blanczosmain.f90:2253-2294), `.res`
+(src/bparticleocc.f90:229-259), `.lcoef` (src/blanczosmain.f90:1685) and, with
+the oracle/ref_hooks.c flag switch, `.basis` (src/boutputlib.f90:473-700).
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+from bigstickpublick_amd.inputs import config_case, small_case, write_case, menu_script  # noqa: E402
+from oracle_api import read_basis_dump  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "bigstick.x")
+
+# (name, space, Z, N, 2Jz, seed): small systems covering the edge cases of the
+# path: no PP (Z=1), no protons (Z=0), odd A, M != 0, one particle (SPE not
+# subsumed, src/bsppot.f90:20-34), the pf space.
+SMALL = [
+    ("sd2p2n", "sd", 2, 2, 0, 77),
+    ("sd1p2n_m1", "sd", 1, 2, 1, 78),
+    ("sd3p1n_mm2", "sd", 3, 1, -2, 79),
+    ("sd0p3n_m3", "sd", 0, 3, 3, 80),
+    ("sd1p0n_m1", "sd", 1, 0, 1, 81),
+    ("sd1p1n", "sd", 1, 1, 0, 82),
+    ("pf2p1n_m3", "pf", 2, 1, 3, 83),
+    ("pf2p2n", "pf", 2, 2, 0, 84),
+]
+
+
+def run_ref(workdir, script, env_extra=None, timeout=3600):
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = "8"
+    if env_extra:
+        env.update(env_extra)
+    with open(os.path.join(workdir, "stdin.txt"), "w") as f:
+        f.write(script)
+    with open(os.path.join(workdir, "stdin.txt")) as fin, \
+            open(os.path.join(workdir, "stdout.txt"), "w") as fout:
+        r = subprocess.run([REF], stdin=fin, stdout=fout, stderr=subprocess.STDOUT,
+                           cwd=workdir, env=env, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError("reference run failed in %s" % workdir)
+
+
+def read_res(path):
+    """Energies, J, T from the .res table (format at src/bparticleocc.f90:256)."""
+    rows = []
+    with open(path) as f:
+        lines = f.readlines()
+    for i, line in enumerate(lines):
+        if line.strip().startswith("State") and "Ex" in line:
+            for l2 in lines[i + 1:]:
+                p = l2.split()
+                if len(p) < 5 or not p[0].isdigit():
+                    break
+                rows.append([float(p[1]), float(p[3]), float(p[4])])
+            break
+    return np.array(rows)
+
+
+def read_lcoef(path):
+    a, b = [], []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 3 and p[0].lstrip("-").isdigit():
+                a.append(float(p[1]))
+                b.append(float(p[2]))
+    return np.array(a), np.array(b)
+
+
+def read_hamdat(path):
+    with open(path) as f:
+        D = int(f.readline())
+        data = np.loadtxt(f, ndmin=2)
+    return D, data
+
+
+def probe_vectors(D):
+    """Two fixed probe vectors for H x fixtures (also rebuilt by the tests)."""
+    i = np.arange(D, dtype=np.float64)
+    return np.stack([np.sin(1.3 * i + 0.1), np.full(D, 1.0 / np.sqrt(D))], axis=1)
+
+
+def read_opcounts(txt):
+    """'Counts of many-body matrix elements (operations)' block and the total
+    (src/bigstick_main.f90:1150-1200)."""
+    out = {}
+    lines = txt.splitlines()
+    for i, line in enumerate(lines):
+        if "Counts of many-body matrix elements" in line:
+            for l2 in lines[i + 1:i + 8]:
+                p = l2.split()
+                if len(p) == 2 and p[0] in ("PN", "PP", "NN", "SPE"):
+                    out["nops_" + p[0].lower()] = int(p[1])
+        if "total # operations" in line:
+            out["nops_total"] = int(line.split("operations")[1].split(",")[0])
+    return out
+
+
+def small(name, space, Z, N, jz2, seed):
+    case = small_case(space, Z, N, jz2, seed=seed)
+    case.name = name
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        # the `wh` dump races under OpenMP (its output changes run to run with
+        # more than one thread), so it is written single-threaded
+        run_ref(wd, menu_script(case, "wh"), env_extra={"OMP_NUM_THREADS": "1"})
+        D, data = read_hamdat(os.path.join(wd, "ham.dat"))
+        lanc = case.niter if case.niter < D else max(D - 1, 1)
+        case.niter = lanc
+        case.nkeep = min(case.nkeep, lanc)
+        run_ref(wd, menu_script(case, "n", "lf"), env_extra={"BS_REF_PRINT": "b"})
+        nsps = 12 if space == "sd" else 20
+        pm, nm, tr = read_basis_dump(os.path.join(wd, name + ".basis"), nsps)
+        res = read_res(os.path.join(wd, name + ".res"))
+        a, b = read_lcoef(os.path.join(wd, name + ".lcoef"))
+        hi = data[:, 0].astype(np.int64) - 1
+        hj = data[:, 1].astype(np.int64) - 1
+        hv = data[:, 2]
+        if len(data) <= 40000:
+            ham = dict(ham_i=hi.astype(np.int32), ham_j=hj.astype(np.int32),
+                       ham_v=hv.astype(np.float32))
+        else:
+            # larger matrices: keep y = H x for two fixed probe vectors
+            X = probe_vectors(D)
+            Y = np.zeros_like(X)
+            for k in range(X.shape[1]):
+                np.add.at(Y[:, k], hi, hv * X[hj, k])
+                off = hi != hj
+                np.add.at(Y[:, k], hj[off], hv[off] * X[hi[off], k])
+            ham = dict(ham_hx=Y)
+        np.savez_compressed(
+            os.path.join(HERE, name + ".npz"), dim=D, **ham,
+            pmask=np.array([pm[i] for i in range(len(pm))], dtype=np.uint64),
+            nmask=np.array([nm[i] for i in range(len(nm))], dtype=np.uint64),
+            states=tr, res=res, alpha=a, beta=b)
+        meta = dict(space=space, Z=Z, N=N, jz2=jz2, seed=seed, nkeep=case.nkeep, niter=case.niter)
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(meta, f, indent=1)
+        print(name, "D=%d" % D, "nnz(i<=j)=%d" % len(data), "E0=%s" % res[0, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
+def config(cfg, with_basis=True, niter=None):
+    case = config_case(cfg)
+    if niter is not None:
+        case.niter = niter
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        env = {"BS_REF_PRINT": "b"} if with_basis else None
+        run_ref(wd, menu_script(case, "n", "lf"), env_extra=env, timeout=7200)
+        res = read_res(os.path.join(wd, case.name + ".res"))
+        a, b = read_lcoef(os.path.join(wd, case.name + ".lcoef"))
+        out = dict(res=res, alpha=a, beta=b)
+        if with_basis:
+            nsps = 12 if case.space == "sd" else 20
+            pm, nm, tr = read_basis_dump(os.path.join(wd, case.name + ".basis"), nsps)
+            # pstart/nstart: state = pstart[p] + nstart[n] (0-based)
+            P = max(pm) + 1
+            Nn = max(nm) + 1
+            pstart = np.full(P, -1, np.int64)
+            nstart = np.full(Nn, -1, np.int64)
+            # first state of each p row has the first n of its conjugate sector
+            for s, p, n in tr:
+                if pstart[p] < 0:
+                    pstart[p] = s - 0  # provisional; fixed below
+            # resolve with one known nstart per sector: nstart is 0-based within sector
+            nfirst = {}
+            for s, p, n in tr:
+                nfirst.setdefault(p, (s, n))
+            for p, (s, n) in nfirst.items():
+                pstart[p] = s
+            # s = pstart[p] + nstart[n]  with nstart[first n of row] = 0
+            for s, p, n in tr:
+                nstart[n] = s - pstart[p]
+            out.update(pmask=np.array([pm[i] for i in range(P)], dtype=np.uint64),
+                       nmask=np.array([nm[i] for i in range(Nn)], dtype=np.uint64),
+                       pstart=pstart, nstart=nstart, dim=len(tr))
+        with open(os.path.join(wd, "stdout.txt")) as f:
+            txt = f.read()
+        out.update(read_opcounts(txt))
+        out["niter"] = case.niter
+        np.savez_compressed(os.path.join(HERE, "config%d.npz" % cfg), **out)
+        print("config", cfg, case.name, "E=", res[:, 0])
+        return txt
+    finally:
+        shutil.rmtree(wd)
+
+
+if __name__ == "__main__":
+    want = sys.argv[1:] or [c[0] for c in SMALL] + ["config0", "config1"]
+    for c in SMALL:
+        if c[0] in want:
+            small(*c)
+    for cfg in (0, 1):
+        if "config%d" % cfg in want:
+            config(cfg, with_basis=True)
+    if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps
+        config(3, with_basis=True, niter=20)

@@ -1,0 +1,48 @@
+"""Helpers that rebuild the inputs behind each committed fixture."""
+import json
+import os
+
+import numpy as np
+
+from bigstickpublick_amd.inputs import config_case, small_case, write_case
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = ["sd2p2n", "sd1p2n_m1", "sd3p1n_mm2", "sd0p3n_m3", "sd1p0n_m1", "sd1p1n",
+         "pf2p1n_m3", "pf2p2n"]
+
+
+def small_fixture(name, workdir):
+    with open(os.path.join(GOLD, name + ".json")) as f:
+        meta = json.load(f)
+    case = small_case(meta["space"], meta["Z"], meta["N"], meta["jz2"], seed=meta["seed"])
+    case.name = name
+    d = os.path.join(workdir, name)
+    write_case(case, d)
+    fx = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    return case, d, fx
+
+
+def config_fixture(cfg, workdir):
+    case = config_case(cfg)
+    d = os.path.join(workdir, "cfg%d" % cfg)
+    write_case(case, d)
+    fx = dict(np.load(os.path.join(GOLD, "config%d.npz" % cfg)))
+    return case, d, fx
+
+
+def paths(case, d):
+    return os.path.join(d, case.space + ".sps"), os.path.join(d, case.name + ".int")
+
+
+def probe_vectors(D):
+    i = np.arange(D, dtype=np.float64)
+    return np.stack([np.sin(1.3 * i + 0.1), np.full(D, 1.0 / np.sqrt(D))], axis=1)
+
+
+def dense_from_fixture(fx):
+    D = int(fx["dim"])
+    R = np.zeros((D, D))
+    i, j, v = fx["ham_i"], fx["ham_j"], fx["ham_v"].astype(np.float64)
+    R[i, j] = v
+    R[j, i] = v
+    return R

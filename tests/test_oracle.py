@@ -1,0 +1,146 @@
+"""The oracle (oracle/bsoracle.c) pinned against the reference's own outputs
+(tests/golden, written by tests/golden/make_golden.py from the reference run)
+and against closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from golden_cases import (SMALL, config_fixture, dense_from_fixture, paths, probe_vectors,
+                          small_fixture)
+from lanczos_ref import lanczos
+from oracle_api import Oracle, lib
+
+
+def cg_exact(j1, m1, j2, m2, J, M):
+    """Racah's closed formula in exact rationals (arguments are 2j, 2m)."""
+    from fractions import Fraction
+    if m1 + m2 != M or J < abs(j1 - j2) or J > j1 + j2 or (j1 + j2 + J) % 2:
+        return 0.0
+    if abs(m1) > j1 or abs(m2) > j2 or abs(M) > J:
+        return 0.0
+    f = math.factorial
+    a, b, c = j1 // 1, j2, J
+
+    def h(x):
+        return x // 2
+    pre = Fraction((J + 1) * f(h(j1 + j2 - J)) * f(h(j1 - j2 + J)) * f(h(-j1 + j2 + J)),
+                   f(h(j1 + j2 + J) + 1))
+    pre *= f(h(J + M)) * f(h(J - M)) * f(h(j1 - m1)) * f(h(j1 + m1)) * f(h(j2 - m2)) * f(h(j2 + m2))
+    s = Fraction(0)
+    for k in range(0, 100):
+        d = [k, h(j1 + j2 - J) - k, h(j1 - m1) - k, h(j2 + m2) - k, h(J - j2 + m1) + k,
+             h(J - j1 - m2) + k]
+        if min(d) < 0:
+            if d[1] < 0 or d[2] < 0 or d[3] < 0:
+                break
+            continue
+        den = 1
+        for x in d:
+            den *= f(x)
+        s += Fraction((-1) ** k, den)
+    del a, b, c
+    return float(s) * math.sqrt(float(pre))
+
+
+def test_cleb_against_racah_formula():
+    """src/libra.f CLEB (restated in float) against exact Racah values."""
+    L = lib()
+    worst = 0.0
+    for j1 in range(1, 10, 2):
+        for j2 in range(1, 10, 2):
+            for J in range(abs(j1 - j2), j1 + j2 + 1, 2):
+                for m1 in range(-j1, j1 + 1, 2):
+                    for m2 in range(-j2, j2 + 1, 2):
+                        M = m1 + m2
+                        if abs(M) > J:
+                            continue
+                        got = L.bso_cleb(j1, m1, j2, m2, J, M)
+                        worst = max(worst, abs(got - cg_exact(j1, m1, j2, m2, J, M)))
+    assert worst < 2e-6
+
+
+def test_cleb_special_cases():
+    L = lib()
+    for j in range(1, 12, 2):
+        for m in range(-j, j + 1, 2):
+            want = (-1) ** ((j - m) // 2) / math.sqrt(j + 1)
+            assert abs(L.bso_cleb(j, m, j, -m, 0, 0) - want) < 1e-6
+        assert abs(L.bso_cleb(j, j, j, j, 2 * j, 2 * j) - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_basis_matches_reference(name, casedir):
+    case, d, fx = small_fixture(name, casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    assert o.dim == int(fx["dim"])
+    mp, sp, _ = o.sds(0)
+    mn, sn, _ = o.sds(1)
+    np.testing.assert_array_equal(mp, fx["pmask"])
+    np.testing.assert_array_equal(mn, fx["nmask"])
+    st = fx["states"]
+    np.testing.assert_array_equal(sp[st[:, 1]] + sn[st[:, 2]], st[:, 0])
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_hamiltonian_matches_reference(name, casedir):
+    """H from the oracle against the reference's `wh` dump (ham.dat): the
+    reference sums rows in FP32, so agreement is to a few FP32 ulp."""
+    case, d, fx = small_fixture(name, casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    if "ham_hx" in fx:
+        X = probe_vectors(o.dim)
+        for k in range(X.shape[1]):
+            y = o.apply(X[:, k])
+            ref = fx["ham_hx"][:, k]
+            assert np.abs(y - ref).max() < 4e-6 * max(1.0, np.abs(ref).max())
+        return
+    H = o.dense()
+    R = dense_from_fixture(fx)
+    scale = np.maximum(np.abs(R), 1.0)
+    assert (np.abs(H - R) / scale).max() < 4 * 2.0 ** -23 * 16
+    assert np.array_equal(np.abs(H) > 1e-5, np.abs(R) > 1e-5)
+    # eigenvalues of the two matrices
+    e1 = np.linalg.eigvalsh(H)[:5]
+    e2 = np.linalg.eigvalsh(R)[:5]
+    assert np.abs(e1 - e2).max() < 2e-5
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_config_basis_and_spectrum(cfg, casedir):
+    """Configs 0/1: basis bit-exact; FP64 Lanczos from the same uniform pivot
+    reproduces the reference's alpha/beta (FP32 vectors there) and energies."""
+    case, d, fx = config_fixture(cfg, casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    assert o.dim == int(fx["dim"])
+    mp, sp, _ = o.sds(0)
+    mn, sn, _ = o.sds(1)
+    np.testing.assert_array_equal(mp, fx["pmask"])
+    np.testing.assert_array_equal(mn, fx["nmask"])
+    np.testing.assert_array_equal(sp, fx["pstart"])
+    np.testing.assert_array_equal(sn, fx["nstart"])
+    if cfg != 0:
+        return
+    a, b, e, _ = lanczos(o.apply, o.dim, 150, case.nkeep)
+    ra, rb = fx["alpha"], fx["beta"]
+    assert np.abs(a[:30] - ra[:30]).max() < 2e-5
+    assert np.abs(b[:30] - rb[:30]).max() < 2e-5
+    assert np.abs(e - fx["res"][: case.nkeep, 0]).max() < 2e-5
+
+
+def test_hermitian_and_planted_mistake(casedir):
+    """A flipped sign in one coupled TBME must break agreement with ham.dat."""
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    sps, intf = paths(case, d)
+    o = Oracle(sps, intf, case.Z, case.N, case.jz2)
+    H = o.dense()
+    assert np.abs(H - H.T).max() == 0.0
+    lines = open(intf).read().splitlines()
+    p = lines[5].split()
+    p[6] = repr(-float(p[6]))
+    lines[5] = " ".join(p)
+    bad = intf.replace(".int", "_bad.int")
+    open(bad, "w").write("\n".join(lines) + "\n")
+    Hb = Oracle(sps, bad, case.Z, case.N, case.jz2).dense()
+    R = dense_from_fixture(fx)
+    assert np.abs(Hb - R).max() > 1e-2
