@@ -1,0 +1,162 @@
+"""ctypes binding of libbigstick_amd.so (include/bigstick_amd.h).
+
+The library must have been built (bigstickpublick_amd.build); nothing here
+falls back to another implementation: a missing library is an error."""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbigstick_amd.so")
+BS_F32, BS_F64 = 4, 8
+
+_lib = None
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("nkeep", ctypes.c_int), ("niter", ctypes.c_int), ("thick", ctypes.c_int),
+                ("max_restarts", ctypes.c_int), ("dtype", ctypes.c_int), ("tol", ctypes.c_double),
+                ("polish", ctypes.c_int)]
+
+
+def load_library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libbigstick_amd.so is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64p = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)
+    f32 = ctypes.c_float
+    L.bs_last_error.restype = ctypes.c_char_p
+    for name in ("bs_init_host", "bs_init"):
+        fn = getattr(L, name)
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, i32, i32, f32, f32, f32, f32,
+                       ctypes.POINTER(vp)]
+    L.bs_upload.argtypes = [vp]
+    L.bs_finalize.argtypes = [vp]
+    L.bs_dim.argtypes = [vp, i64p]
+    L.bs_num_sd.argtypes = [vp, i32, i64p]
+    L.bs_basis.argtypes = [vp, i32, vp, vp]
+    L.bs_num_sps.argtypes = [vp, ctypes.POINTER(i32)]
+    L.bs_sps.argtypes = [vp, vp, vp, vp]
+    L.bs_me_table.argtypes = [vp, i32, ctypes.POINTER(i32), vp]
+    L.bs_op_counts.argtypes = [vp, vp]
+    L.bs_apply_h.argtypes = [vp, vp, vp, i32, i32, vp]
+    L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
+                             ctypes.POINTER(i32)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("bigstick_amd: " + load_library().bs_last_error().decode())
+
+
+@dataclass
+class LanczosResult:
+    evals: np.ndarray
+    alpha: np.ndarray
+    beta: np.ndarray
+    resid: np.ndarray
+    matvecs: int
+    evecs: object = None
+
+
+class System:
+    """One nucleus in one model space: tables built once on the host, kept in
+    HBM after upload()."""
+
+    def __init__(self, sps, intfile, Z, N, jz2=0, scaling=(1.0, 1.0, 1.0, 1.0), upload=False):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.bs_init_host(str(sps).encode(), str(intfile).encode(), Z, N, jz2,
+                              *[float(v) for v in scaling], ctypes.byref(h)))
+        self.h = h
+        d = ctypes.c_int64()
+        _check(L.bs_dim(h, ctypes.byref(d)))
+        self.dim = d.value
+        n = ctypes.c_int()
+        _check(L.bs_num_sps(h, ctypes.byref(n)))
+        self.nsps = n.value
+        self.uploaded = False
+        if upload:
+            self.upload()
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().bs_finalize(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def upload(self):
+        if not self.uploaded:
+            _check(load_library().bs_upload(self.h))
+            self.uploaded = True
+
+    def basis(self, it):
+        L = load_library()
+        n = ctypes.c_int64()
+        _check(L.bs_num_sd(self.h, it, ctypes.byref(n)))
+        mask = np.zeros(n.value, np.uint64)
+        start = np.zeros(n.value, np.int64)
+        _check(L.bs_basis(self.h, it, mask.ctypes.data, start.ctypes.data))
+        return mask, start
+
+    def me_table(self, kind):
+        L = load_library()
+        n = ctypes.c_int()
+        _check(L.bs_me_table(self.h, kind, ctypes.byref(n), None))
+        t = np.zeros((n.value, n.value), np.float32)
+        _check(L.bs_me_table(self.h, kind, ctypes.byref(n), t.ctypes.data))
+        return t
+
+    def op_counts(self):
+        out = np.zeros(8, np.int64)
+        _check(load_library().bs_op_counts(self.h, out.ctypes.data))
+        return dict(spe=int(out[0]), pp=int(out[1]), nn=int(out[2]), pn=int(out[3]),
+                    ref_spe=int(out[4]), ref_pp=int(out[5]), ref_nn=int(out[6]),
+                    ref_pn=int(out[7]), ref_total=int(out[4:].sum()))
+
+    # ---- device side: torch tensors are used only as device buffers ----
+    def apply(self, x, y=None, accumulate=False, stream=None):
+        import torch
+        self.upload()
+        if x.dtype not in (torch.float32, torch.float64) or not x.is_cuda or x.numel() != self.dim:
+            raise ValueError("x must be a contiguous CUDA float32/float64 tensor of length dim")
+        x = x.contiguous()
+        if y is None:
+            y = torch.empty_like(x)
+        dt = BS_F32 if x.dtype == torch.float32 else BS_F64
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().bs_apply_h(self.h, ctypes.c_void_p(x.data_ptr()),
+                                         ctypes.c_void_p(y.data_ptr()), dt, int(accumulate),
+                                         ctypes.c_void_p(st)))
+        return y
+
+    def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
+                want_vectors=False):
+        import torch
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish))
+        m = min(niter, self.dim)
+        alpha = np.zeros(m)
+        beta = np.zeros(m)
+        ev = np.zeros(nkeep)
+        res = np.zeros(nkeep)
+        it = ctypes.c_int()
+        vec = None
+        if want_vectors:
+            vec = torch.empty((nkeep, self.dim), device="cuda",
+                              dtype=torch.float32 if dt == BS_F32 else torch.float64)
+        torch.cuda.synchronize()
+        _check(load_library().bs_lanczos(self.h, ctypes.byref(p), alpha.ctypes.data,
+                                         beta.ctypes.data, ev.ctypes.data,
+                                         ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
+                                         res.ctypes.data, ctypes.byref(it)))
+        return LanczosResult(ev, alpha, beta, res, it.value, vec)

@@ -1,0 +1,112 @@
+/*
+ * bigstick_amd.h -- C ABI of the MI355X M-scheme Hamiltonian engine.
+ *
+ * Plain C types only; every function returns 0 on success or a negative error
+ * code (message in bs_last_error()).  Device pointers are HIP device memory on
+ * the current device; `stream` is a hipStream_t (NULL = default stream).
+ * Calls on one system are not reentrant (the reference's apply routine is not
+ * either, src/bapplyhlibOMP.f90:934).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   bs_init      setup chain of src/bigstick_main.f90:305-676: get_orbit_info
+ *                (src/binput.f90:43), define_system (:649), basismaster
+ *                (src/bbasislib.f90:92), master_readin_tbmes
+ *                (src/binput_tbme.f90:58), subsume_sp_pot_into_2body
+ *                (src/bsppot.f90:39), jumpmaster (src/bjumplib_master.f90:342),
+ *                uncoupleXXtbme / count_uncouplepn (src/btbme.f90:1209,
+ *                src/btbmepn.f90:177)
+ *   bs_apply_h   applyHbundled_omp('n') (src/bapplyhlibOMP.f90:882) and
+ *                applyHbundled_g (src/bapplyhlib1.f90:56): vec2 (+)= H vec1
+ *   bs_lanczos   lanczos_p (src/blanczosmain.f90:1413-1793) with the br_*
+ *                reorthogonalization (src/breorthog.f90:683-891),
+ *                find_lanczos_eigenvalues (src/blanczoslib.f90:143-376) and
+ *                thick_restart_sub_p (src/blanczoslib.f90:1656-1799)
+ *   bs_finalize  end of run deallocation
+ */
+#ifndef BIGSTICK_AMD_H
+#define BIGSTICK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bs_system bs_system;
+
+#define BS_F32 4
+#define BS_F64 8
+
+/* Build the basis, jump tables and FP32 matrix elements from an iso .sps and
+ * an OXBASH iso .int file for Z protons and N neutrons at 2*Jz = jz2, with the
+ * reference's scaling line (spescale, A0, A, x): TBMEs scaled by (A0/A)^x, or
+ * by A0 when A or x is zero (src/binput_tbme.f90:1024-1031).  Host only; no
+ * device memory is touched. */
+int bs_init_host(const char *sps_path, const char *int_path, int Z, int N, int jz2,
+                 float spscale, float a0, float a, float x, bs_system **out);
+
+/* Copy the tables to the current HIP device (once).  Required before any
+ * device call. */
+int bs_upload(bs_system *sys);
+
+/* bs_init_host + bs_upload. */
+int bs_init(const char *sps_path, const char *int_path, int Z, int N, int jz2,
+            float spscale, float a0, float a, float x, bs_system **out);
+
+int bs_finalize(bs_system *sys);
+const char *bs_last_error(void);
+
+/* ---- host-side inspection (no device needed) ---- */
+int bs_dim(const bs_system *sys, int64_t *dim);
+/* it = 0 protons, 1 neutrons */
+int bs_num_sd(const bs_system *sys, int it, int64_t *nsd);
+/* occupation masks over the reference's single-particle labels (left m<0
+ * states first, then m>=0; bit k = label k+1) and the 0-based offsets
+ * pstart/nstart with state = pstart[p] + nstart[n] (src/bbasislib.f90:539-677). */
+int bs_basis(const bs_system *sys, int it, uint64_t *mask, int64_t *start);
+int bs_num_sps(const bs_system *sys, int *nsps);
+/* single-particle labels: orbit index (0-based, .sps order), 2j, 2m */
+int bs_sps(const bs_system *sys, int *orb, int *j2, int *m2);
+/* FP32 uncoupled matrix elements, dense npair x npair row-major:
+ * kind 0 pp, 1 nn (pair index c1*(c1-1)/2+c2, c1>c2 labels, 0-based),
+ * kind 2 pn (pair index cp*nsps+cn).  Pass table=NULL to query npair. */
+int bs_me_table(const bs_system *sys, int kind, int *npair, float *table);
+/* Work counts per H x.  out[0..3] = SPE, PP, NN, PN updates done by this
+ * engine (two-body terms merged per target/source pair); out[4..7] = the
+ * same terms counted the reference's way, whose sum is the "total #
+ * operations" printed at src/bigstick_main.f90:1192 (two-body operator
+ * applications, diagonal pair terms twice). */
+int bs_op_counts(const bs_system *sys, int64_t out[8]);
+
+/* ---- device ---- */
+/* y = H x (accumulate = 0) or y += H x (accumulate = 1).  x, y: device
+ * arrays of dim elements, dtype BS_F32 (FP32 vectors, FP32 accumulation) or
+ * BS_F64 (FP64 vectors and accumulation, FP32 matrix elements promoted).
+ * x and y must not overlap. */
+int bs_apply_h(bs_system *sys, const void *x, void *y, int dtype, int accumulate, void *stream);
+
+typedef struct {
+  int nkeep;        /* eigenpairs wanted */
+  int niter;        /* Lanczos iterations (plain) or restart length (thick) */
+  int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart */
+  int max_restarts; /* thick restart: maximum number of restarts */
+  int dtype;        /* BS_F32: FP32 vectors/matvec, FP64 coefficients; BS_F64: all FP64 */
+  double tol;       /* thick restart: stop when max |E_i - E_i(previous)| < tol (0: run all) */
+  int polish;       /* 1: FP64 Rayleigh quotient / residual polish of the Ritz vectors */
+} bs_lanczos_params;
+
+/* Lanczos from the reference's uniform pivot 1/sqrt(D)
+ * (src/blanczoslib.f90:825-831) with full FP64-accumulated classical
+ * Gram-Schmidt.  alpha/beta receive the coefficients of the last Krylov
+ * sequence (niter entries each, may be NULL); evals receives nkeep Ritz
+ * values; evecs (device, may be NULL) receives nkeep Ritz vectors of dim
+ * elements in dtype, row-major; resid (host, may be NULL) receives the FP64
+ * residual norms ||H v - E v||.  iters (may be NULL) receives the total number
+ * of matvecs. */
+int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
+               double *evals, void *evecs, double *resid, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

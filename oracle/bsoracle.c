@@ -860,101 +860,103 @@ static int fsign(uint64_t mask, int c) {
 
 /* y += H x with FP64 accumulation (x,y of length dim).  Basis states are
  * 0-based.  Slater determinants are a^+ products in ascending combined label
- * order, protons to the left of neutrons. */
+ * order, protons to the left of neutrons.  H is real symmetric, so each row
+ * y[s] is gathered from the states the operators reach from s:
+ * y[s] = sum_t <t|H|s> x[t]; rows are independent (OpenMP over proton SDs). */
 void bso_apply(const bso_t *o, const double *x, double *y) {
   int ns = o->nsps;
   sdmap_t mp[2];
   for (int it = 0; it < 2; it++) map_build(&mp[it], o->mask[it], o->nsd[it]);
   int npx = ns * (ns - 1) / 2;
-  for (int sp = 0; sp < o->nsec[0]; sp++) {
+  int64_t nP = o->nsd[0];
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t ip = 0; ip < nP; ip++) {
+    int sp = o->sdsec[0][ip];
     int sn = o->conj[0][sp];
-    for (int64_t ip = o->secfirst[0][sp]; ip < o->secfirst[0][sp] + o->seccount[0][sp]; ip++) {
-      uint64_t pm = o->mask[0][ip];
-      for (int64_t in = o->secfirst[1][sn]; in < o->secfirst[1][sn] + o->seccount[1][sn]; in++) {
-        uint64_t nm = o->mask[1][in];
-        int64_t s = o->start[0][ip] + o->start[1][in];
-        double xs = x[s];
-        if (xs == 0.0) continue;
-        /* SPE (only when not subsumed into the two-body terms) */
-        if (o->call_spe) {
-          double e = 0.0;
-          for (uint64_t t = pm; t; t &= t - 1) e += o->pspe[o->sorb[__builtin_ctzll(t)]];
-          for (uint64_t t = nm; t; t &= t - 1) e += o->nspe[o->sorb[__builtin_ctzll(t)]];
-          y[s] += e * xs;
-        }
-        /* PP and NN: a+_i a+_j a_l a_k, c(k)>c(l), c(i)>c(j) */
-        for (int it = 0; it < 2; it++) {
-          uint64_t m0 = it == 0 ? pm : nm;
-          const float *v = it == 0 ? o->vpp : o->vnn;
-          if (!v || (it == 0 ? o->Z : o->N) < 2) continue;
-          for (uint64_t tk = m0; tk; tk &= tk - 1) {
-            int k = __builtin_ctzll(tk);
-            for (uint64_t tl = m0 & ((1ULL << k) - 1); tl; tl &= tl - 1) {
-              int l = __builtin_ctzll(tl);
-              int sg = fsign(m0, k);
-              uint64_t m1 = m0 & ~(1ULL << k);
-              sg *= fsign(m1, l);
-              uint64_t m2 = m1 & ~(1ULL << l);
-              int dkl = k * (k - 1) / 2 + l;
-              int mkl = o->sm2[k] + o->sm2[l];
-              for (int i = 1; i < ns; i++) {
-                if (m2 >> i & 1) continue;
-                for (int j = 0; j < i; j++) {
-                  if (m2 >> j & 1) continue;
-                  if (o->sm2[i] + o->sm2[j] != mkl) continue;
-                  float me = v[(size_t)(i * (i - 1) / 2 + j) * npx + dkl];
-                  if (me == 0.0f) continue;
-                  int s2 = sg * fsign(m2, j);
-                  uint64_t m3 = m2 | (1ULL << j);
-                  s2 *= fsign(m3, i);
-                  uint64_t m4 = m3 | (1ULL << i);
-                  int64_t t;
-                  if (it == 0) {
-                    int64_t ipf = map_get(&mp[0], m4);
-                    t = o->start[0][ipf] + o->start[1][in];
-                  } else {
-                    int64_t inf = map_get(&mp[1], m4);
-                    t = o->start[0][ip] + o->start[1][inf];
-                  }
-                  y[t] += (double)me * (double)s2 * xs;
+    uint64_t pm = o->mask[0][ip];
+    for (int64_t in = o->secfirst[1][sn]; in < o->secfirst[1][sn] + o->seccount[1][sn]; in++) {
+      uint64_t nm = o->mask[1][in];
+      int64_t s = o->start[0][ip] + o->start[1][in];
+      double acc = 0.0;
+      if (o->call_spe) {
+        double e = 0.0;
+        for (uint64_t t = pm; t; t &= t - 1) e += o->pspe[o->sorb[__builtin_ctzll(t)]];
+        for (uint64_t t = nm; t; t &= t - 1) e += o->nspe[o->sorb[__builtin_ctzll(t)]];
+        acc += e * x[s];
+      }
+      /* PP and NN: a+_i a+_j a_l a_k, c(k)>c(l), c(i)>c(j) */
+      for (int it = 0; it < 2; it++) {
+        uint64_t m0 = it == 0 ? pm : nm;
+        const float *v = it == 0 ? o->vpp : o->vnn;
+        if (!v || (it == 0 ? o->Z : o->N) < 2) continue;
+        for (uint64_t tk = m0; tk; tk &= tk - 1) {
+          int k = __builtin_ctzll(tk);
+          for (uint64_t tl = m0 & ((1ULL << k) - 1); tl; tl &= tl - 1) {
+            int l = __builtin_ctzll(tl);
+            int sg = fsign(m0, k);
+            uint64_t m1 = m0 & ~(1ULL << k);
+            sg *= fsign(m1, l);
+            uint64_t m2 = m1 & ~(1ULL << l);
+            int dkl = k * (k - 1) / 2 + l;
+            int mkl = o->sm2[k] + o->sm2[l];
+            for (int i = 1; i < ns; i++) {
+              if (m2 >> i & 1) continue;
+              for (int j = 0; j < i; j++) {
+                if (m2 >> j & 1) continue;
+                if (o->sm2[i] + o->sm2[j] != mkl) continue;
+                float me = v[(size_t)(i * (i - 1) / 2 + j) * npx + dkl];
+                if (me == 0.0f) continue;
+                int s2 = sg * fsign(m2, j);
+                uint64_t m3 = m2 | (1ULL << j);
+                s2 *= fsign(m3, i);
+                uint64_t m4 = m3 | (1ULL << i);
+                int64_t t;
+                if (it == 0) {
+                  int64_t ipf = map_get(&mp[0], m4);
+                  t = o->start[0][ipf] + o->start[1][in];
+                } else {
+                  int64_t inf = map_get(&mp[1], m4);
+                  t = o->start[0][ip] + o->start[1][inf];
                 }
-              }
-            }
-          }
-        }
-        /* PN: a+_i b+_j b_l a_k */
-        if (o->Z > 0 && o->N > 0) {
-          for (uint64_t tk = pm; tk; tk &= tk - 1) {
-            int k = __builtin_ctzll(tk);
-            int sk = fsign(pm, k);
-            uint64_t pm1 = pm & ~(1ULL << k);
-            for (uint64_t tl = nm; tl; tl &= tl - 1) {
-              int l = __builtin_ctzll(tl);
-              int sl = fsign(nm, l);
-              uint64_t nm1 = nm & ~(1ULL << l);
-              int mkl = o->sm2[k] + o->sm2[l];
-              for (int i = 0; i < ns; i++) {
-                if (pm1 >> i & 1) continue;
-                int si = fsign(pm1, i);
-                uint64_t pm2 = pm1 | (1ULL << i);
-                int64_t ipf = -1;
-                for (int j = 0; j < ns; j++) {
-                  if (nm1 >> j & 1) continue;
-                  if (o->sm2[i] + o->sm2[j] != mkl) continue;
-                  float me = o->vpn[(size_t)(i * ns + j) * (ns * ns) + (k * ns + l)];
-                  if (me == 0.0f) continue;
-                  int sj = fsign(nm1, j);
-                  uint64_t nm2 = nm1 | (1ULL << j);
-                  if (ipf < 0) ipf = map_get(&mp[0], pm2);
-                  int64_t inf = map_get(&mp[1], nm2);
-                  int64_t t = o->start[0][ipf] + o->start[1][inf];
-                  y[t] += (double)me * (double)(sk * sl * si * sj) * xs;
-                }
+                acc += (double)me * (double)s2 * x[t];
               }
             }
           }
         }
       }
+      /* PN: a+_i b+_j b_l a_k */
+      if (o->Z > 0 && o->N > 0) {
+        for (uint64_t tk = pm; tk; tk &= tk - 1) {
+          int k = __builtin_ctzll(tk);
+          int sk = fsign(pm, k);
+          uint64_t pm1 = pm & ~(1ULL << k);
+          for (uint64_t tl = nm; tl; tl &= tl - 1) {
+            int l = __builtin_ctzll(tl);
+            int sl = fsign(nm, l);
+            uint64_t nm1 = nm & ~(1ULL << l);
+            int mkl = o->sm2[k] + o->sm2[l];
+            for (int i = 0; i < ns; i++) {
+              if (pm1 >> i & 1) continue;
+              int si = fsign(pm1, i);
+              uint64_t pm2 = pm1 | (1ULL << i);
+              int64_t ipf = -1;
+              for (int j = 0; j < ns; j++) {
+                if (nm1 >> j & 1) continue;
+                if (o->sm2[i] + o->sm2[j] != mkl) continue;
+                float me = o->vpn[(size_t)(i * ns + j) * (ns * ns) + (k * ns + l)];
+                if (me == 0.0f) continue;
+                int sj = fsign(nm1, j);
+                uint64_t nm2 = nm1 | (1ULL << j);
+                if (ipf < 0) ipf = map_get(&mp[0], pm2);
+                int64_t inf = map_get(&mp[1], nm2);
+                int64_t t = o->start[0][ipf] + o->start[1][inf];
+                acc += (double)me * (double)(sk * sl * si * sj) * x[t];
+              }
+            }
+          }
+        }
+      }
+      y[s] += acc;
     }
   }
   for (int it = 0; it < 2; it++) { free(mp[it].keys); free(mp[it].vals); }

@@ -1,0 +1,98 @@
+"""HIP path against the oracle on the GPU (through the C ABI)."""
+import numpy as np
+import pytest
+
+from golden_cases import SMALL, config_fixture, paths, small_fixture
+from lanczos_ref import lanczos
+from oracle_api import Oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _sys(case, d):
+    from bigstickpublick_amd import System
+    return System(*paths(case, d), case.Z, case.N, case.jz2, upload=True)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_apply_small(name, casedir):
+    case, d, fx = small_fixture(name, casedir)
+    s = _sys(case, d)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    rng = np.random.default_rng(1)
+    for trial in range(2):
+        x = rng.standard_normal(s.dim)
+        want = o.apply(x)
+        y64 = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
+        assert _rel(y64, want) < 1e-12
+        y32 = s.apply(torch.tensor(x, device="cuda", dtype=torch.float32)).cpu().numpy()
+        assert _rel(y32, want) < 2e-6 * 8
+
+
+def test_accumulate_semantics(casedir):
+    """y += H x (the reference's vec2 = vec2 + H vec1) versus y = H x."""
+    case, d, fx = small_fixture("sd3p1n_mm2", casedir)
+    s = _sys(case, d)
+    rng = np.random.default_rng(2)
+    x = torch.tensor(rng.standard_normal(s.dim), device="cuda")
+    y0 = torch.tensor(rng.standard_normal(s.dim), device="cuda")
+    y = y0.clone()
+    s.apply(x, y, accumulate=True)
+    hx = s.apply(x)
+    assert torch.allclose(y, y0 + hx, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 3])
+def test_apply_configs(cfg, casedir):
+    case, d, fx = config_fixture(cfg, casedir)
+    s = _sys(case, d)
+    assert s.dim == int(fx["dim"])
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    x = np.random.default_rng(3).standard_normal(s.dim)
+    want = o.apply(x)
+    y64 = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
+    assert _rel(y64, want) < 1e-12
+    y32 = s.apply(torch.tensor(x, device="cuda", dtype=torch.float32)).cpu().numpy()
+    assert _rel(y32, want) < 2e-5
+
+
+def test_lanczos_fp64_config0(casedir):
+    """FP64 GPU Lanczos: alpha/beta equal to the FP64 oracle's from the same
+    pivot; energies equal to the reference's printed ones (f10.5)."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    a, b, e, _ = lanczos(o.apply, o.dim, 150, case.nkeep)
+    r = s.lanczos(nkeep=case.nkeep, niter=150, dtype="f64", polish=False)
+    assert np.abs(r.alpha[:60] - a[:60]).max() < 1e-9
+    assert np.abs(r.beta[:60] - b[:60]).max() < 1e-9
+    assert np.abs(r.evals - e).max() < 1e-8
+    assert np.abs(r.evals - fx["res"][:5, 0]).max() < 1.5e-5
+    # reference alpha/beta (FP32 vectors there) agree to FP32 accuracy early on
+    assert np.abs(r.alpha[:20] - fx["alpha"][:20]).max() < 2e-5
+
+
+def test_lanczos_mixed_vs_fp64_config0(casedir):
+    """FP32 vectors + FP64 coefficients + FP64 polish: every eigenvalue within
+    1e-4 MeV of the FP64 path (the stated mixed-precision target)."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    r64 = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True)
+    r32 = s.lanczos(nkeep=5, niter=150, dtype="f32", polish=True)
+    assert np.abs(r32.evals - r64.evals).max() < 1e-4
+    assert np.all(r64.resid < 1e-3)
+
+
+def test_thick_restart_config0(casedir):
+    """Thick-restart Lanczos (keep 10, restart at 40) reaches the plain
+    150-iteration FP64 energies."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    ref = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True)
+    tr = s.lanczos(nkeep=5, niter=40, thick=10, max_restarts=30, dtype="f64", tol=1e-10, polish=True)
+    assert np.abs(tr.evals - ref.evals).max() < 1e-7
