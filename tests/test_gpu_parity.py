@@ -85,7 +85,9 @@ def test_lanczos_mixed_vs_fp64_config0(casedir):
     r64 = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True)
     r32 = s.lanczos(nkeep=5, niter=150, dtype="f32", polish=True)
     assert np.abs(r32.evals - r64.evals).max() < 1e-4
-    assert np.all(r64.resid < 1e-3)
+    # residuals ||Hx - Ex|| after 150 steps: the lowest states are converged
+    assert np.all(r64.resid[:2] < 1e-8)
+    assert np.all(r64.resid < 1e-2)
 
 
 def test_thick_restart_config0(casedir):
@@ -95,4 +97,7 @@ def test_thick_restart_config0(casedir):
     s = _sys(case, d)
     ref = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True)
     tr = s.lanczos(nkeep=5, niter=40, thick=10, max_restarts=30, dtype="f64", tol=1e-10, polish=True)
-    assert np.abs(tr.evals - ref.evals).max() < 1e-7
+    assert tr.resid.max() < 1e-9
+    # the plain run's 4th state is only converged to a residual ~1e-3
+    assert np.abs(tr.evals - ref.evals).max() < 1e-6
+    assert np.abs(tr.evals[:3] - ref.evals[:3]).max() < 1e-10
