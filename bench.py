@@ -105,6 +105,9 @@ def main():
     s.upload()
     torch.cuda.synchronize()
     t_upload = time.time() - t0
+    t0 = time.time()
+    variant = s.autotune("f32")
+    t_tune = time.time() - t0
     D = s.dim
     ops = s.op_counts()
     n_ops = ops["ref_total"]
@@ -153,7 +156,9 @@ def main():
         "n_ops_ref": n_ops,
         "updates_per_s_engine": (ops["spe"] + ops["pp"] + ops["nn"] + ops["pn"]) / (kern_ms * 1e-3),
         "b_alg_bytes": b_alg,
-        "setup_s": {"host_tables": t_host, "upload": t_upload},
+        "setup_s": {"host_tables": t_host, "upload": t_upload, "autotune": t_tune},
+        "kernel_variant": {"rows_per_tile": System.variants()[variant][0],
+                           "staged_rows": System.variants()[variant][1]},
     }
     if not a.no_cpu:
         try:

@@ -52,120 +52,213 @@ struct DevTables {
   int64_t dim;
   int nP, nN, ndelta;
   int call_spe;
+  int lmax;                  // longest neutron row
+  int wcols;                 // LDS row stride of the pn element block (max class size + 1)
   const int64_t* pstart;     // [nP] row offset of proton SD
-  const int32_t* rowlen;     // [nP] neutron row length
-  const int32_t* nfirst;     // [nP] first neutron SD (global) of the row's sector
   const float* pspe_sd;      // [nP]
   const float* nspe_sd;      // [nN]
+  // tiles: up to R consecutive proton rows of one sector
+  const int32_t* tile_f0;
+  const int32_t* tile_nr;
+  const int32_t* psec;       // [nP] proton sector of each SD
+  const int32_t* pconj;      // [nsecP] conjugate neutron sector
+  const int32_t* nlen;       // [nsecN] SDs in neutron sector
+  const int32_t* nfirst;     // [nsecN] first SD of neutron sector
+  const int32_t* srcsec;     // [nsecP * ndelta] proton source sector for class k (-1: none)
   // PP merged CSR by target proton SD
   const int64_t* pp_ptr;
   const int32_t* pp_src;
   const float* pp_val32;
   const double* pp_val64;
-  // NN merged CSR by target neutron SD; src is local (within sector)
-  const int64_t* nn_ptr;
-  const int32_t* nn_srcl;
+  // NN merged, ELL per neutron sector: slot e of column n at nn_base[c] + e*L_c + n
+  const int64_t* nn_base;
+  const int32_t* nn_emax;
+  const int32_t* nn_src;     // local source column
   const float* nn_val32;
   const double* nn_val64;
   // proton one-body: class pointers per row [nP][ndelta+1] (absolute), entries
   const int32_t* pcls;
   const int32_t* p1_src;     // proton source SD (global)
   const int16_t* p1_meta;    // pp_local | (phase<0) << 8
-  // neutron one-body: class pointers per neutron SD [nN][ndelta+1], packed entries
-  const int32_t* ncls;
-  const uint32_t* n1_pk;     // src_local (16b) | nn_local (8b) << 16 | (phase<0) << 24
-  // pn element blocks
+  // neutron one-body, ELL per (target neutron sector c, class k):
+  // slot e of column n at n1_base[c*ndelta+k] + e*L_c + n; packed
+  // src_local (16b) | nn_local (8b) << 16 | (phase<0) << 24; padding slots
+  // point at the zero column nn_local = size of the class
+  const int64_t* n1_base;
+  const int32_t* n1_emax;
+  const uint32_t* n1_pk;
+  // pn element blocks: class k block is [size_k][size_-k] at woff[k]
   const float* W;
-  const int32_t* woff;       // [ndelta]
-  const int32_t* wstride;    // [ndelta] = size of class -k
-  const int32_t* negk;       // [ndelta]
+  const int32_t* woff;
+  const int32_t* wrows;      // size_k
+  const int32_t* wstride;    // size_-k
+  const int32_t* negk;
 };
 
 template <typename T> __device__ __forceinline__ T vload(const float* a, const double* b, int64_t i);
 template <> __device__ __forceinline__ float vload<float>(const float* a, const double*, int64_t i) { return a[i]; }
 template <> __device__ __forceinline__ double vload<double>(const float*, const double* b, int64_t i) { return b[i]; }
 
-// One workgroup per (proton SD row f, column chunk).  CPT columns per thread.
-template <typename T, int CPT>
-__global__ __launch_bounds__(kBlock) void k_apply_rows(DevTables tb, const T* __restrict__ x,
+// One workgroup per tile of up to R proton rows of one sector; BLOCK threads,
+// CPT columns per thread (column = tid + q*BLOCK).  Dynamic LDS holds G staged
+// x rows ([G][lmax]) followed by the pn element block of the current class
+// ([size_k][wcols], last column zero).
+template <typename T, int BLOCK, int CPT, int R, int G>
+__global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __restrict__ x,
                                                        T* __restrict__ y, int accumulate) {
-  const int f = blockIdx.x;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* xs = reinterpret_cast<T*>(smem);
+  float* ws = reinterpret_cast<float*>(smem + (size_t)G * tb.lmax * sizeof(T));
   const int tid = threadIdx.x;
-  const int64_t base_f = tb.pstart[f];
-  const int L = tb.rowlen[f];
-  const int nf0 = tb.nfirst[f];
-  const int c0 = blockIdx.y * (kBlock * CPT);
-  T acc[CPT];
+  const int f0 = tb.tile_f0[blockIdx.x];
+  const int nr = tb.tile_nr[blockIdx.x];
+  const int sp = tb.psec[f0];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  const int nf0 = tb.nfirst[cn];
+  const int lm = tb.lmax;
+  T acc[R][CPT];
 #pragma unroll
-  for (int q = 0; q < CPT; q++) acc[q] = T(0);
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int q = 0; q < CPT; q++) acc[r][q] = T(0);
 
+  // ---- SPE (only when the single-particle energies are not subsumed)
   if (tb.call_spe) {
-    const T ep = (T)tb.pspe_sd[f];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < nr) {
+        const T ep = (T)tb.pspe_sd[f0 + r];
+        const int64_t bf = tb.pstart[f0 + r];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int c = tid + q * BLOCK;
+          if (c < L) acc[r][q] += (ep + (T)tb.nspe_sd[nf0 + c]) * x[bf + c];
+        }
+      }
+    }
+  }
+  // ---- PP: row AXPYs with coalesced reads
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (r < nr) {
+      const int f = f0 + r;
+      for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
+        const int64_t bi = tb.pstart[tb.pp_src[e]];
+        const T v = vload<T>(tb.pp_val32, tb.pp_val64, e);
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int c = tid + q * BLOCK;
+          if (c < L) acc[r][q] += v * x[bi + c];
+        }
+      }
+    }
+  }
+  // ---- NN: stage the tile's own rows, gather in LDS, entries shared by rows
+  for (int r = 0; r < nr; r++) {
+    const int64_t bf = tb.pstart[f0 + r];
+    for (int c = tid; c < L; c += BLOCK) xs[r * lm + c] = x[bf + c];
+  }
+  __syncthreads();
+  {
+    const int64_t base = tb.nn_base[cn];
+    const int E = tb.nn_emax[cn];
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-      const int c = c0 + tid + q * kBlock;
-      if (c < L) acc[q] += (ep + (T)tb.nspe_sd[nf0 + c]) * x[base_f + c];
-    }
-  }
-  // PP: row AXPYs
-  for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
-    const int64_t bi = tb.pstart[tb.pp_src[e]];
-    const T v = vload<T>(tb.pp_val32, tb.pp_val64, e);
+      const int c = tid + q * BLOCK;
+      if (c < L) {
+        for (int e = 0; e < E; e++) {
+          const int64_t o = base + (int64_t)e * L + c;
+          const int sl = tb.nn_src[o];
+          const T v = vload<T>(tb.nn_val32, tb.nn_val64, o);
 #pragma unroll
-    for (int q = 0; q < CPT; q++) {
-      const int c = c0 + tid + q * kBlock;
-      if (c < L) acc[q] += v * x[bi + c];
+          for (int r = 0; r < R; r++)
+            if (r < nr) acc[r][q] += v * xs[r * lm + sl];
+        }
+      }
     }
   }
-  // NN: gathers within row f
+  // ---- PN: per Delta class, batches of G staged source rows
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int k = 0; k < nd; k++) {
+    int p0[R], p1[R];
+    int any = 0;
 #pragma unroll
-  for (int q = 0; q < CPT; q++) {
-    const int c = c0 + tid + q * kBlock;
-    if (c < L) {
-      const int n = nf0 + c;
-      T s = T(0);
-      for (int64_t e = tb.nn_ptr[n]; e < tb.nn_ptr[n + 1]; e++)
-        s += vload<T>(tb.nn_val32, tb.nn_val64, e) * x[base_f + tb.nn_srcl[e]];
-      acc[q] += s;
+    for (int r = 0; r < R; r++) {
+      p0[r] = p1[r] = 0;
+      if (r < nr) {
+        p0[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k];
+        p1[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k + 1];
+        any |= (p1[r] > p0[r]);
+      }
     }
-  }
-  // PN
-  const int nd1 = tb.ndelta + 1;
-  for (int k = 0; k < tb.ndelta; k++) {
-    const int p0 = tb.pcls[(int64_t)f * nd1 + k], p1 = tb.pcls[(int64_t)f * nd1 + k + 1];
-    if (p0 == p1) continue;
+    if (!any) continue;
     const int kn = tb.negk[k];
+    const int si = tb.srcsec[sp * nd + k];
+    const int Li = tb.nlen[tb.pconj[si]];
+    const int wr = tb.wrows[k], wsd = tb.wstride[k], wc = tb.wcols;
     const float* Wk = tb.W + tb.woff[k];
-    const int ws = tb.wstride[k];
-    for (int pj = p0; pj < p1; pj++) {
-      const int64_t bi = tb.pstart[tb.p1_src[pj]];
-      const int meta = tb.p1_meta[pj];
-      const float* Wr = Wk + (meta & 0xff) * ws;
-      const T php = (meta >> 8) ? T(-1) : T(1);
+    __syncthreads();
+    for (int i = tid; i < wr * wc; i += BLOCK) {
+      const int a = i / wc, b = i - a * wc;
+      ws[i] = b < wsd ? Wk[a * wsd + b] : 0.0f;
+    }
+    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
+    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
 #pragma unroll
-      for (int q = 0; q < CPT; q++) {
-        const int c = c0 + tid + q * kBlock;
-        if (c < L) {
-          const int64_t n = nf0 + c;
-          const int e0 = tb.ncls[n * nd1 + kn], e1 = tb.ncls[n * nd1 + kn + 1];
-          T s = T(0);
-          for (int e = e0; e < e1; e++) {
-            const uint32_t pk = tb.n1_pk[e];
-            const T w = (T)Wr[(pk >> 16) & 0xff];
-            const T xv = x[bi + (pk & 0xffff)];
-            s += ((pk >> 24) ? -w : w) * xv;
+    for (int r = 0; r < R; r++) {
+      for (int b0 = p0[r]; b0 < p1[r]; b0 += G) {
+        const int nb = min(G, p1[r] - b0);
+        __syncthreads();
+        for (int g = 0; g < nb; g++) {
+          const int64_t bi = tb.pstart[tb.p1_src[b0 + g]];
+          for (int c = tid; c < Li; c += BLOCK) xs[g * lm + c] = x[bi + c];
+        }
+        __syncthreads();
+        int wo[G];
+        int neg[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          const int meta = g < nb ? (int)tb.p1_meta[b0 + g] : 0;
+          wo[g] = (meta & 0xff) * wc;
+          neg[g] = (meta >> 8) & 1;
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int c = tid + q * BLOCK;
+          if (c < L) {
+            T a = T(0);
+            for (int e = 0; e < E; e++) {
+              const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
+              const int sl = pk & 0xffff;
+              const int nl = (pk >> 16) & 0xff;
+              const int sn = (pk >> 24) & 1;
+#pragma unroll
+              for (int g = 0; g < G; g++) {
+                if (g < nb) {
+                  const T t = (T)ws[wo[g] + nl] * xs[g * lm + sl];
+                  a += (neg[g] ^ sn) ? -t : t;
+                }
+              }
+            }
+            acc[r][q] += a;
           }
-          acc[q] += php * s;
         }
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < CPT; q++) {
-    const int c = c0 + tid + q * kBlock;
-    if (c < L) {
-      if (accumulate) y[base_f + c] += acc[q];
-      else y[base_f + c] = acc[q];
+  for (int r = 0; r < R; r++) {
+    if (r < nr) {
+      const int64_t bf = tb.pstart[f0 + r];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * BLOCK;
+        if (c < L) {
+          if (accumulate) y[bf + c] += acc[r][q];
+          else y[bf + c] = acc[r][q];
+        }
+      }
     }
   }
 }
@@ -363,10 +456,14 @@ int grid_for(int64_t D) {
 struct bs_system {
   bs::Tables t;
   bool uploaded = false;
-  int cpt = 1, nchunk = 1;
+  int cpt = 1;
+  int variant[2] = {-1, -1};   // chosen (R, G) variant per dtype (0: f32, 1: f64)
   DevTables dt{};
-  DevBuf<int64_t> pstart, pp_ptr, nn_ptr;
-  DevBuf<int32_t> rowlen, nfirst, pp_src, nn_srcl, pcls, p1_src, ncls, woff, wstride, negk;
+  DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base;
+  DevBuf<int32_t> tile_f0[2], tile_nr[2];
+  int ntiles[2] = {0, 0};
+  DevBuf<int32_t> psec, pconj, nlen, nfirst, srcsec, pp_src, nn_emax, nn_src, pcls, p1_src, n1_emax;
+  DevBuf<int32_t> woff, wrows, wstride, negk;
   DevBuf<int16_t> p1_meta;
   DevBuf<uint32_t> n1_pk;
   DevBuf<float> pspe_sd, nspe_sd, pp_val32, nn_val32, W;
@@ -375,21 +472,33 @@ struct bs_system {
 
 namespace {
 
+constexpr int kTileRows[2] = {2, 4};   // R of the kernel variants
+constexpr int kApplyBlock = 512;
+
 void upload_tables(bs_system* s) {
   const bs::Tables& t = s->t;
   const bs::Species& P = t.sp[0];
   const bs::Species& N = t.sp[1];
   const int nP = (int)P.mask.size(), nN = (int)N.mask.size();
-  const int nd = t.ndelta, ns = t.nsps;
-  std::vector<int64_t> pstart(P.start);
-  std::vector<int32_t> rowlen(nP), nfirst(nP);
-  for (int f = 0; f < nP; f++) {
-    int c = P.conj[P.sdsec[f]];
-    rowlen[f] = (int32_t)N.seccount[c];
-    nfirst[f] = (int32_t)N.secfirst[c];
+  const int nd = t.ndelta;
+  if ((int64_t)P.mask.size() >= INT32_MAX || (int64_t)N.mask.size() >= INT32_MAX)
+    throw std::runtime_error("too many Slater determinants");
+  std::vector<int32_t> psec(P.sdsec.begin(), P.sdsec.end());
+  std::vector<int32_t> pconj(P.conj.begin(), P.conj.end());
+  std::vector<int32_t> nlen(N.nsec), nfirst(N.nsec);
+  int lmax = 1;
+  for (int c = 0; c < N.nsec; c++) {
+    nlen[c] = (int32_t)N.seccount[c];
+    nfirst[c] = (int32_t)N.secfirst[c];
+    lmax = std::max(lmax, nlen[c]);
   }
-  int maxL = 1;
-  for (int f = 0; f < nP; f++) maxL = std::max(maxL, rowlen[f]);
+  if (lmax > 8 * kApplyBlock) throw std::runtime_error("neutron sector longer than 4096 SDs is not supported");
+  if (lmax > 0xffff) throw std::runtime_error("neutron sector too long for packed jumps");
+  std::vector<int32_t> srcsec((size_t)P.nsec * nd, -1);
+  for (int sp = 0; sp < P.nsec; sp++)
+    for (int k = 0; k < nd; k++)
+      for (int q = 0; q < P.nsec; q++)
+        if (P.secjz[q] == P.secjz[sp] - t.delta_of_class[k]) srcsec[(size_t)sp * nd + k] = q;
   // per-SD single-particle energies (used only when not subsumed)
   std::vector<float> pspe_sd(nP, 0.0f), nspe_sd(nN, 0.0f);
   for (int it = 0; it < 2; it++) {
@@ -402,27 +511,49 @@ void upload_tables(bs_system* s) {
       out[i] = acc;
     }
   }
-  // PP / NN merged two-body
-  std::vector<float> ppv32(t.tb[0].val.size()), nnv32(t.tb[1].val.size());
+  // PP merged two-body (CSR by target)
+  std::vector<float> ppv32(t.tb[0].val.size());
   for (size_t i = 0; i < ppv32.size(); i++) ppv32[i] = (float)t.tb[0].val[i];
-  for (size_t i = 0; i < nnv32.size(); i++) nnv32[i] = (float)t.tb[1].val[i];
-  std::vector<int32_t> nn_srcl(t.tb[1].src.size());
-  for (int n = 0; n < nN; n++)
-    for (int64_t e = t.tb[1].rowptr[n]; e < t.tb[1].rowptr[n + 1]; e++) {
-      int32_t src = t.tb[1].src[e];
-      if (N.sdsec[src] != N.sdsec[n]) throw std::runtime_error("NN jump leaves its sector");
-      nn_srcl[e] = (int32_t)(src - N.secfirst[N.sdsec[n]]);
-    }
   std::vector<int32_t> pp_src(t.tb[0].src.begin(), t.tb[0].src.end());
   for (int f = 0; f < nP; f++)
     for (int64_t e = t.tb[0].rowptr[f]; e < t.tb[0].rowptr[f + 1]; e++)
       if (P.sdsec[t.tb[0].src[e]] != P.sdsec[f]) throw std::runtime_error("PP jump leaves its sector");
+  // NN merged two-body in ELL per neutron sector
+  std::vector<int64_t> nn_base(N.nsec);
+  std::vector<int32_t> nn_emax(N.nsec);
+  int64_t tot = 0;
+  for (int c = 0; c < N.nsec; c++) {
+    int em = 0;
+    for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++)
+      em = std::max<int>(em, (int)(t.tb[1].rowptr[n + 1] - t.tb[1].rowptr[n]));
+    nn_base[c] = tot;
+    nn_emax[c] = em;
+    tot += (int64_t)em * N.seccount[c];
+  }
+  std::vector<int32_t> nn_src(std::max<int64_t>(tot, 1), 0);
+  std::vector<float> nnv32(std::max<int64_t>(tot, 1), 0.0f);
+  std::vector<double> nnv64(std::max<int64_t>(tot, 1), 0.0);
+  for (int c = 0; c < N.nsec; c++) {
+    const int L = nlen[c];
+    for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++) {
+      const int64_t col = n - N.secfirst[c];
+      int e = 0;
+      for (int64_t q = t.tb[1].rowptr[n]; q < t.tb[1].rowptr[n + 1]; q++, e++) {
+        int32_t src = t.tb[1].src[q];
+        if (N.sdsec[src] != c) throw std::runtime_error("NN jump leaves its sector");
+        const int64_t o = nn_base[c] + (int64_t)e * L + col;
+        nn_src[o] = (int32_t)(src - N.secfirst[c]);
+        nnv64[o] = t.tb[1].val[q];
+        nnv32[o] = (float)t.tb[1].val[q];
+      }
+    }
+  }
   // proton one-body with class pointers
   const bs::OneBody& O0 = t.ob[0];
+  if (O0.src.size() >= (size_t)INT32_MAX) throw std::runtime_error("too many proton one-body jumps");
   std::vector<int32_t> pcls((size_t)nP * (nd + 1));
   std::vector<int32_t> p1_src(O0.src.size());
   std::vector<int16_t> p1_meta(O0.src.size());
-  if (O0.src.size() >= (size_t)INT32_MAX) throw std::runtime_error("too many proton one-body jumps");
   for (int f = 0; f < nP; f++) {
     int64_t e = O0.rowptr[f];
     for (int k = 0; k <= nd; k++) {
@@ -431,97 +562,121 @@ void upload_tables(bs_system* s) {
     }
     pcls[(size_t)f * (nd + 1) + nd] = (int32_t)O0.rowptr[f + 1];
     for (int64_t q = O0.rowptr[f]; q < O0.rowptr[f + 1]; q++) {
-      p1_src[q] = O0.src[q];
-      int pl = t.local_of_op[O0.op[q]];
+      const int k = t.class_of_op[O0.op[q]];
+      const int i = O0.src[q];
+      if (srcsec[(size_t)P.sdsec[f] * nd + k] != P.sdsec[i]) throw std::runtime_error("proton jump source in the wrong sector");
+      p1_src[q] = i;
+      const int pl = t.local_of_op[O0.op[q]];
       if (pl > 255) throw std::runtime_error("operator class too large");
       p1_meta[q] = (int16_t)(pl | (O0.phase[q] < 0 ? 256 : 0));
     }
   }
-  // neutron one-body packed entries, source index local to the source sector
+  // neutron one-body: ELL per (target sector, class)
   const bs::OneBody& O1 = t.ob[1];
-  std::vector<int32_t> ncls((size_t)nN * (nd + 1));
-  std::vector<uint32_t> n1_pk(O1.src.size());
-  if (O1.src.size() >= (size_t)INT32_MAX) throw std::runtime_error("too many neutron one-body jumps");
-  for (int n = 0; n < nN; n++) {
-    int64_t e = O1.rowptr[n];
-    for (int k = 0; k <= nd; k++) {
-      while (e < O1.rowptr[n + 1] && t.class_of_op[O1.op[e]] < k) e++;
-      ncls[(size_t)n * (nd + 1) + k] = (int32_t)e;
+  std::vector<int64_t> n1_base((size_t)N.nsec * nd);
+  std::vector<int32_t> n1_emax((size_t)N.nsec * nd);
+  tot = 0;
+  for (int c = 0; c < N.nsec; c++)
+    for (int k = 0; k < nd; k++) {
+      int em = 0;
+      for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++) {
+        int cnt = 0;
+        for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) cnt += t.class_of_op[O1.op[q]] == k;
+        em = std::max(em, cnt);
+      }
+      n1_base[(size_t)c * nd + k] = tot;
+      n1_emax[(size_t)c * nd + k] = em;
+      tot += (int64_t)em * N.seccount[c];
     }
-    ncls[(size_t)n * (nd + 1) + nd] = (int32_t)O1.rowptr[n + 1];
-    for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) {
-      int32_t src = O1.src[q];
-      int64_t sl = src - N.secfirst[N.sdsec[src]];
-      int nl = t.local_of_op[O1.op[q]];
-      if (sl > 0xffff || nl > 255) throw std::runtime_error("neutron row too long for packed jumps");
-      n1_pk[q] = (uint32_t)sl | ((uint32_t)nl << 16) | (O1.phase[q] < 0 ? (1u << 24) : 0u);
+  std::vector<uint32_t> n1_pk(std::max<int64_t>(tot, 1), 0);
+  for (int c = 0; c < N.nsec; c++) {
+    const int L = nlen[c];
+    for (int k = 0; k < nd; k++) {
+      const uint32_t pad = (uint32_t)t.class_size[k] << 16;  // zero column of the block
+      const int64_t b = n1_base[(size_t)c * nd + k];
+      const int em = n1_emax[(size_t)c * nd + k];
+      for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++) {
+        const int64_t col = n - N.secfirst[c];
+        int e = 0;
+        for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) {
+          if (t.class_of_op[O1.op[q]] != k) continue;
+          const int32_t src = O1.src[q];
+          int want = -1;
+          for (int z = 0; z < N.nsec; z++)
+            if (N.secjz[z] == N.secjz[c] - t.delta_of_class[k]) want = z;
+          if (want < 0 || N.sdsec[src] != want) throw std::runtime_error("neutron jump source in the wrong sector");
+          const int64_t sl = src - N.secfirst[want];
+          const int nl = t.local_of_op[O1.op[q]];
+          if (sl > 0xffff || nl >= 255) throw std::runtime_error("neutron jump does not pack");
+          n1_pk[b + (int64_t)e * L + col] = (uint32_t)sl | ((uint32_t)nl << 16) | (O1.phase[q] < 0 ? (1u << 24) : 0u);
+          e++;
+        }
+        for (; e < em; e++) n1_pk[b + (int64_t)e * L + col] = pad;
+      }
     }
   }
-  std::vector<int32_t> woff(nd), wstride(nd), negk(nd);
+  std::vector<int32_t> woff(nd), wrows(nd), wstride(nd), negk(nd);
   for (int k = 0; k < nd; k++) {
     int kn = (int)(std::lower_bound(t.delta_of_class.begin(), t.delta_of_class.end(), -t.delta_of_class[k]) -
                    t.delta_of_class.begin());
     negk[k] = kn;
     woff[k] = t.class_offset[k];
+    wrows[k] = t.class_size[k];
     wstride[k] = t.class_size[kn];
   }
-  // Host-side validation of everything the kernel dereferences, O(entries):
-  // rows inside the vector; two-body sources in the target's sector; every
-  // one-body source in the sector that Delta(2m) of its class implies, so
-  // that a proton jump of class k and the neutron jumps of class -k always
-  // meet on the source row's columns.
-  auto sec_of_jz = [](const bs::Species& S, int jz) {
-    for (int q = 0; q < S.nsec; q++)
-      if (S.secjz[q] == jz) return q;
-    return -1;
-  };
-  for (int f = 0; f < nP; f++) {
-    if (pstart[f] < 0 || pstart[f] + rowlen[f] > t.dim) throw std::runtime_error("row outside the vector");
-    for (int k = 0; k < nd; k++)
-      for (int q = pcls[(size_t)f * (nd + 1) + k]; q < pcls[(size_t)f * (nd + 1) + k + 1]; q++) {
-        int i = p1_src[q];
-        if (i < 0 || i >= nP) throw std::runtime_error("proton jump source out of range");
-        if (P.secjz[P.sdsec[i]] != P.secjz[P.sdsec[f]] - t.delta_of_class[k])
-          throw std::runtime_error("proton jump source in the wrong sector");
-        if ((p1_meta[q] & 0xff) >= t.class_size[k]) throw std::runtime_error("bad pp index");
-        if (N.secjz[P.conj[P.sdsec[i]]] != N.secjz[P.conj[P.sdsec[f]]] + t.delta_of_class[k])
-          throw std::runtime_error("conjugate sectors inconsistent with Delta");
+  // a proton jump of class k and the neutron jumps of class -k must meet on
+  // the source row's columns: conj(source proton sector) == source neutron sector
+  for (int sp = 0; sp < P.nsec; sp++)
+    for (int k = 0; k < nd; k++) {
+      int si = srcsec[(size_t)sp * nd + k];
+      if (si < 0) continue;
+      if (N.secjz[P.conj[si]] != N.secjz[P.conj[sp]] + t.delta_of_class[k])
+        throw std::runtime_error("conjugate sectors inconsistent with Delta");
+    }
+  for (int f = 0; f < nP; f++)
+    if (P.start[f] < 0 || P.start[f] + nlen[P.conj[P.sdsec[f]]] > t.dim) throw std::runtime_error("row outside the vector");
+  // tiles for each R
+  for (int v = 0; v < 2; v++) {
+    std::vector<int32_t> f0, nr;
+    for (int sp = 0; sp < P.nsec; sp++)
+      for (int64_t f = P.secfirst[sp]; f < P.secfirst[sp] + P.seccount[sp]; f += kTileRows[v]) {
+        f0.push_back((int32_t)f);
+        nr.push_back((int32_t)std::min<int64_t>(kTileRows[v], P.secfirst[sp] + P.seccount[sp] - f));
       }
+    s->tile_f0[v].upload(f0);
+    s->tile_nr[v].upload(nr);
+    s->ntiles[v] = (int)f0.size();
   }
-  for (int n = 0; n < nN; n++)
-    for (int k = 0; k < nd; k++)
-      for (int q = ncls[(size_t)n * (nd + 1) + k]; q < ncls[(size_t)n * (nd + 1) + k + 1]; q++) {
-        int src = O1.src[q];
-        int want = sec_of_jz(N, N.secjz[N.sdsec[n]] - t.delta_of_class[k]);
-        if (want < 0 || N.sdsec[src] != want) throw std::runtime_error("neutron jump source in the wrong sector");
-        if ((int)((n1_pk[q] >> 16) & 0xff) >= t.class_size[k]) throw std::runtime_error("bad nn index");
-      }
-  (void)ns;
   int cpt = 1;
-  while (cpt * kBlock < maxL && cpt < 16) cpt *= 2;
+  while (cpt * kApplyBlock < lmax) cpt *= 2;
   s->cpt = cpt;
-  s->nchunk = (maxL + cpt * kBlock - 1) / (cpt * kBlock);
 
-  s->pstart.upload(pstart);
-  s->rowlen.upload(rowlen);
+  s->pstart.upload(P.start);
+  s->psec.upload(psec);
+  s->pconj.upload(pconj);
+  s->nlen.upload(nlen);
   s->nfirst.upload(nfirst);
+  s->srcsec.upload(srcsec);
   s->pspe_sd.upload(pspe_sd);
   s->nspe_sd.upload(nspe_sd);
   s->pp_ptr.upload(t.tb[0].rowptr);
   s->pp_src.upload(pp_src);
   s->pp_val32.upload(ppv32);
   s->pp_val64.upload(t.tb[0].val);
-  s->nn_ptr.upload(t.tb[1].rowptr);
-  s->nn_srcl.upload(nn_srcl);
+  s->nn_base.upload(nn_base);
+  s->nn_emax.upload(nn_emax);
+  s->nn_src.upload(nn_src);
   s->nn_val32.upload(nnv32);
-  s->nn_val64.upload(t.tb[1].val);
+  s->nn_val64.upload(nnv64);
   s->pcls.upload(pcls);
   s->p1_src.upload(p1_src);
   s->p1_meta.upload(p1_meta);
-  s->ncls.upload(ncls);
+  s->n1_base.upload(n1_base);
+  s->n1_emax.upload(n1_emax);
   s->n1_pk.upload(n1_pk);
   s->W.upload(t.wpn);
   s->woff.upload(woff);
+  s->wrows.upload(wrows);
   s->wstride.upload(wstride);
   s->negk.upload(negk);
 
@@ -531,42 +686,135 @@ void upload_tables(bs_system* s) {
   d.nN = nN;
   d.ndelta = nd;
   d.call_spe = t.call_spe ? 1 : 0;
+  d.lmax = lmax;
+  d.wcols = t.max_class + 1;
   d.pstart = s->pstart.p;
-  d.rowlen = s->rowlen.p;
-  d.nfirst = s->nfirst.p;
   d.pspe_sd = s->pspe_sd.p;
   d.nspe_sd = s->nspe_sd.p;
+  d.psec = s->psec.p;
+  d.pconj = s->pconj.p;
+  d.nlen = s->nlen.p;
+  d.nfirst = s->nfirst.p;
+  d.srcsec = s->srcsec.p;
   d.pp_ptr = s->pp_ptr.p;
   d.pp_src = s->pp_src.p;
   d.pp_val32 = s->pp_val32.p;
   d.pp_val64 = s->pp_val64.p;
-  d.nn_ptr = s->nn_ptr.p;
-  d.nn_srcl = s->nn_srcl.p;
+  d.nn_base = s->nn_base.p;
+  d.nn_emax = s->nn_emax.p;
+  d.nn_src = s->nn_src.p;
   d.nn_val32 = s->nn_val32.p;
   d.nn_val64 = s->nn_val64.p;
   d.pcls = s->pcls.p;
   d.p1_src = s->p1_src.p;
   d.p1_meta = s->p1_meta.p;
-  d.ncls = s->ncls.p;
+  d.n1_base = s->n1_base.p;
+  d.n1_emax = s->n1_emax.p;
   d.n1_pk = s->n1_pk.p;
   d.W = s->W.p;
   d.woff = s->woff.p;
+  d.wrows = s->wrows.p;
   d.wstride = s->wstride.p;
   d.negk = s->negk.p;
   s->uploaded = true;
 }
 
+// Kernel variants: (R rows per tile) x (G staged source rows).
+constexpr int kNumVariants = 4;
+constexpr int kVarR[kNumVariants] = {2, 2, 4, 4};
+constexpr int kVarG[kNumVariants] = {4, 8, 4, 8};
+
+template <typename T>
+size_t lds_bytes(const bs_system* s, int v) {
+  return (size_t)kVarG[v] * s->dt.lmax * sizeof(T) + (size_t)s->t.max_class * s->dt.wcols * sizeof(float);
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+template <typename T, int CPT, int R, int G>
+void launch_variant(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st) {
+  const int ri = R == 2 ? 0 : 1;
+  DevTables d = s->dt;
+  d.tile_f0 = s->tile_f0[ri].p;
+  d.tile_nr = s->tile_nr[ri].p;
+  const size_t lds = (size_t)G * d.lmax * sizeof(T) + (size_t)s->t.max_class * d.wcols * sizeof(float);
+  auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kLdsLimit));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, dim3(s->ntiles[ri]), dim3(kApplyBlock), lds, st, d, x, y, accumulate);
+  HIP_OK(hipGetLastError());
+}
+
+template <typename T, int CPT>
+void launch_cpt(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
+  switch (v) {
+    case 0: launch_variant<T, CPT, 2, 4>(s, x, y, accumulate, st); break;
+    case 1: launch_variant<T, CPT, 2, 8>(s, x, y, accumulate, st); break;
+    case 2: launch_variant<T, CPT, 4, 4>(s, x, y, accumulate, st); break;
+    default: launch_variant<T, CPT, 4, 8>(s, x, y, accumulate, st); break;
+  }
+}
+
+template <typename T>
+void launch_apply_v(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
+  if (v < 0 || v >= kNumVariants) throw std::runtime_error("bad kernel variant");
+  if (lds_bytes<T>(s, v) > kLdsLimit) throw std::runtime_error("kernel variant does not fit in LDS");
+  switch (s->cpt) {
+    case 1: launch_cpt<T, 1>(s, v, x, y, accumulate, st); break;
+    case 2: launch_cpt<T, 2>(s, v, x, y, accumulate, st); break;
+    case 4: launch_cpt<T, 4>(s, v, x, y, accumulate, st); break;
+    default: launch_cpt<T, 8>(s, v, x, y, accumulate, st); break;
+  }
+}
+
+// Autotuner: time every variant that fits in LDS on this system (two launches
+// each on a scratch vector) and keep the fastest.
+template <typename T>
+int autotune(bs_system* s, hipStream_t st) {
+  const int di = sizeof(T) == 4 ? 0 : 1;
+  const int64_t D = s->t.dim;
+  DevBuf<T> x, y;
+  x.alloc(D);
+  y.alloc(D);
+  k_fill<T><<<grid_for(D), kBlock, 0, st>>>(x.p, (T)0.5, D);
+  HIP_OK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestv = -1;
+  for (int v = 0; v < kNumVariants; v++) {
+    if (lds_bytes<T>(s, v) > kLdsLimit) continue;
+    launch_apply_v<T>(s, v, x.p, y.p, 0, st);
+    HIP_OK(hipEventRecord(e0, st));
+    launch_apply_v<T>(s, v, x.p, y.p, 0, st);
+    HIP_OK(hipEventRecord(e1, st));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; bestv = v; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (bestv < 0) throw std::runtime_error("no kernel variant fits in LDS");
+  s->variant[di] = bestv;
+  return bestv;
+}
+
 template <typename T>
 void launch_apply(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st) {
-  dim3 grid(s->dt.nP, s->nchunk);
-  switch (s->cpt) {
-    case 1: k_apply_rows<T, 1><<<grid, kBlock, 0, st>>>(s->dt, x, y, accumulate); break;
-    case 2: k_apply_rows<T, 2><<<grid, kBlock, 0, st>>>(s->dt, x, y, accumulate); break;
-    case 4: k_apply_rows<T, 4><<<grid, kBlock, 0, st>>>(s->dt, x, y, accumulate); break;
-    case 8: k_apply_rows<T, 8><<<grid, kBlock, 0, st>>>(s->dt, x, y, accumulate); break;
-    default: k_apply_rows<T, 16><<<grid, kBlock, 0, st>>>(s->dt, x, y, accumulate); break;
+  const int di = sizeof(T) == 4 ? 0 : 1;
+  if (s->variant[di] < 0) {
+    // default: the largest tile that fits, until autotuned
+    int v = kNumVariants - 1;
+    while (v > 0 && lds_bytes<T>(s, v) > kLdsLimit) v--;
+    s->variant[di] = v;
   }
-  HIP_OK(hipGetLastError());
+  launch_apply_v<T>(s, s->variant[di], x, y, accumulate, st);
 }
 
 // Lanczos workspace and loop
@@ -848,6 +1096,57 @@ int bs_apply_h(bs_system* s, const void* x, void* y, int dtype, int accumulate, 
     if (dtype == BS_F32) launch_apply<float>(s, (const float*)x, (float*)y, accumulate ? 1 : 0, st);
     else if (dtype == BS_F64) launch_apply<double>(s, (const double*)x, (double*)y, accumulate ? 1 : 0, st);
     else return set_err("dtype must be BS_F32 or BS_F64");
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_num_variants(int* n) {
+  if (!n) return set_err("null argument");
+  *n = kNumVariants;
+  return 0;
+}
+
+int bs_variant_info(int v, int* rows, int* staged) {
+  if (v < 0 || v >= kNumVariants) return set_err("bad variant");
+  if (rows) *rows = kVarR[v];
+  if (staged) *staged = kVarG[v];
+  return 0;
+}
+
+int bs_set_variant(bs_system* s, int dtype, int v) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (v < -1 || v >= kNumVariants) return set_err("bad variant");
+    if (v >= 0) {
+      size_t need = dtype == BS_F32 ? lds_bytes<float>(s, v) : lds_bytes<double>(s, v);
+      if (need > kLdsLimit) return set_err("variant does not fit in LDS for this system");
+    }
+    s->variant[dtype == BS_F32 ? 0 : 1] = v;
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_get_variant(const bs_system* s, int dtype, int* v) {
+  if (!s || !v || (dtype != BS_F32 && dtype != BS_F64)) return set_err("bad argument");
+  *v = s->variant[dtype == BS_F32 ? 0 : 1];
+  return 0;
+}
+
+int bs_autotune(bs_system* s, int dtype, int* v) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    int r;
+    if (dtype == BS_F32) r = autotune<float>(s, nullptr);
+    else if (dtype == BS_F64) r = autotune<double>(s, nullptr);
+    else return set_err("dtype must be BS_F32 or BS_F64");
+    if (v) *v = r;
     return 0;
   } catch (const std::exception& e) {
     return set_err(e.what());

@@ -45,6 +45,11 @@ def load_library():
     L.bs_me_table.argtypes = [vp, i32, ctypes.POINTER(i32), vp]
     L.bs_op_counts.argtypes = [vp, vp]
     L.bs_apply_h.argtypes = [vp, vp, vp, i32, i32, vp]
+    L.bs_num_variants.argtypes = [ctypes.POINTER(i32)]
+    L.bs_variant_info.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.bs_set_variant.argtypes = [vp, i32, i32]
+    L.bs_get_variant.argtypes = [vp, i32, ctypes.POINTER(i32)]
+    L.bs_autotune.argtypes = [vp, i32, ctypes.POINTER(i32)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     _lib = L
@@ -137,6 +142,39 @@ class System:
                                          ctypes.c_void_p(y.data_ptr()), dt, int(accumulate),
                                          ctypes.c_void_p(st)))
         return y
+
+    @staticmethod
+    def variants():
+        L = load_library()
+        n = ctypes.c_int()
+        _check(L.bs_num_variants(ctypes.byref(n)))
+        out = []
+        for v in range(n.value):
+            r, g = ctypes.c_int(), ctypes.c_int()
+            _check(L.bs_variant_info(v, ctypes.byref(r), ctypes.byref(g)))
+            out.append((r.value, g.value))
+        return out
+
+    def set_variant(self, dtype, v):
+        """Force a kernel variant for dtype ('f32'/'f64'); returns False if it
+        does not fit in LDS for this system."""
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        rc = load_library().bs_set_variant(self.h, dt, v)
+        return rc == 0
+
+    def get_variant(self, dtype):
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        v = ctypes.c_int()
+        _check(load_library().bs_get_variant(self.h, dt, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, dtype="f32"):
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        v = ctypes.c_int()
+        _check(load_library().bs_autotune(self.h, dt, ctypes.byref(v)))
+        return v.value
 
     def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
                 want_vectors=False):

@@ -85,6 +85,16 @@ int bs_op_counts(const bs_system *sys, int64_t out[8]);
  * x and y must not overlap. */
 int bs_apply_h(bs_system *sys, const void *x, void *y, int dtype, int accumulate, void *stream);
 
+/* Kernel variants of the H x kernel: a tile of `rows` proton rows per
+ * workgroup and `staged` source rows held in LDS at a time.  bs_autotune
+ * times every variant that fits in LDS on this system and keeps the fastest
+ * for that dtype; bs_set_variant forces one (-1 = default choice). */
+int bs_num_variants(int *n);
+int bs_variant_info(int variant, int *rows, int *staged);
+int bs_set_variant(bs_system *sys, int dtype, int variant);
+int bs_get_variant(const bs_system *sys, int dtype, int *variant);
+int bs_autotune(bs_system *sys, int dtype, int *variant);
+
 typedef struct {
   int nkeep;        /* eigenpairs wanted */
   int niter;        /* Lanczos iterations (plain) or restart length (thick) */

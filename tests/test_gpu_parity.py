@@ -25,13 +25,15 @@ def test_apply_small(name, casedir):
     s = _sys(case, d)
     o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
     rng = np.random.default_rng(1)
-    for trial in range(2):
-        x = rng.standard_normal(s.dim)
-        want = o.apply(x)
-        y64 = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
-        assert _rel(y64, want) < 1e-12
-        y32 = s.apply(torch.tensor(x, device="cuda", dtype=torch.float32)).cpu().numpy()
-        assert _rel(y32, want) < 2e-6 * 8
+    x = rng.standard_normal(s.dim)
+    want = o.apply(x)
+    for v in range(len(s.variants())):
+        if s.set_variant("f64", v):
+            y64 = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
+            assert _rel(y64, want) < 1e-12, v
+        if s.set_variant("f32", v):
+            y32 = s.apply(torch.tensor(x, device="cuda", dtype=torch.float32)).cpu().numpy()
+            assert _rel(y32, want) < 2e-6 * 8, v
 
 
 def test_accumulate_semantics(casedir):
@@ -55,10 +57,17 @@ def test_apply_configs(cfg, casedir):
     o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
     x = np.random.default_rng(3).standard_normal(s.dim)
     want = o.apply(x)
-    y64 = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
-    assert _rel(y64, want) < 1e-12
-    y32 = s.apply(torch.tensor(x, device="cuda", dtype=torch.float32)).cpu().numpy()
-    assert _rel(y32, want) < 2e-5
+    xd = torch.tensor(x, device="cuda")
+    xf = xd.float()
+    for v in range(len(s.variants())):
+        if s.set_variant("f64", v):
+            assert _rel(s.apply(xd).cpu().numpy(), want) < 1e-12, v
+        if s.set_variant("f32", v):
+            assert _rel(s.apply(xf).cpu().numpy(), want) < 2e-5, v
+    # the autotuner picks one of the variants and keeps it
+    v = s.autotune("f32")
+    assert s.get_variant("f32") == v
+    assert _rel(s.apply(xf).cpu().numpy(), want) < 2e-5
 
 
 def test_lanczos_fp64_config0(casedir):
