@@ -4,7 +4,7 @@ Workload: pf valence space (0f7/2 1p3/2 0f5/2 1p1/2), 6 protons + 6 neutrons,
 M = 0, D = 109,954,620, seeded Gaussian two-body ensemble; FP32 vectors and
 FP32 matrix elements.  One step = one full H x over the resident vector.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--solve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 Prints one JSON line.  value = H x per second on the whole job (one GPU).
 roofline.achieved = B_alg / (mean kernel time), B_alg = N_ops/2 * 8 + 3*4*D
@@ -16,8 +16,6 @@ engine reproduces.
 import argparse
 import json
 import os
-import re
-import subprocess
 import sys
 import tempfile
 import time
@@ -35,7 +33,6 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-iters", type=int, default=2, help="reference Lanczos steps in the CPU sample")
     ap.add_argument("--config", type=int, default=4, help="benchmark configuration (4 = pf 6p6n)")
     return ap.parse_args()
 
@@ -50,37 +47,48 @@ def traffic_from_profiles():
         return json.load(f).get("bytes_per_launch")
 
 
-def cpu_baseline(case, iters):
-    """The reference executable (oracle/_ref, built from the reference's
-    sources) on this host's cores: a Lanczos run of `iters` steps on the same
-    workload; the matvec rate comes from its own end-of-run report
-    ('Time total in H mat-vec multiply', src/bigstick_main.f90)."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "bigstick.x")
-    if not os.path.exists(exe):
-        return None
-    from bigstickpublick_amd.inputs import menu_script, write_case
-    ncores = len(os.sched_getaffinity(0))
-    threads = min(ncores, 16)
-    with tempfile.TemporaryDirectory() as d:
-        write_case(case, d)
-        c = case
-        c.niter = iters
-        c.nkeep = 1
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+def cpu_baseline(sps, intf, case, target_s=15.0):
+    """CPU leg: the factorized FP32 OpenMP port of the reference's apply
+    (oracle/bsoracle.c, bso_port_*; same loop structure as
+    src/bapplyhlibOMP.f90:979-1450) on this host's cores, timed on a bounded
+    sample of whole proton rows of the same workload and scaled to full H x by
+    the ratio of updates (the port counts its own updates per row)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    from oracle_api import Oracle
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = Oracle(sps, intf, case.Z, case.N, case.jz2)
+    p = o.port()
+    total = p.count()
+    x = np.random.default_rng(7).standard_normal(o.dim).astype(np.float32)
+    lo = p.nP // 3
+    step = max(1, p.nP // 4000)
+
+    def rows_for(work):
+        hi, acc = lo, 0
+        while hi < p.nP and acc < work:
+            nxt = min(p.nP, hi + step)
+            acc += p.count(hi, nxt)
+            hi = nxt
+        return hi
+
+    # calibrate on ~1 s of work (rate guess 1e9 updates/s), then size the sample
+    rate = 1e9
+    for _ in range(2):
+        hi = rows_for(rate * 1.0)
         t0 = time.time()
-        with open(os.path.join(d, "out.log"), "w") as fo:
-            r = subprocess.run([exe], input=menu_script(c, "n", "lf"), text=True, stdout=fo,
-                               stderr=subprocess.STDOUT, cwd=d, env=env, timeout=3000)
-        wall = time.time() - t0
-        txt = open(os.path.join(d, "out.log")).read()
-    m = re.search(r"Time total in H mat-vec multiply\s*:\s*([0-9.Ee+-]+)", txt)
-    if r.returncode != 0 or not m:
-        return None
-    tmv = float(m.group(1))
-    return {"value": iters / tmv, "unit": "H·v/s", "cores": threads, "kind": "reference",
-            "sample": "reference BIGSTICK (oracle/_ref, OpenMP) on the same pf 6p6n inputs: "
-                      "%d Lanczos iterations = %d H x, %.1f s in H x of %.1f s wall (setup included "
-                      "in wall only)" % (iters, iters, tmv, wall)}
+        _, n = p.apply(x, lo, hi)
+        rate = n / max(time.time() - t0, 1e-3)
+    hi2 = rows_for(rate * target_s)
+    t0 = time.time()
+    _, n = p.apply(x, lo, hi2)
+    dt = time.time() - t0
+    frac = n / total
+    return {"value": frac / dt, "unit": "H·v/s", "cores": threads, "kind": "port",
+            "sample": "factorized FP32 OpenMP port of the reference apply (oracle/bsoracle.c) on "
+                      "proton rows %d..%d of %d = %.2f%% of the %d updates of one H x, %.2f s; "
+                      "scaled by the update fraction" % (lo, hi2, p.nP, 100 * frac, total, dt)}
 
 
 def main():
@@ -162,7 +170,8 @@ def main():
     }
     if not a.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(case, a.cpu_iters)
+            del x, y
+            out["cpu_baseline"] = cpu_baseline(sps, intf, case)
         except Exception as e:  # the CPU leg is reported, never fatal
             out["cpu_baseline"] = None
             out["cpu_baseline_error"] = str(e)

@@ -962,3 +962,263 @@ void bso_apply(const bso_t *o, const double *x, double *y) {
   for (int it = 0; it < 2; it++) { free(mp[it].keys); free(mp[it].vals); }
   (void)find_sd;
 }
+
+/* ------------------------------------------------ factorized CPU port -- */
+/* A CPU restatement of the reference's factorized apply,
+ * applyHbundled_omp (src/bapplyhlibOMP.f90:979-1450): precomputed proton and
+ * neutron one- and two-body jump lists, FP32 vectors and FP32 matrix elements,
+ * OpenMP threads owning disjoint output rows (the reference splits output
+ * ranges between threads, :34-868).  It is used only as the CPU timing
+ * baseline (bench.py) and as a second, independent check of the factorized
+ * data flow against bso_apply. */
+typedef struct {
+  int64_t *p1ptr[2]; int32_t *p1src[2]; int16_t *p1a[2], *p1c[2]; int8_t *p1ph[2];
+  int64_t *p1dptr[2];  /* per target SD: (ndm+1) offsets of its jumps grouped by m_a - m_c */
+  int dmmin, ndm;
+  int64_t *p2ptr[2]; int32_t *p2src[2]; float *p2val[2];
+  int built;
+} bso_jumps_t;
+
+static bso_jumps_t g_jumps_dummy;
+
+typedef struct bso_port {
+  const bso_t *o;
+  bso_jumps_t j;
+} bso_port_t;
+
+bso_port_t *bso_port_create(const bso_t *o) {
+  bso_port_t *p = (bso_port_t *)calloc(1, sizeof(bso_port_t));
+  p->o = o;
+  int ns = o->nsps, npx = ns * (ns - 1) / 2;
+  for (int it = 0; it < 2; it++) {
+    sdmap_t mp;
+    map_build(&mp, o->mask[it], o->nsd[it]);
+    int64_t n = o->nsd[it];
+    /* one-body a+_a a_c (all a, c incl. a == c), listed by target */
+    int np = it == 0 ? o->Z : o->N;
+    int64_t cap1 = n * (int64_t)(np * (ns - np + 1) + 1);
+    p->j.p1ptr[it] = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    p->j.p1src[it] = (int32_t *)malloc(sizeof(int32_t) * (cap1 ? cap1 : 1));
+    p->j.p1a[it] = (int16_t *)malloc(sizeof(int16_t) * (cap1 ? cap1 : 1));
+    p->j.p1c[it] = (int16_t *)malloc(sizeof(int16_t) * (cap1 ? cap1 : 1));
+    p->j.p1ph[it] = (int8_t *)malloc(cap1 ? cap1 : 1);
+    int64_t e = 0;
+    p->j.p1ptr[it][0] = 0;
+    for (int64_t f = 0; f < n; f++) {
+      uint64_t fm = o->mask[it][f];
+      for (uint64_t ta = fm; ta; ta &= ta - 1) {
+        int a = __builtin_ctzll(ta);
+        uint64_t m1 = fm & ~(1ULL << a);
+        for (int c = 0; c < ns; c++) {
+          if (c != a && ((m1 >> c) & 1)) continue;
+          uint64_t im = m1 | (1ULL << c);
+          int64_t i = map_get(&mp, im);
+          if (i < 0) continue;
+          p->j.p1src[it][e] = (int32_t)i;
+          p->j.p1a[it][e] = (int16_t)a;
+          p->j.p1c[it][e] = (int16_t)c;
+          p->j.p1ph[it][e] = (int8_t)(fsign(im, c) * fsign(im & ~(1ULL << c), a));
+          e++;
+        }
+      }
+      p->j.p1ptr[it][f + 1] = e;
+    }
+    /* group each target's jumps by Delta = m_a - m_c (insertion sort) */
+    {
+      int dmax = 0;
+      for (int q = 0; q < ns; q++) if (o->sm2[q] > dmax) dmax = o->sm2[q];
+      p->j.dmmin = -2 * dmax;
+      p->j.ndm = 2 * dmax + 1;
+      const int ndm = p->j.ndm;
+      p->j.p1dptr[it] = (int64_t *)malloc(sizeof(int64_t) * (n * (ndm + 1) + 1));
+      for (int64_t f = 0; f < n; f++) {
+        int64_t b = p->j.p1ptr[it][f], en = p->j.p1ptr[it][f + 1];
+        for (int64_t q = b + 1; q < en; q++) {
+          int32_t s0 = p->j.p1src[it][q]; int16_t a0 = p->j.p1a[it][q], c0 = p->j.p1c[it][q]; int8_t h0 = p->j.p1ph[it][q];
+          int d0 = o->sm2[a0] - o->sm2[c0];
+          int64_t r = q - 1;
+          while (r >= b && o->sm2[p->j.p1a[it][r]] - o->sm2[p->j.p1c[it][r]] > d0) {
+            p->j.p1src[it][r + 1] = p->j.p1src[it][r]; p->j.p1a[it][r + 1] = p->j.p1a[it][r];
+            p->j.p1c[it][r + 1] = p->j.p1c[it][r]; p->j.p1ph[it][r + 1] = p->j.p1ph[it][r];
+            r--;
+          }
+          p->j.p1src[it][r + 1] = s0; p->j.p1a[it][r + 1] = a0; p->j.p1c[it][r + 1] = c0; p->j.p1ph[it][r + 1] = h0;
+        }
+        int64_t q = b;
+        for (int k = 0; k <= ndm; k++) {
+          int dk = p->j.dmmin + 2 * k;  /* group k holds Delta == dk (odd k slots stay empty) */
+          while (q < en && o->sm2[p->j.p1a[it][q]] - o->sm2[p->j.p1c[it][q]] < dk) q++;
+          p->j.p1dptr[it][f * (ndm + 1) + k] = q;
+        }
+        p->j.p1dptr[it][f * (ndm + 1) + ndm] = en;
+      }
+    }
+    /* two-body a+_i a+_j a_l a_k, listed by target with value*phase */
+    const float *v = it == 0 ? o->vpp : o->vnn;
+    int64_t cap2 = 0;
+    for (int64_t f = 0; f < n; f++) cap2 += (int64_t)(np * (np - 1) / 2) * (npx > 0 ? npx : 1);
+    if (cap2 > 400000000LL) cap2 = 400000000LL;
+    p->j.p2ptr[it] = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    int64_t cap = 1024;
+    p->j.p2src[it] = (int32_t *)malloc(sizeof(int32_t) * cap);
+    p->j.p2val[it] = (float *)malloc(sizeof(float) * cap);
+    e = 0;
+    p->j.p2ptr[it][0] = 0;
+    for (int64_t f = 0; f < n; f++) {
+      uint64_t fm = o->mask[it][f];
+      if (np >= 2)
+        for (uint64_t ti = fm; ti; ti &= ti - 1) {
+          int i = __builtin_ctzll(ti);
+          for (uint64_t tj = fm & ((1ULL << i) - 1); tj; tj &= tj - 1) {
+            int j = __builtin_ctzll(tj);
+            uint64_t m2 = fm & ~(1ULL << i) & ~(1ULL << j);
+            int mij = o->sm2[i] + o->sm2[j];
+            int cidx = i * (i - 1) / 2 + j;
+            for (int k = 1; k < ns; k++) {
+              if ((m2 >> k) & 1) continue;
+              for (int l = 0; l < k; l++) {
+                if ((m2 >> l) & 1) continue;
+                if (o->sm2[k] + o->sm2[l] != mij) continue;
+                float me = v[(size_t)cidx * npx + (k * (k - 1) / 2 + l)];
+                if (me == 0.0f) continue;
+                uint64_t sm = m2 | (1ULL << k) | (1ULL << l);
+                int sg = fsign(sm, k);
+                uint64_t s1 = sm & ~(1ULL << k);
+                sg *= fsign(s1, l);
+                uint64_t s2 = s1 & ~(1ULL << l);
+                sg *= fsign(s2, j);
+                sg *= fsign(s2 | (1ULL << j), i);
+                int64_t src = map_get(&mp, sm);
+                if (src < 0) continue;
+                if (e == cap) {
+                  cap *= 2;
+                  p->j.p2src[it] = (int32_t *)realloc(p->j.p2src[it], sizeof(int32_t) * cap);
+                  p->j.p2val[it] = (float *)realloc(p->j.p2val[it], sizeof(float) * cap);
+                }
+                p->j.p2src[it][e] = (int32_t)src;
+                p->j.p2val[it][e] = me * (float)sg;
+                e++;
+              }
+            }
+          }
+        }
+      p->j.p2ptr[it][f + 1] = e;
+    }
+    free(mp.keys);
+    free(mp.vals);
+  }
+  p->j.built = 1;
+  (void)g_jumps_dummy;
+  return p;
+}
+
+void bso_port_free(bso_port_t *p) {
+  if (!p) return;
+  for (int it = 0; it < 2; it++) {
+    free(p->j.p1ptr[it]); free(p->j.p1src[it]); free(p->j.p1a[it]); free(p->j.p1c[it]); free(p->j.p1ph[it]);
+    free(p->j.p1dptr[it]);
+    free(p->j.p2ptr[it]); free(p->j.p2src[it]); free(p->j.p2val[it]);
+  }
+  free(p);
+}
+
+/* Number of updates bso_port_apply performs for proton rows lo..hi. */
+int64_t bso_port_count(const bso_port_t *p, int64_t lo, int64_t hi) {
+  const bso_t *o = p->o;
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+  for (int64_t f = lo; f < hi; f++) {
+    int cn = o->conj[0][o->sdsec[0][f]];
+    int64_t L = o->seccount[1][cn], n0 = o->secfirst[1][cn];
+    int64_t u = o->call_spe ? L : 0;
+    u += (p->j.p2ptr[0][f + 1] - p->j.p2ptr[0][f]) * L;
+    u += p->j.p2ptr[1][n0 + L] - p->j.p2ptr[1][n0];
+    if (o->Z > 0 && o->N > 0) {
+      const int ndm = p->j.ndm;
+      for (int64_t e = p->j.p1ptr[0][f]; e < p->j.p1ptr[0][f + 1]; e++) {
+        const int dm = o->sm2[p->j.p1a[0][e]] - o->sm2[p->j.p1c[0][e]];
+        const int k = (-dm - p->j.dmmin) / 2;
+        for (int64_t c = 0; c < L; c++)
+          u += p->j.p1dptr[1][(n0 + c) * (ndm + 1) + k + 1] - p->j.p1dptr[1][(n0 + c) * (ndm + 1) + k];
+      }
+    }
+    total += u;
+  }
+  return total;
+}
+
+/* y[rows of proton SDs lo..hi) = H x, FP32 throughout; returns the number of
+ * updates performed.  Threads take whole proton rows. */
+int64_t bso_port_apply(const bso_port_t *p, const float *x, float *y, int64_t lo, int64_t hi) {
+  const bso_t *o = p->o;
+  const int ns = o->nsps;
+  const int npn = ns * ns;
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 2) reduction(+ : total)
+  for (int64_t f = lo; f < hi; f++) {
+    int sp = o->sdsec[0][f];
+    int cn = o->conj[0][sp];
+    int64_t L = o->seccount[1][cn], n0 = o->secfirst[1][cn];
+    float *yr = y + o->start[0][f];
+    int64_t upd = 0;
+    for (int64_t c = 0; c < L; c++) yr[c] = 0.0f;
+    if (o->call_spe) {
+      float ep = 0.0f;
+      for (uint64_t t = o->mask[0][f]; t; t &= t - 1) ep += o->pspe[o->sorb[__builtin_ctzll(t)]];
+      const float *xr = x + o->start[0][f];
+      for (int64_t c = 0; c < L; c++) {
+        float en = 0.0f;
+        for (uint64_t t = o->mask[1][n0 + c]; t; t &= t - 1) en += o->nspe[o->sorb[__builtin_ctzll(t)]];
+        yr[c] += (ep + en) * xr[c];
+      }
+      upd += L;
+    }
+    /* PP: conjugate neutron runs are contiguous (bapplyhlibOMP.f90:1090-1105) */
+    for (int64_t e = p->j.p2ptr[0][f]; e < p->j.p2ptr[0][f + 1]; e++) {
+      const float v = p->j.p2val[0][e];
+      const float *xr = x + o->start[0][p->j.p2src[0][e]];
+      for (int64_t c = 0; c < L; c++) yr[c] += v * xr[c];
+      upd += L;
+    }
+    /* NN within the row (:1032-1042) */
+    {
+      const float *xr = x + o->start[0][f];
+      for (int64_t c = 0; c < L; c++) {
+        int64_t n = n0 + c;
+        float s = 0.0f;
+        for (int64_t e = p->j.p2ptr[1][n]; e < p->j.p2ptr[1][n + 1]; e++)
+          s += p->j.p2val[1][e] * xr[p->j.p2src[1][e] - n0];
+        yr[c] += s;
+        upd += p->j.p2ptr[1][n + 1] - p->j.p2ptr[1][n];
+      }
+    }
+    /* PN: proton jump x neutron jump (:1253-1277) */
+    if (o->Z > 0 && o->N > 0) {
+      for (int64_t e = p->j.p1ptr[0][f]; e < p->j.p1ptr[0][f + 1]; e++) {
+        const int64_t i = p->j.p1src[0][e];
+        const int a = p->j.p1a[0][e], cc = p->j.p1c[0][e];
+        const float php = (float)p->j.p1ph[0][e];
+        const int dm = o->sm2[a] - o->sm2[cc];
+        const int ci = o->conj[0][o->sdsec[0][i]];
+        const int64_t n0i = o->secfirst[1][ci];
+        const float *xr = x + o->start[0][i];
+        const int k = (-dm - p->j.dmmin) / 2;
+        const int ndm = p->j.ndm;
+        for (int64_t c = 0; c < L; c++) {
+          int64_t n = n0 + c;
+          float s = 0.0f;
+          const int64_t q0 = p->j.p1dptr[1][n * (ndm + 1) + k], q1 = p->j.p1dptr[1][n * (ndm + 1) + k + 1];
+          for (int64_t q = q0; q < q1; q++) {
+            const int b = p->j.p1a[1][q], d = p->j.p1c[1][q];
+            const float me = o->vpn[(size_t)(a * ns + b) * npn + (cc * ns + d)];
+            s += me * (float)p->j.p1ph[1][q] * xr[p->j.p1src[1][q] - n0i];
+            upd++;
+          }
+          yr[c] += php * s;
+        }
+      }
+    }
+    total += upd;
+  }
+  return total;
+}

@@ -37,6 +37,13 @@ def lib():
         L.bso_coupled.restype = ctypes.c_float
         L.bso_coupled.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6
         L.bso_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.bso_port_create.restype = ctypes.c_void_p
+        L.bso_port_create.argtypes = [ctypes.c_void_p]
+        L.bso_port_free.argtypes = [ctypes.c_void_p]
+        L.bso_port_apply.restype = ctypes.c_int64
+        L.bso_port_apply.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 2
+        L.bso_port_count.restype = ctypes.c_int64
+        L.bso_port_count.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
         L.bso_cleb.restype = ctypes.c_float
         L.bso_cleb.argtypes = [ctypes.c_int] * 6
         _lib = L
@@ -90,6 +97,10 @@ class Oracle:
         y = np.zeros_like(x)
         lib().bso_apply(self.h, x.ctypes.data, y.ctypes.data)
         return y
+
+    def port(self):
+        """Factorized FP32 CPU port (OpenMP) sharing this oracle's tables."""
+        return Port(self)
 
     def dense(self):
         D = self.dim
@@ -152,3 +163,28 @@ def read_basis_dump(path, nsps):
             nm[n] = nmask
             triples.append((s, p, n))
     return pm, nm, np.array(triples, dtype=np.int64)
+
+
+class Port:
+    """The factorized CPU restatement of the reference's apply (bso_port_*)."""
+
+    def __init__(self, oracle):
+        self.o = oracle
+        self.h = lib().bso_port_create(oracle.h)
+        self.nP = lib().bso_nsd(oracle.h, 0)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bso_port_free(self.h)
+            self.h = None
+
+    def count(self, lo=0, hi=None):
+        return lib().bso_port_count(self.h, lo, self.nP if hi is None else hi)
+
+    def apply(self, x, lo=0, hi=None):
+        """FP32 y = H x for proton rows lo..hi (others left zero)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        n = lib().bso_port_apply(self.h, x.ctypes.data, y.ctypes.data, lo,
+                                 self.nP if hi is None else hi)
+        return y, n

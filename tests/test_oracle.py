@@ -144,3 +144,17 @@ def test_hermitian_and_planted_mistake(casedir):
     Hb = Oracle(sps, bad, case.Z, case.N, case.jz2).dense()
     R = dense_from_fixture(fx)
     assert np.abs(Hb - R).max() > 1e-2
+
+
+@pytest.mark.parametrize("name", ["sd2p2n", "sd1p0n_m1", "sd0p3n_m3", "pf2p1n_m3"])
+def test_factorized_port_matches_bruteforce(name, casedir):
+    """The factorized CPU port (reference loop structure, FP32) against the
+    brute-force FP64 oracle: two independent constructions of H x."""
+    case, d, fx = small_fixture(name, casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    p = o.port()
+    x = np.random.default_rng(0).standard_normal(o.dim)
+    y, n = p.apply(x)
+    want = o.apply(x)
+    assert np.abs(y - want).max() < 1e-6 * np.abs(want).max()
+    assert n == p.count()
