@@ -99,16 +99,25 @@ template <typename T> __device__ __forceinline__ T vload(const float* a, const d
 template <> __device__ __forceinline__ float vload<float>(const float* a, const double*, int64_t i) { return a[i]; }
 template <> __device__ __forceinline__ double vload<double>(const float*, const double* b, int64_t i) { return b[i]; }
 
+// G values of one LDS column, read with one wide LDS load.
+template <typename T, int G>
+struct alignas(sizeof(T) * G) VecG {
+  T v[G];
+};
+
 // One workgroup per tile of up to R proton rows of one sector; BLOCK threads,
-// CPT columns per thread (column = tid + q*BLOCK).  Dynamic LDS holds G staged
-// x rows ([G][lmax]) followed by the pn element block of the current class
-// ([size_k][wcols], last column zero).
+// CPT columns per thread (column = tid + q*BLOCK).  Dynamic LDS holds staged
+// x rows stored transposed -- column-major [lmax][max(R,G)], so one wide load
+// fetches every staged row at a source column -- followed by the batch's
+// coefficient table Wb[nl][g] = phase_g * W_k[pp_g][nl] (zero row at the
+// padding index).
 template <typename T, int BLOCK, int CPT, int R, int G>
 __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __restrict__ x,
                                                        T* __restrict__ y, int accumulate) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int S = R > G ? R : G;
   T* xs = reinterpret_cast<T*>(smem);
-  float* ws = reinterpret_cast<float*>(smem + (size_t)G * tb.lmax * sizeof(T));
+  float* wb = reinterpret_cast<float*>(smem + (size_t)S * tb.lmax * sizeof(T));
   const int tid = threadIdx.x;
   const int f0 = tb.tile_f0[blockIdx.x];
   const int nr = tb.tile_nr[blockIdx.x];
@@ -116,7 +125,6 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   const int cn = tb.pconj[sp];
   const int L = tb.nlen[cn];
   const int nf0 = tb.nfirst[cn];
-  const int lm = tb.lmax;
   T acc[R][CPT];
 #pragma unroll
   for (int r = 0; r < R; r++)
@@ -154,13 +162,16 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
       }
     }
   }
-  // ---- NN: stage the tile's own rows, gather in LDS, entries shared by rows
-  for (int r = 0; r < nr; r++) {
-    const int64_t bf = tb.pstart[f0 + r];
-    for (int c = tid; c < L; c += BLOCK) xs[r * lm + c] = x[bf + c];
-  }
-  __syncthreads();
+  // ---- NN: the tile's rows staged transposed ([col][R]); one wide LDS read
+  // per entry serves every row of the tile
   {
+    VecG<T, R>* xr = reinterpret_cast<VecG<T, R>*>(xs);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t bf = r < nr ? tb.pstart[f0 + r] : 0;
+      for (int c = tid; c < L; c += BLOCK) xs[c * R + r] = r < nr ? x[bf + c] : T(0);
+    }
+    __syncthreads();
     const int64_t base = tb.nn_base[cn];
     const int E = tb.nn_emax[cn];
 #pragma unroll
@@ -171,14 +182,16 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           const int64_t o = base + (int64_t)e * L + c;
           const int sl = tb.nn_src[o];
           const T v = vload<T>(tb.nn_val32, tb.nn_val64, o);
+          const VecG<T, R> xv = xr[sl];
 #pragma unroll
-          for (int r = 0; r < R; r++)
-            if (r < nr) acc[r][q] += v * xs[r * lm + sl];
+          for (int r = 0; r < R; r++) acc[r][q] += v * xv.v[r];
         }
       }
     }
   }
-  // ---- PN: per Delta class, batches of G staged source rows
+  // ---- PN: per Delta class, batches of G source rows staged transposed
+  VecG<T, G>* xg = reinterpret_cast<VecG<T, G>*>(xs);
+  const VecG<float, G>* wg = reinterpret_cast<const VecG<float, G>*>(wb);
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k = 0; k < nd; k++) {
     int p0[R], p1[R];
@@ -196,13 +209,8 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     const int kn = tb.negk[k];
     const int si = tb.srcsec[sp * nd + k];
     const int Li = tb.nlen[tb.pconj[si]];
-    const int wr = tb.wrows[k], wsd = tb.wstride[k], wc = tb.wcols;
+    const int wsd = tb.wstride[k];
     const float* Wk = tb.W + tb.woff[k];
-    __syncthreads();
-    for (int i = tid; i < wr * wc; i += BLOCK) {
-      const int a = i / wc, b = i - a * wc;
-      ws[i] = b < wsd ? Wk[a * wsd + b] : 0.0f;
-    }
     const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
     const int E = tb.n1_emax[(int64_t)cn * nd + kn];
 #pragma unroll
@@ -210,19 +218,21 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
       for (int b0 = p0[r]; b0 < p1[r]; b0 += G) {
         const int nb = min(G, p1[r] - b0);
         __syncthreads();
-        for (int g = 0; g < nb; g++) {
-          const int64_t bi = tb.pstart[tb.p1_src[b0 + g]];
-          for (int c = tid; c < Li; c += BLOCK) xs[g * lm + c] = x[bi + c];
+        for (int g = 0; g < G; g++) {
+          const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
+          for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = g < nb ? x[bi + c] : T(0);
+        }
+        for (int i = tid; i < (wsd + 1) * G; i += BLOCK) {
+          const int nl = i / G, g = i - nl * G;
+          float w = 0.0f;
+          if (g < nb && nl < wsd) {
+            const int meta = tb.p1_meta[b0 + g];
+            w = Wk[(meta & 0xff) * wsd + nl];
+            if (meta >> 8) w = -w;
+          }
+          wb[i] = w;
         }
         __syncthreads();
-        int wo[G];
-        int neg[G];
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-          const int meta = g < nb ? (int)tb.p1_meta[b0 + g] : 0;
-          wo[g] = (meta & 0xff) * wc;
-          neg[g] = (meta >> 8) & 1;
-        }
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
           const int c = tid + q * BLOCK;
@@ -230,16 +240,12 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
             T a = T(0);
             for (int e = 0; e < E; e++) {
               const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
-              const int sl = pk & 0xffff;
-              const int nl = (pk >> 16) & 0xff;
-              const int sn = (pk >> 24) & 1;
+              const VecG<T, G> xv = xg[pk & 0xffff];
+              const VecG<float, G> wv = wg[(pk >> 16) & 0xff];
+              T d = T(0);
 #pragma unroll
-              for (int g = 0; g < G; g++) {
-                if (g < nb) {
-                  const T t = (T)ws[wo[g] + nl] * xs[g * lm + sl];
-                  a += (neg[g] ^ sn) ? -t : t;
-                }
-              }
+              for (int g = 0; g < G; g++) d += (T)wv.v[g] * xv.v[g];
+              a += ((pk >> 24) & 1) ? -d : d;
             }
             acc[r][q] += a;
           }
@@ -721,12 +727,17 @@ void upload_tables(bs_system* s) {
 
 // Kernel variants: (R rows per tile) x (G staged source rows).
 constexpr int kNumVariants = 4;
-constexpr int kVarR[kNumVariants] = {2, 2, 4, 4};
-constexpr int kVarG[kNumVariants] = {4, 8, 4, 8};
+constexpr int kVarR[kNumVariants] = {2, 4, 2, 4};
+constexpr int kVarG[kNumVariants] = {2, 2, 4, 4};
 
 template <typename T>
+size_t lds_bytes_rg(const bs_system* s, int R, int G) {
+  const int S = R > G ? R : G;
+  return (size_t)S * s->dt.lmax * sizeof(T) + (size_t)s->dt.wcols * G * sizeof(float);
+}
+template <typename T>
 size_t lds_bytes(const bs_system* s, int v) {
-  return (size_t)kVarG[v] * s->dt.lmax * sizeof(T) + (size_t)s->t.max_class * s->dt.wcols * sizeof(float);
+  return lds_bytes_rg<T>(s, kVarR[v], kVarG[v]);
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -737,7 +748,7 @@ void launch_variant(bs_system* s, const T* x, T* y, int accumulate, hipStream_t 
   DevTables d = s->dt;
   d.tile_f0 = s->tile_f0[ri].p;
   d.tile_nr = s->tile_nr[ri].p;
-  const size_t lds = (size_t)G * d.lmax * sizeof(T) + (size_t)s->t.max_class * d.wcols * sizeof(float);
+  const size_t lds = lds_bytes_rg<T>(s, R, G);
   auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;
   static bool attr_set = false;
   if (!attr_set) {
@@ -752,10 +763,10 @@ void launch_variant(bs_system* s, const T* x, T* y, int accumulate, hipStream_t 
 template <typename T, int CPT>
 void launch_cpt(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
   switch (v) {
-    case 0: launch_variant<T, CPT, 2, 4>(s, x, y, accumulate, st); break;
-    case 1: launch_variant<T, CPT, 2, 8>(s, x, y, accumulate, st); break;
-    case 2: launch_variant<T, CPT, 4, 4>(s, x, y, accumulate, st); break;
-    default: launch_variant<T, CPT, 4, 8>(s, x, y, accumulate, st); break;
+    case 0: launch_variant<T, CPT, 2, 2>(s, x, y, accumulate, st); break;
+    case 1: launch_variant<T, CPT, 4, 2>(s, x, y, accumulate, st); break;
+    case 2: launch_variant<T, CPT, 2, 4>(s, x, y, accumulate, st); break;
+    default: launch_variant<T, CPT, 4, 4>(s, x, y, accumulate, st); break;
   }
 }
 
