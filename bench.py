@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", type=int, default=4, help="benchmark configuration (4 = pf 6p6n)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, copy x into y once (a plain copy kernel of 8*D bytes) "
+                         "so hardware-counter runs can calibrate FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -138,6 +141,9 @@ def main():
     torch.cuda.synchronize()
     wall = time.time() - t0
     kern_ms = ev0.elapsed_time(ev1) / a.steps
+    if a.copy_probe:
+        y.copy_(x)
+        torch.cuda.synchronize()
     ms_per_step = wall * 1000.0 / a.steps
     b_alg = n_ops / 2 * 8 + 3 * 4 * D
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
@@ -166,7 +172,8 @@ def main():
         "b_alg_bytes": b_alg,
         "setup_s": {"host_tables": t_host, "upload": t_upload, "autotune": t_tune},
         "kernel_variant": {"rows_per_tile": System.variants()[variant][0],
-                           "staged_rows": System.variants()[variant][1]},
+                           "staged_rows": System.variants()[variant][1],
+                           "xcd_map": System.variants()[variant][2]},
     }
     if not a.no_cpu:
         try:

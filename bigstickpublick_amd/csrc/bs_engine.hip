@@ -57,9 +57,11 @@ struct DevTables {
   const int64_t* pstart;     // [nP] row offset of proton SD
   const float* pspe_sd;      // [nP]
   const float* nspe_sd;      // [nN]
-  // tiles: up to R consecutive proton rows of one sector
+  // tiles: up to R consecutive proton rows of one sector; tile_map sends
+  // workgroup b to a tile (-1: idle slot)
   const int32_t* tile_f0;
   const int32_t* tile_nr;
+  const int32_t* tile_map;
   const int32_t* psec;       // [nP] proton sector of each SD
   const int32_t* pconj;      // [nsecP] conjugate neutron sector
   const int32_t* nlen;       // [nsecN] SDs in neutron sector
@@ -119,8 +121,10 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   T* xs = reinterpret_cast<T*>(smem);
   float* wb = reinterpret_cast<float*>(smem + (size_t)S * tb.lmax * sizeof(T));
   const int tid = threadIdx.x;
-  const int f0 = tb.tile_f0[blockIdx.x];
-  const int nr = tb.tile_nr[blockIdx.x];
+  const int tile = tb.tile_map[blockIdx.x];
+  if (tile < 0) return;  // padding slot of the XCD map (uniform per workgroup)
+  const int f0 = tb.tile_f0[tile];
+  const int nr = tb.tile_nr[tile];
   const int sp = tb.psec[f0];
   const int cn = tb.pconj[sp];
   const int L = tb.nlen[cn];
@@ -466,8 +470,9 @@ struct bs_system {
   int variant[2] = {-1, -1};   // chosen (R, G) variant per dtype (0: f32, 1: f64)
   DevTables dt{};
   DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base;
-  DevBuf<int32_t> tile_f0[2], tile_nr[2];
-  int ntiles[2] = {0, 0};
+  DevBuf<int32_t> tile_f0[3], tile_nr[3], tile_map[3][2];
+  int ntiles[3] = {0, 0, 0};
+  int nslots[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   DevBuf<int32_t> psec, pconj, nlen, nfirst, srcsec, pp_src, nn_emax, nn_src, pcls, p1_src, n1_emax;
   DevBuf<int32_t> woff, wrows, wstride, negk;
   DevBuf<int16_t> p1_meta;
@@ -478,7 +483,12 @@ struct bs_system {
 
 namespace {
 
-constexpr int kTileRows[2] = {2, 4};   // R of the kernel variants
+constexpr int kTileRows[3] = {1, 2, 4};   // R of the kernel variants
+
+int negk_of(const bs::Tables& t, int k) {
+  return (int)(std::lower_bound(t.delta_of_class.begin(), t.delta_of_class.end(), -t.delta_of_class[k]) -
+               t.delta_of_class.begin());
+}
 constexpr int kApplyBlock = 512;
 
 void upload_tables(bs_system* s) {
@@ -641,17 +651,63 @@ void upload_tables(bs_system* s) {
     }
   for (int f = 0; f < nP; f++)
     if (P.start[f] < 0 || P.start[f] + nlen[P.conj[P.sdsec[f]]] > t.dim) throw std::runtime_error("row outside the vector");
-  // tiles for each R
-  for (int v = 0; v < 2; v++) {
+  // estimated work (updates) of each proton row, for balancing the XCD map
+  std::vector<double> rowwork(nP, 0.0);
+  {
+    std::vector<int64_t> ncnt((size_t)N.nsec * nd, 0);  // neutron jumps per (target sector, class)
+    for (int n = 0; n < nN; n++)
+      for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) ncnt[(size_t)N.sdsec[n] * nd + t.class_of_op[O1.op[q]]]++;
+    std::vector<int64_t> nnsec(N.nsec, 0);
+    for (int n = 0; n < nN; n++) nnsec[N.sdsec[n]] += t.tb[1].rowptr[n + 1] - t.tb[1].rowptr[n];
+    for (int f = 0; f < nP; f++) {
+      const int c = P.conj[P.sdsec[f]];
+      double w = (double)(t.tb[0].rowptr[f + 1] - t.tb[0].rowptr[f]) * nlen[c] + (double)nnsec[c];
+      for (int k = 0; k < nd; k++)
+        w += (double)(pcls[(size_t)f * (nd + 1) + k + 1] - pcls[(size_t)f * (nd + 1) + k]) * ncnt[(size_t)c * nd + negk_of(t, k)];
+      rowwork[f] = w + nlen[c];
+    }
+  }
+  // tiles for each R, and two workgroup->tile maps: identity, and XCD-aware
+  // (8 contiguous chunks of equal work; chunk j on the workgroups with
+  // id % 8 == j, which the dispatcher places on XCD j)
+  for (int v = 0; v < 3; v++) {
     std::vector<int32_t> f0, nr;
+    std::vector<double> tw;
     for (int sp = 0; sp < P.nsec; sp++)
       for (int64_t f = P.secfirst[sp]; f < P.secfirst[sp] + P.seccount[sp]; f += kTileRows[v]) {
         f0.push_back((int32_t)f);
-        nr.push_back((int32_t)std::min<int64_t>(kTileRows[v], P.secfirst[sp] + P.seccount[sp] - f));
+        int n = (int)std::min<int64_t>(kTileRows[v], P.secfirst[sp] + P.seccount[sp] - f);
+        nr.push_back(n);
+        double w = 0.0;
+        for (int r = 0; r < n; r++) w += rowwork[f + r];
+        tw.push_back(w);
       }
+    const int nt = (int)f0.size();
     s->tile_f0[v].upload(f0);
     s->tile_nr[v].upload(nr);
-    s->ntiles[v] = (int)f0.size();
+    s->ntiles[v] = nt;
+    std::vector<int32_t> ident(nt);
+    for (int i = 0; i < nt; i++) ident[i] = i;
+    s->tile_map[v][0].upload(ident);
+    s->nslots[v][0] = nt;
+    constexpr int kXcd = 8;
+    double total = 0.0;
+    for (double w : tw) total += w;
+    std::vector<int> cstart(kXcd + 1, nt);
+    cstart[0] = 0;
+    double acc = 0.0;
+    int j = 1;
+    for (int i = 0; i < nt && j < kXcd; i++) {
+      acc += tw[i];
+      while (j < kXcd && acc >= total * j / kXcd) cstart[j++] = i + 1;
+    }
+    int maxlen = 0;
+    for (int q = 0; q < kXcd; q++) maxlen = std::max(maxlen, cstart[q + 1] - cstart[q]);
+    std::vector<int32_t> xmap((size_t)maxlen * kXcd, -1);
+    for (int q = 0; q < kXcd; q++)
+      for (int i = cstart[q]; i < cstart[q + 1]; i++) xmap[(size_t)(i - cstart[q]) * kXcd + q] = i;
+    s->tile_map[v][1].upload(xmap);
+    s->nslots[v][1] = (int)xmap.size();
   }
   int cpt = 1;
   while (cpt * kApplyBlock < lmax) cpt *= 2;
@@ -726,9 +782,10 @@ void upload_tables(bs_system* s) {
 }
 
 // Kernel variants: (R rows per tile) x (G staged source rows).
-constexpr int kNumVariants = 4;
-constexpr int kVarR[kNumVariants] = {2, 4, 2, 4};
-constexpr int kVarG[kNumVariants] = {2, 2, 4, 4};
+constexpr int kNumShapes = 7;
+constexpr int kNumVariants = 2 * kNumShapes;  // (R, G) shape x {identity, XCD} map
+constexpr int kVarR[kNumShapes] = {1, 1, 2, 2, 2, 4, 4};
+constexpr int kVarG[kNumShapes] = {1, 2, 1, 2, 4, 2, 4};
 
 template <typename T>
 size_t lds_bytes_rg(const bs_system* s, int R, int G) {
@@ -737,17 +794,18 @@ size_t lds_bytes_rg(const bs_system* s, int R, int G) {
 }
 template <typename T>
 size_t lds_bytes(const bs_system* s, int v) {
-  return lds_bytes_rg<T>(s, kVarR[v], kVarG[v]);
+  return lds_bytes_rg<T>(s, kVarR[v % kNumShapes], kVarG[v % kNumShapes]);
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
 
 template <typename T, int CPT, int R, int G>
-void launch_variant(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st) {
-  const int ri = R == 2 ? 0 : 1;
+void launch_variant(bs_system* s, int map, const T* x, T* y, int accumulate, hipStream_t st) {
+  const int ri = R == 1 ? 0 : (R == 2 ? 1 : 2);
   DevTables d = s->dt;
   d.tile_f0 = s->tile_f0[ri].p;
   d.tile_nr = s->tile_nr[ri].p;
+  d.tile_map = s->tile_map[ri][map].p;
   const size_t lds = lds_bytes_rg<T>(s, R, G);
   auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;
   static bool attr_set = false;
@@ -756,17 +814,21 @@ void launch_variant(bs_system* s, const T* x, T* y, int accumulate, hipStream_t 
                                (int)kLdsLimit));
     attr_set = true;
   }
-  hipLaunchKernelGGL(fn, dim3(s->ntiles[ri]), dim3(kApplyBlock), lds, st, d, x, y, accumulate);
+  hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), lds, st, d, x, y, accumulate);
   HIP_OK(hipGetLastError());
 }
 
 template <typename T, int CPT>
 void launch_cpt(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
-  switch (v) {
-    case 0: launch_variant<T, CPT, 2, 2>(s, x, y, accumulate, st); break;
-    case 1: launch_variant<T, CPT, 4, 2>(s, x, y, accumulate, st); break;
-    case 2: launch_variant<T, CPT, 2, 4>(s, x, y, accumulate, st); break;
-    default: launch_variant<T, CPT, 4, 4>(s, x, y, accumulate, st); break;
+  const int m = v / kNumShapes;
+  switch (v % kNumShapes) {
+    case 0: launch_variant<T, CPT, 1, 1>(s, m, x, y, accumulate, st); break;
+    case 1: launch_variant<T, CPT, 1, 2>(s, m, x, y, accumulate, st); break;
+    case 2: launch_variant<T, CPT, 2, 1>(s, m, x, y, accumulate, st); break;
+    case 3: launch_variant<T, CPT, 2, 2>(s, m, x, y, accumulate, st); break;
+    case 4: launch_variant<T, CPT, 2, 4>(s, m, x, y, accumulate, st); break;
+    case 5: launch_variant<T, CPT, 4, 2>(s, m, x, y, accumulate, st); break;
+    default: launch_variant<T, CPT, 4, 4>(s, m, x, y, accumulate, st); break;
   }
 }
 
@@ -1119,10 +1181,11 @@ int bs_num_variants(int* n) {
   return 0;
 }
 
-int bs_variant_info(int v, int* rows, int* staged) {
+int bs_variant_info(int v, int* rows, int* staged, int* xcd_map) {
   if (v < 0 || v >= kNumVariants) return set_err("bad variant");
-  if (rows) *rows = kVarR[v];
-  if (staged) *staged = kVarG[v];
+  if (rows) *rows = kVarR[v % kNumShapes];
+  if (staged) *staged = kVarG[v % kNumShapes];
+  if (xcd_map) *xcd_map = v / kNumShapes;
   return 0;
 }
 
