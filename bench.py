@@ -171,6 +171,8 @@ def main():
         "updates_per_s_engine": (ops["spe"] + ops["pp"] + ops["nn"] + ops["pn"]) / (kern_ms * 1e-3),
         "b_alg_bytes": b_alg,
         "setup_s": {"host_tables": t_host, "upload": t_upload, "autotune": t_tune},
+        "autotune_ms": {"R%d_G%d_xcd%d" % v: round(float(t), 3)
+                        for v, t in zip(System.variants(), s.autotune_times("f32")) if t > 0},
         "kernel_variant": {"rows_per_tile": System.variants()[variant][0],
                            "staged_rows": System.variants()[variant][1],
                            "xcd_map": System.variants()[variant][2]},

@@ -31,5 +31,33 @@ def build_native(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FDRIVER = os.path.join(FORTRAN, "bs_driver")
+
+
+def build_fortran():
+    """The iso_c_binding module and the bs_driver Fortran host (amdflang);
+    skipped when no Fortran compiler is installed."""
+    fc = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    steps = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN, "bigstick_amd.f90"), "-o",
+         os.path.join(bdir, "bigstick_amd.o"), "-module-dir", bdir],
+        [fc, "-O2", "-I", bdir, os.path.join(FORTRAN, "bs_driver.f90"),
+         os.path.join(bdir, "bigstick_amd.o"), "-L", HERE, "-lbigstick_amd",
+         "-Wl,-rpath,$ORIGIN/..", "-L/opt/rocm/lib", "-lamdhip64", "-o", FDRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed:\n" + r.stdout + r.stderr)
+    os.replace(FDRIVER + ".tmp", FDRIVER)
+    return FDRIVER
+
+
 if __name__ == "__main__":
     print(build_native(verbose=True))
+    print(build_fortran())

@@ -178,18 +178,22 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     __syncthreads();
     const int64_t base = tb.nn_base[cn];
     const int E = tb.nn_emax[cn];
+    // entry-major over the thread's columns: CPT independent loads in flight
+    for (int e = 0; e < E; e++) {
+      int sl[CPT];
+      T v[CPT];
 #pragma unroll
-    for (int q = 0; q < CPT; q++) {
-      const int c = tid + q * BLOCK;
-      if (c < L) {
-        for (int e = 0; e < E; e++) {
-          const int64_t o = base + (int64_t)e * L + c;
-          const int sl = tb.nn_src[o];
-          const T v = vload<T>(tb.nn_val32, tb.nn_val64, o);
-          const VecG<T, R> xv = xr[sl];
+      for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * BLOCK;
+        const int64_t o = base + (int64_t)e * L + c;
+        sl[q] = c < L ? tb.nn_src[o] : 0;
+        v[q] = c < L ? vload<T>(tb.nn_val32, tb.nn_val64, o) : T(0);
+      }
 #pragma unroll
-          for (int r = 0; r < R; r++) acc[r][q] += v * xv.v[r];
-        }
+      for (int q = 0; q < CPT; q++) {
+        const VecG<T, R> xv = xr[sl[q]];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r][q] += v[q] * xv.v[r];
       }
     }
   }
@@ -237,21 +241,23 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           wb[i] = w;
         }
         __syncthreads();
+        // entry-major over the thread's columns: CPT independent loads in flight
+        const uint32_t pad = (uint32_t)wsd << 16;  // zero row of Wb
+        for (int e = 0; e < E; e++) {
+          uint32_t pk[CPT];
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int c = tid + q * BLOCK;
-          if (c < L) {
-            T a = T(0);
-            for (int e = 0; e < E; e++) {
-              const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
-              const VecG<T, G> xv = xg[pk & 0xffff];
-              const VecG<float, G> wv = wg[(pk >> 16) & 0xff];
-              T d = T(0);
+          for (int q = 0; q < CPT; q++) {
+            const int c = tid + q * BLOCK;
+            pk[q] = c < L ? tb.n1_pk[ebase + (int64_t)e * L + c] : pad;
+          }
 #pragma unroll
-              for (int g = 0; g < G; g++) d += (T)wv.v[g] * xv.v[g];
-              a += ((pk >> 24) & 1) ? -d : d;
-            }
-            acc[r][q] += a;
+          for (int q = 0; q < CPT; q++) {
+            const VecG<T, G> xv = xg[pk[q] & 0xffff];
+            const VecG<float, G> wv = wg[(pk[q] >> 16) & 0xff];
+            T d = T(0);
+#pragma unroll
+            for (int g = 0; g < G; g++) d += (T)wv.v[g] * xv.v[g];
+            acc[r][q] += ((pk[q] >> 24) & 1) ? -d : d;
           }
         }
       }
@@ -465,9 +471,13 @@ int grid_for(int64_t D) {
 
 struct bs_system {
   bs::Tables t;
+  std::string sps_path;
+  std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
+  DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
   bool uploaded = false;
   int cpt = 1;
-  int variant[2] = {-1, -1};   // chosen (R, G) variant per dtype (0: f32, 1: f64)
+  int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
+  float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
   DevTables dt{};
   DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base;
   DevBuf<int32_t> tile_f0[3], tile_nr[3], tile_map[3][2];
@@ -869,6 +879,7 @@ int autotune(bs_system* s, hipStream_t st) {
     HIP_OK(hipEventSynchronize(e1));
     float ms = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    s->tune_ms[di][v] = ms;
     if (ms < best) { best = ms; bestv = v; }
   }
   (void)hipEventDestroy(e0);
@@ -1058,6 +1069,28 @@ bs_system* checked(bs_system* s) {
 
 }  // namespace
 
+namespace {
+template <typename T>
+double expectation(bs_system* o, const T* psi) {
+  const int64_t D = o->t.dim;
+  const int nb = std::min(grid_for(D), 1024);
+  DevBuf<T> hy;
+  DevBuf<double> part, out;
+  hy.alloc(D);
+  part.alloc(nb);
+  out.alloc(2);
+  launch_apply<T>(o, psi, hy.p, 0, nullptr);
+  k_multidot<T><<<nb, kBlock>>>(psi, D, 1, hy.p, D, part.p);
+  k_reduce<<<1, kBlock>>>(part.p, nb, 1, out.p, 0);
+  k_multidot<T><<<nb, kBlock>>>(psi, D, 1, psi, D, part.p);
+  k_reduce<<<1, kBlock>>>(part.p, nb, 1, out.p + 1, 0);
+  HIP_OK(hipGetLastError());
+  double h[2];
+  HIP_OK(hipMemcpy(h, out.p, sizeof h, hipMemcpyDeviceToHost));
+  return h[0] / h[1];
+}
+}  // namespace
+
 extern "C" {
 
 const char* bs_last_error(void) { return g_err.c_str(); }
@@ -1068,6 +1101,7 @@ int bs_init_host(const char* sps, const char* intf, int Z, int N, int jz2, float
     if (!out) return set_err("null output pointer");
     std::unique_ptr<bs_system> s(new bs_system());
     s->t = bs::build_tables(sps, intf, Z, N, jz2, spscale, a0, a, x, true);
+    s->sps_path = sps;
     *out = s.release();
     return 0;
   } catch (const std::exception& e) {
@@ -1225,6 +1259,58 @@ int bs_autotune(bs_system* s, int dtype, int* v) {
   } catch (const std::exception& e) {
     return set_err(e.what());
   }
+}
+
+int bs_apply_h_host(bs_system* s, const void* x, void* y, int dtype, int accumulate) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (!x || !y) return set_err("null vector");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    const size_t bytes = (size_t)s->t.dim * (dtype == BS_F32 ? 4 : 8);
+    if (s->hostx.n < bytes) {
+      s->hostx.alloc(bytes);
+      s->hosty.alloc(bytes);
+    }
+    HIP_OK(hipMemcpy(s->hostx.p, x, bytes, hipMemcpyHostToDevice));
+    if (accumulate) HIP_OK(hipMemcpy(s->hosty.p, y, bytes, hipMemcpyHostToDevice));
+    if (dtype == BS_F32) launch_apply<float>(s, (const float*)s->hostx.p, (float*)s->hosty.p, accumulate ? 1 : 0, nullptr);
+    else launch_apply<double>(s, (const double*)s->hostx.p, (double*)s->hosty.p, accumulate ? 1 : 0, nullptr);
+    HIP_OK(hipMemcpy(y, s->hosty.p, bytes, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+
+int bs_observable(bs_system* s, int which, const void* psi, int dtype, double* value) {
+  try {
+    checked(s);
+    if (which != 0 && which != 1) return set_err("which must be 0 (J^2) or 1 (T^2)");
+    if (!psi || !value) return set_err("null argument");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (!s->obs[which]) {
+      std::unique_ptr<bs_system> o(new bs_system());
+      o->t = bs::build_tables_text(s->sps_path, bs::observable_int_text(s->t.in.orbits, which), s->t.in.Z,
+                                   s->t.in.N, s->t.in.jz2, true);
+      o->sps_path = s->sps_path;
+      if (o->t.dim != s->t.dim) return set_err("observable basis differs");
+      upload_tables(o.get());
+      s->obs[which] = std::move(o);
+    }
+    if (dtype == BS_F32) *value = expectation<float>(s->obs[which].get(), (const float*)psi);
+    else *value = expectation<double>(s->obs[which].get(), (const double*)psi);
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_autotune_times(const bs_system* s, int dtype, float* ms) {
+  if (!s || !ms || (dtype != BS_F32 && dtype != BS_F64)) return set_err("bad argument");
+  for (int v = 0; v < kNumVariants; v++) ms[v] = s->tune_ms[dtype == BS_F32 ? 0 : 1][v];
+  return 0;
 }
 
 int bs_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,

@@ -184,6 +184,9 @@ struct Builder {
   void read_int(const std::string& path) {
     std::ifstream f(path);
     if (!f) fail("cannot open .int file " + path);
+    read_int_stream(f);
+  }
+  void read_int_stream(std::istream& f) {
     auto tok = data_tokens(f);
     if (tok.empty()) fail(".int: empty file");
     long nme = std::strtol(tok[0].c_str(), nullptr, 10);
@@ -790,8 +793,55 @@ struct Builder {
 
 }  // namespace
 
+namespace {
+Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
+                 int jz2, float spscale, float a0, float a, float x, bool with_jumps);
+}
+
 Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps) {
+  return build_any(sps, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, with_jumps);
+}
+
+Tables build_tables_text(const std::string& sps, const std::string& inttext, int Z, int N, int jz2,
+                         bool with_jumps) {
+  return build_any(sps, nullptr, &inttext, Z, N, jz2, 1.0f, 1.0f, 1.0f, 1.0f, with_jumps);
+}
+
+// J^2 = sum_i j_i^2 + sum_{i<j} (J_ij^2 - j_i^2 - j_j^2) and the same for T^2,
+// written as an iso interaction on the orbits of `orbits`.
+std::string observable_int_text(const std::vector<Orbit>& orbits, int which) {
+  const int no = (int)orbits.size();
+  std::ostringstream os;
+  std::vector<std::string> lines;
+  char buf[160];
+  for (int a = 1; a <= no; a++)
+    for (int b = 1; b <= a; b++) {
+      const double ja = orbits[a - 1].j2 / 2.0, jb = orbits[b - 1].j2 / 2.0;
+      const int jmin = std::abs(orbits[a - 1].j2 - orbits[b - 1].j2) / 2;
+      const int jmax = (orbits[a - 1].j2 + orbits[b - 1].j2) / 2;
+      for (int J = jmin; J <= jmax; J++)
+        for (int T = 0; T <= 1; T++) {
+          if (a == b && (J + T) % 2 == 0) continue;
+          const double v = which == 0 ? J * (J + 1.0) - ja * (ja + 1.0) - jb * (jb + 1.0) : T * (T + 1.0) - 1.5;
+          std::snprintf(buf, sizeof buf, "%d %d %d %d %d %d %.9g", a, b, a, b, J, T, v);
+          lines.push_back(buf);
+        }
+    }
+  os << lines.size();
+  for (int a = 0; a < no; a++) {
+    const double ja = orbits[a].j2 / 2.0;
+    std::snprintf(buf, sizeof buf, " %.9g", which == 0 ? ja * (ja + 1.0) : 0.75);
+    os << buf;
+  }
+  os << "\n";
+  for (auto& l : lines) os << l << "\n";
+  return os.str();
+}
+
+namespace {
+Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
+                 int jz2, float spscale, float a0, float a, float x, bool with_jumps) {
   Tables t;
   read_sps(sps, t.in);
   for (auto& o : t.in.orbits) {
@@ -807,7 +857,12 @@ Tables build_tables(const std::string& sps, const std::string& intfile, int Z, i
   // src/binput_tbme.f90:1024-1031: TBMEs scaled by (A0/A)^x, or by A0 when A or x is 0
   t.in.vscale = (a == 0.0f || x == 0.0f) ? a0 : std::pow(a0 / a, x);
   Builder b(t);
-  b.read_int(intfile);
+  if (intfile) {
+    b.read_int(*intfile);
+  } else {
+    std::istringstream is(*inttext);
+    b.read_int_stream(is);
+  }
   b.subsume();
   b.make_sps();
   int nmax = 0;
@@ -825,6 +880,8 @@ Tables build_tables(const std::string& sps, const std::string& intfile, int Z, i
   }
   return t;
 }
+
+}  // namespace
 
 void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out) {
   const Species& S = t.sp[it];

@@ -110,6 +110,12 @@ struct Tables {
 void read_sps(const std::string& path, ModelInput& in);
 Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps = true);
+// Same as build_tables with the interaction given as the text of an iso .int
+// file (scaling 1 1 1 1).
+Tables build_tables_text(const std::string& sps, const std::string& inttext, int Z, int N, int jz2,
+                         bool with_jumps = true);
+// Iso interaction text of J^2 (which = 0) or T^2 (which = 1) on these orbits.
+std::string observable_int_text(const std::vector<Orbit>& orbits, int which);
 void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out);
 float cleb(int j1, int m1, int j2, int m2, int j3, int m3);
 

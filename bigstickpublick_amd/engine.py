@@ -50,6 +50,9 @@ def load_library():
     L.bs_set_variant.argtypes = [vp, i32, i32]
     L.bs_get_variant.argtypes = [vp, i32, ctypes.POINTER(i32)]
     L.bs_autotune.argtypes = [vp, i32, ctypes.POINTER(i32)]
+    L.bs_autotune_times.argtypes = [vp, i32, vp]
+    L.bs_apply_h_host.argtypes = [vp, vp, vp, i32, i32]
+    L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     _lib = L
@@ -175,6 +178,34 @@ class System:
         v = ctypes.c_int()
         _check(load_library().bs_autotune(self.h, dt, ctypes.byref(v)))
         return v.value
+
+    def apply_host(self, x, accumulate=False, y=None):
+        """numpy in, numpy out (host-pointer entry point, PCIe included)."""
+        self.upload()
+        x = np.ascontiguousarray(x)
+        if x.dtype not in (np.float32, np.float64) or x.size != self.dim:
+            raise ValueError("x must be float32/float64 of length dim")
+        if y is None:
+            y = np.zeros_like(x)
+        dt = BS_F32 if x.dtype == np.float32 else BS_F64
+        _check(load_library().bs_apply_h_host(self.h, x.ctypes.data, y.ctypes.data, dt, int(accumulate)))
+        return y
+
+    def observable(self, which, psi):
+        """<psi|J^2|psi> (which='J2') or <psi|T^2|psi> (which='T2') for a CUDA vector."""
+        import torch  # noqa: F401
+        self.upload()
+        dt = BS_F32 if psi.dtype.itemsize == 4 else BS_F64
+        v = ctypes.c_double()
+        _check(load_library().bs_observable(self.h, 0 if which == "J2" else 1,
+                                            ctypes.c_void_p(psi.data_ptr()), dt, ctypes.byref(v)))
+        return v.value
+
+    def autotune_times(self, dtype="f32"):
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        out = np.zeros(len(self.variants()), np.float32)
+        _check(load_library().bs_autotune_times(self.h, dt, out.ctypes.data))
+        return out
 
     def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
                 want_vectors=False):

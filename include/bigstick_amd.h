@@ -98,6 +98,22 @@ int bs_variant_info(int variant, int *rows, int *staged, int *xcd_map);
 int bs_set_variant(bs_system *sys, int dtype, int variant);
 int bs_get_variant(const bs_system *sys, int dtype, int *variant);
 int bs_autotune(bs_system *sys, int dtype, int *variant);
+/* Per-variant kernel times (ms) of the last bs_autotune for dtype; 0 for
+ * variants that were not timed.  ms has bs_num_variants entries. */
+int bs_autotune_times(const bs_system *sys, int dtype, float *ms);
+
+/* Host-pointer form of bs_apply_h for hosts that keep vectors in host memory
+ * (e.g. the reference's Fortran vec1/vec2): copies x (and y when
+ * accumulating) to the device, applies H, copies y back; synchronous.  The
+ * PCIe copies dominate its cost; device-resident callers use bs_apply_h. */
+int bs_apply_h_host(bs_system *sys, const void *x, void *y, int dtype, int accumulate);
+
+/* <psi|O|psi>/<psi|psi> for O = J^2 (which = 0) or T^2 (which = 1) on the
+ * same basis, psi a device vector of dim elements (the reference's
+ * applyobsbundled, src/bapplyobslib1.f90:110-197).  The operator tables are
+ * built on first use as the iso interaction J^2 = sum_i j_i^2 +
+ * sum_{i<j} 2 j_i.j_j (and likewise T^2) and applied with the same kernel. */
+int bs_observable(bs_system *sys, int which, const void *psi, int dtype, double *value);
 
 typedef struct {
   int nkeep;        /* eigenpairs wanted */

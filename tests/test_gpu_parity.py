@@ -110,3 +110,21 @@ def test_thick_restart_config0(casedir):
     # the plain run's 4th state is only converged to a residual ~1e-3
     assert np.abs(tr.evals - ref.evals).max() < 1e-6
     assert np.abs(tr.evals[:3] - ref.evals[:3]).max() < 1e-10
+
+
+def test_observables_and_host_apply(casedir):
+    """J^2, T^2 of Ritz vectors are J(J+1), T(T+1) with the reference's J, T
+    (.res of config 0); the host-pointer apply equals the device apply."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    r = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True, want_vectors=True)
+    for k in range(3):
+        j2 = s.observable("J2", r.evecs[k])
+        t2 = s.observable("T2", r.evecs[k])
+        J, T = fx["res"][k, 1], fx["res"][k, 2]
+        assert abs(j2 - J * (J + 1)) < 1e-6
+        assert abs(t2 - T * (T + 1)) < 1e-6
+    x = np.random.default_rng(9).standard_normal(s.dim)
+    yd = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
+    yh = s.apply_host(x)
+    assert np.array_equal(yd, yh)
