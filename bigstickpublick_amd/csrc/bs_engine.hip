@@ -389,6 +389,134 @@ __global__ void k_convert(U* __restrict__ out, const T* __restrict__ in, int64_t
     out[i] = (U)in[i];
 }
 
+// ------------------------------------------------------- adjoint (densities)
+// rho[me] = <psi| O_me |psi> for every uncoupled operator, walking the same
+// jump tables as the apply (the analogue of applyh{PP,NN,PN}bundled_den,
+// src/bdenslib4.f90:129-725).  FP64 accumulation.
+struct DensTables {
+  // PP: unmerged operators by target proton row
+  const int64_t* pp_ptr;
+  const int32_t* pp_src;
+  const int32_t* pp_me;
+  const int8_t* pp_ph;
+  // NN: per neutron sector, flat lists of (target col | source col << 12 | sign << 24) and compact element id
+  const int64_t* nn_off;     // [nsecN + 1]
+  const uint32_t* nn_pk;
+  const int32_t* nn_mc;
+  int nn_ncomp;
+  // PN: lists per (target neutron sector, class, neutron operator)
+  const int64_t* pn_off;     // [(c*nd + k)*(maxcls+1) + nl]
+  const uint32_t* pn_pk;     // target col | source col << 12 | sign << 24
+  int maxcls1;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_density_pp(DevTables tb, DensTables dn, const T* __restrict__ x,
+                                                    double* __restrict__ rho) {
+  const int f = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L = tb.nlen[tb.pconj[tb.psec[f]]];
+  const int64_t bf = tb.pstart[f];
+  for (int64_t e = dn.pp_ptr[f] + wave; e < dn.pp_ptr[f + 1]; e += 4) {
+    const int64_t bs = tb.pstart[dn.pp_src[e]];
+    double a = 0.0;
+    for (int c = lane; c < L; c += 64) a += (double)x[bf + c] * (double)x[bs + c];
+    a = wave_sum(a);
+    if (lane == 0) atomicAdd(&rho[dn.pp_me[e]], (double)dn.pp_ph[e] * a);
+  }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_density_nn(DevTables tb, DensTables dn, const T* __restrict__ x,
+                                                    double* __restrict__ rho_comp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* rl = reinterpret_cast<double*>(smem);
+  T* xs = reinterpret_cast<T*>(smem + (size_t)dn.nn_ncomp * sizeof(double));
+  const int tile = blockIdx.x;
+  const int f0 = tb.tile_f0[tile], nr = tb.tile_nr[tile];
+  const int cn = tb.pconj[tb.psec[f0]];
+  const int L = tb.nlen[cn];
+  for (int i = threadIdx.x; i < dn.nn_ncomp; i += 256) rl[i] = 0.0;
+  for (int r = 0; r < R; r++) {
+    const int64_t bf = r < nr ? tb.pstart[f0 + r] : 0;
+    for (int c = threadIdx.x; c < L; c += 256) xs[c * R + r] = r < nr ? x[bf + c] : T(0);
+  }
+  __syncthreads();
+  const VecG<T, R>* xv = reinterpret_cast<const VecG<T, R>*>(xs);
+  for (int64_t q = dn.nn_off[cn] + threadIdx.x; q < dn.nn_off[cn + 1]; q += 256) {
+    const uint32_t pk = dn.nn_pk[q];
+    const VecG<T, R> a = xv[pk & 0xfff];
+    const VecG<T, R> b = xv[(pk >> 12) & 0xfff];
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) v += (double)a.v[r] * (double)b.v[r];
+    atomicAdd(&rl[dn.nn_mc[q]], ((pk >> 24) & 1) ? -v : v);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < dn.nn_ncomp; i += 256)
+    if (rl[i] != 0.0) atomicAdd(&rho_comp[i], rl[i]);
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_density_pn(DevTables tb, DensTables dn, const T* __restrict__ x,
+                                                    double* __restrict__ rho_w) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* xs = reinterpret_cast<T*>(smem);                               // [lmax][G] staged sources
+  T* yf = reinterpret_cast<T*>(smem + (size_t)G * tb.lmax * sizeof(T));  // own row
+  const int f = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sp = tb.psec[f];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  const int64_t bf = tb.pstart[f];
+  for (int c = threadIdx.x; c < L; c += 256) yf[c] = x[bf + c];
+  const VecG<T, G>* xg = reinterpret_cast<const VecG<T, G>*>(xs);
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int k = 0; k < nd; k++) {
+    const int p0 = tb.pcls[(int64_t)f * nd1 + k], p1 = tb.pcls[(int64_t)f * nd1 + k + 1];
+    if (p0 == p1) continue;
+    const int kn = tb.negk[k];
+    const int si = tb.srcsec[sp * nd + k];
+    const int Li = tb.nlen[tb.pconj[si]];
+    const int ws = tb.wstride[k];
+    const int64_t lbase = ((int64_t)cn * nd + kn) * dn.maxcls1;
+    for (int b0 = p0; b0 < p1; b0 += G) {
+      const int nb = min(G, p1 - b0);
+      __syncthreads();
+      for (int g = 0; g < G; g++) {
+        const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
+        for (int c = threadIdx.x; c < Li; c += 256) xs[c * G + g] = g < nb ? x[bi + c] : T(0);
+      }
+      __syncthreads();
+      for (int nl = wave; nl < ws; nl += 4) {
+        double a[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) a[g] = 0.0;
+        for (int64_t q = dn.pn_off[lbase + nl] + lane; q < dn.pn_off[lbase + nl + 1]; q += 64) {
+          const uint32_t pk = dn.pn_pk[q];
+          const double yv = ((pk >> 24) & 1) ? -(double)yf[pk & 0xfff] : (double)yf[pk & 0xfff];
+          const VecG<T, G> xv = xg[(pk >> 12) & 0xfff];
+#pragma unroll
+          for (int g = 0; g < G; g++) a[g] += yv * (double)xv.v[g];
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) a[g] = wave_sum(a[g]);
+        if (lane == 0)
+          for (int g = 0; g < nb; g++) {
+            const int meta = tb.p1_meta[b0 + g];
+            const double s = (meta >> 8) ? -a[g] : a[g];
+            atomicAdd(&rho_w[tb.woff[k] + (meta & 0xff) * ws + nl], s);
+          }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- helpers
 template <typename T>
 struct DevBuf {
@@ -474,6 +602,16 @@ struct bs_system {
   std::string sps_path;
   std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
   DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
+  // adjoint tables (built on first bs_density)
+  bool dens_ready = false;
+  DensTables dn{};
+  DevBuf<int64_t> d_pp_ptr, d_nn_off, d_pn_off;
+  DevBuf<int32_t> d_pp_src, d_pp_me, d_nn_mc;
+  DevBuf<int8_t> d_pp_ph;
+  DevBuf<uint32_t> d_nn_pk, d_pn_pk;
+  std::vector<int32_t> nn_comp_to_me;
+  bool jac_ready = false;
+  std::vector<bs::ParamColumn> jac;
   bool uploaded = false;
   int cpt = 1;
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
@@ -1091,6 +1229,167 @@ double expectation(bs_system* o, const T* psi) {
 }
 }  // namespace
 
+namespace {
+
+void build_density_tables(bs_system* s) {
+  const bs::Tables& t = s->t;
+  const bs::Species& P = t.sp[0];
+  const bs::Species& N = t.sp[1];
+  const int nd = t.ndelta;
+  // PP: unmerged proton operators
+  bs::TwoBodyOps op;
+  bs::build_twobody_ops(t, 0, op);
+  s->d_pp_ptr.upload(op.rowptr);
+  s->d_pp_src.upload(op.src.empty() ? std::vector<int32_t>(1, 0) : op.src);
+  s->d_pp_me.upload(op.me.empty() ? std::vector<int32_t>(1, 0) : op.me);
+  s->d_pp_ph.upload(op.phase.empty() ? std::vector<int8_t>(1, 0) : op.phase);
+  // NN: flat per-sector lists with compact element ids
+  bs::TwoBodyOps on;
+  bs::build_twobody_ops(t, 1, on);
+  std::vector<int32_t> me_to_comp((size_t)t.npairxx * t.npairxx, -1);
+  s->nn_comp_to_me.clear();
+  std::vector<int64_t> off(N.nsec + 1, 0);
+  std::vector<uint32_t> pk;
+  std::vector<int32_t> mc;
+  for (int c = 0; c < N.nsec; c++) {
+    off[c] = (int64_t)pk.size();
+    for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++)
+      for (int64_t q = on.rowptr[n]; q < on.rowptr[n + 1]; q++) {
+        const int64_t nf = n - N.secfirst[c], ns = on.src[q] - N.secfirst[c];
+        if (ns < 0 || ns >= N.seccount[c]) throw std::runtime_error("NN operator leaves its sector");
+        int32_t& cid = me_to_comp[on.me[q]];
+        if (cid < 0) {
+          cid = (int32_t)s->nn_comp_to_me.size();
+          s->nn_comp_to_me.push_back(on.me[q]);
+        }
+        pk.push_back((uint32_t)nf | ((uint32_t)ns << 12) | (on.phase[q] < 0 ? (1u << 24) : 0u));
+        mc.push_back(cid);
+      }
+  }
+  off[N.nsec] = (int64_t)pk.size();
+  if (pk.empty()) { pk.push_back(0); mc.push_back(0); }
+  s->d_nn_off.upload(off);
+  s->d_nn_pk.upload(pk);
+  s->d_nn_mc.upload(mc);
+  // PN: per (target neutron sector, class, neutron operator) lists
+  const int mc1 = t.max_class + 1;
+  std::vector<int64_t> poff((size_t)N.nsec * nd * mc1 + 1, 0);
+  std::vector<std::vector<uint32_t>> bucket(mc1);
+  std::vector<uint32_t> ppk;
+  const bs::OneBody& O1 = t.ob[1];
+  for (int c = 0; c < N.nsec; c++)
+    for (int k = 0; k < nd; k++) {
+      for (auto& b : bucket) b.clear();
+      int want = -1;
+      for (int z = 0; z < N.nsec; z++)
+        if (N.secjz[z] == N.secjz[c] - t.delta_of_class[k]) want = z;
+      for (int64_t n = N.secfirst[c]; n < N.secfirst[c] + N.seccount[c]; n++)
+        for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) {
+          if (t.class_of_op[O1.op[q]] != k) continue;
+          const int64_t nf = n - N.secfirst[c], ni = O1.src[q] - N.secfirst[want];
+          bucket[t.local_of_op[O1.op[q]]].push_back((uint32_t)nf | ((uint32_t)ni << 12) |
+                                                    (O1.phase[q] < 0 ? (1u << 24) : 0u));
+        }
+      for (int nl = 0; nl < mc1; nl++) {
+        poff[((size_t)c * nd + k) * mc1 + nl] = (int64_t)ppk.size();
+        ppk.insert(ppk.end(), bucket[nl].begin(), bucket[nl].end());
+      }
+    }
+  poff[(size_t)N.nsec * nd * mc1] = (int64_t)ppk.size();
+  // the kernel reads off[base + nl + 1]; the next base starts where this one ends
+  if (ppk.empty()) ppk.push_back(0);
+  s->d_pn_off.upload(poff);
+  s->d_pn_pk.upload(ppk);
+  DensTables& d = s->dn;
+  d.pp_ptr = s->d_pp_ptr.p;
+  d.pp_src = s->d_pp_src.p;
+  d.pp_me = s->d_pp_me.p;
+  d.pp_ph = s->d_pp_ph.p;
+  d.nn_off = s->d_nn_off.p;
+  d.nn_pk = s->d_nn_pk.p;
+  d.nn_mc = s->d_nn_mc.p;
+  d.nn_ncomp = (int)s->nn_comp_to_me.size();
+  d.pn_off = s->d_pn_off.p;
+  d.pn_pk = s->d_pn_pk.p;
+  d.maxcls1 = mc1;
+  (void)P;
+  s->dens_ready = true;
+}
+
+template <typename T>
+void run_density(bs_system* s, const T* psi, double* rho_pp, double* rho_nn, double* rho_pn, double* norm2) {
+  if (!s->dens_ready) build_density_tables(s);
+  const bs::Tables& t = s->t;
+  const int64_t D = t.dim;
+  const int npx = t.npairxx, nd = t.ndelta;
+  DevBuf<double> rpp, rnn, rw, part, nrm;
+  rpp.alloc((size_t)std::max(npx, 1) * std::max(npx, 1));
+  rnn.alloc(std::max<size_t>(s->nn_comp_to_me.size(), 1));
+  rw.alloc(std::max<size_t>(t.wpn.size(), 1));
+  HIP_OK(hipMemset(rpp.p, 0, rpp.n * sizeof(double)));
+  HIP_OK(hipMemset(rnn.p, 0, rnn.n * sizeof(double)));
+  HIP_OK(hipMemset(rw.p, 0, rw.n * sizeof(double)));
+  DevTables d = s->dt;
+  const int nP = d.nP;
+  if (t.in.Z >= 2 && nP > 0) k_density_pp<T><<<nP, 256>>>(d, s->dn, psi, rpp.p);
+  if (t.in.N >= 2 && s->dn.nn_ncomp > 0) {
+    d.tile_f0 = s->tile_f0[2].p;
+    d.tile_nr = s->tile_nr[2].p;
+    const size_t lds = (size_t)s->dn.nn_ncomp * sizeof(double) + (size_t)4 * d.lmax * sizeof(T);
+    if (lds > kLdsLimit) throw std::runtime_error("NN density tables too large for LDS");
+    auto fn = k_density_nn<T, 4>;
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kLdsLimit));
+    hipLaunchKernelGGL(fn, dim3(s->ntiles[2]), dim3(256), lds, nullptr, d, s->dn, psi, rnn.p);
+  }
+  if (t.in.Z >= 1 && t.in.N >= 1 && nP > 0) {
+    const size_t lds = (size_t)5 * d.lmax * sizeof(T);
+    auto fn = k_density_pn<T, 4>;
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kLdsLimit));
+    hipLaunchKernelGGL(fn, dim3(nP), dim3(256), lds, nullptr, d, s->dn, psi, rw.p);
+  }
+  HIP_OK(hipGetLastError());
+  const int nb = std::min(grid_for(D), 1024);
+  part.alloc(nb);
+  nrm.alloc(1);
+  k_multidot<T><<<nb, kBlock>>>(psi, D, 1, psi, D, part.p);
+  k_reduce<<<1, kBlock>>>(part.p, nb, 1, nrm.p, 0);
+  HIP_OK(hipGetLastError());
+  double n2 = 0.0;
+  HIP_OK(hipMemcpy(&n2, nrm.p, sizeof(double), hipMemcpyDeviceToHost));
+  if (norm2) *norm2 = n2;
+  if (rho_pp) {
+    if (npx > 0) HIP_OK(hipMemcpy(rho_pp, rpp.p, sizeof(double) * (size_t)npx * npx, hipMemcpyDeviceToHost));
+  }
+  if (rho_nn) {
+    std::vector<double> comp(s->nn_comp_to_me.size());
+    if (!comp.empty()) HIP_OK(hipMemcpy(comp.data(), rnn.p, sizeof(double) * comp.size(), hipMemcpyDeviceToHost));
+    std::fill(rho_nn, rho_nn + (size_t)npx * npx, 0.0);
+    for (size_t i = 0; i < comp.size(); i++) rho_nn[s->nn_comp_to_me[i]] = comp[i];
+  }
+  if (rho_pn) {
+    std::vector<double> w(t.wpn.size());
+    if (!w.empty()) HIP_OK(hipMemcpy(w.data(), rw.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+    const int ns = t.nsps, npn = ns * ns;
+    std::fill(rho_pn, rho_pn + (size_t)npn * npn, 0.0);
+    for (int a = 0; a < ns; a++)
+      for (int c = 0; c < ns; c++) {
+        const int k = t.class_of_op[a * ns + c], pl = t.local_of_op[a * ns + c];
+        const int kn = negk_of(t, k), ws = t.class_size[kn];
+        for (int b = 0; b < ns; b++)
+          for (int dd = 0; dd < ns; dd++) {
+            if (t.class_of_op[b * ns + dd] != kn) continue;
+            const int nl = t.local_of_op[b * ns + dd];
+            rho_pn[(size_t)(a * ns + b) * npn + (c * ns + dd)] = w[t.class_offset[k] + pl * ws + nl];
+          }
+      }
+  }
+  (void)nd;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bs_last_error(void) { return g_err.c_str(); }
@@ -1301,6 +1600,84 @@ int bs_observable(bs_system* s, int which, const void* psi, int dtype, double* v
     }
     if (dtype == BS_F32) *value = expectation<float>(s->obs[which].get(), (const float*)psi);
     else *value = expectation<double>(s->obs[which].get(), (const double*)psi);
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+
+int bs_num_params(const bs_system* s, int* nspe, int* ntbme) {
+  if (!s) return set_err("null system");
+  if (nspe) *nspe = (int)s->t.in.orbits.size();
+  if (ntbme) *ntbme = (int)s->t.lines.size();
+  return 0;
+}
+
+int bs_density(bs_system* s, const void* psi, int dtype, double* rho_pp, double* rho_nn, double* rho_pn,
+               double* norm2) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (!psi) return set_err("null vector");
+    if (dtype == BS_F32) run_density<float>(s, (const float*)psi, rho_pp, rho_nn, rho_pn, norm2);
+    else if (dtype == BS_F64) run_density<double>(s, (const double*)psi, rho_pp, rho_nn, rho_pn, norm2);
+    else return set_err("dtype must be BS_F32 or BS_F64");
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_gradient(bs_system* s, const void* psi, int dtype, double* grad) {
+  try {
+    checked(s);
+    if (!grad) return set_err("null gradient");
+    const bs::Tables& t = s->t;
+    const size_t npx2 = (size_t)t.npairxx * t.npairxx, npn2 = (size_t)t.npairpn * t.npairpn;
+    std::vector<double> rho[3] = {std::vector<double>(npx2), std::vector<double>(npx2), std::vector<double>(npn2)};
+    double n2 = 0.0;
+    int rc = bs_density(s, psi, dtype, rho[0].data(), rho[1].data(), rho[2].data(), &n2);
+    if (rc) return rc;
+    if (!(n2 > 0.0)) return set_err("zero vector");
+    if (!s->jac_ready) {
+      s->jac = bs::param_jacobian(t, s->sps_path);
+      s->jac_ready = true;
+    }
+    const int norb = (int)t.in.orbits.size();
+    for (size_t p = 0; p < s->jac.size(); p++) {
+      double g = 0.0;
+      for (int k = 0; k < 3; k++)
+        for (size_t i = 0; i < s->jac[p].idx[k].size(); i++) g += (double)s->jac[p].coef[k][i] * rho[k][s->jac[p].idx[k][i]];
+      grad[p] = g / n2;
+    }
+    if (t.call_spe) {
+      // single particle: the SPEs act through the diagonal, dE/deps_a = spescale * <n_a>
+      const int64_t D = t.dim;
+      std::vector<double> h(D);
+      if (dtype == BS_F32) {
+        std::vector<float> f(D);
+        HIP_OK(hipMemcpy(f.data(), psi, sizeof(float) * D, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < D; i++) h[i] = f[i];
+      } else {
+        HIP_OK(hipMemcpy(h.data(), psi, sizeof(double) * D, hipMemcpyDeviceToHost));
+      }
+      std::vector<double> occ(norb, 0.0);
+      for (int it = 0; it < 2; it++) {
+        const bs::Species& S = t.sp[it];
+        const bs::Species& C = t.sp[1 - it];
+        for (size_t sd = 0; sd < S.mask.size(); sd++) {
+          double w = 0.0;
+          const int cs = S.conj[S.sdsec[sd]];
+          for (int64_t o = C.secfirst[cs]; o < C.secfirst[cs] + C.seccount[cs]; o++) {
+            const int64_t st = it == 0 ? S.start[sd] + C.start[o] : C.start[o] + S.start[sd];
+            w += h[st] * h[st];
+          }
+          for (uint64_t q = S.mask[sd]; q; q &= q - 1) occ[t.sorb[__builtin_ctzll(q)]] += w;
+        }
+      }
+      for (int a = 0; a < norb; a++) grad[a] += (double)t.in.spscale * occ[a] / n2;
+    }
     return 0;
   } catch (const std::exception& e) {
     return set_err(e.what());

@@ -195,8 +195,11 @@ struct Builder {
     const float spscale = t.in.spscale, vscale = t.in.vscale;
     t.pspe.assign(norb, 0.0f);
     t.nspe.assign(norb, 0.0f);
+    t.spe_raw.assign(norb, 0.0f);
+    t.lines.clear();
     for (int i = 0; i < norb; i++) {
       float e = std::strtof(tok[1 + i].c_str(), nullptr);
+      t.spe_raw[i] = e;
       t.pspe[i] = t.pspe[i] + spscale * e;
       t.nspe[i] = t.nspe[i] + spscale * e;
     }
@@ -211,6 +214,7 @@ struct Builder {
       int ic = std::atoi(tok[p + 2].c_str()), id = std::atoi(tok[p + 3].c_str());
       int J = std::atoi(tok[p + 4].c_str()), T = std::atoi(tok[p + 5].c_str());
       float vv = std::strtof(tok[p + 6].c_str(), nullptr);
+      t.lines.push_back({ia, ib, ic, id, J, T, vv});
       if (T != 0 && T != 1) fail(".int: bad T value");
       if (std::min({ia, ib, ic, id}) < 1 || std::max({ia, ib, ic, id}) > norb) fail(".int: orbit label out of range");
       if ((ia == ib || ic == id) && (J + T) % 2 == 0 && vv != 0.0f) fail(".int: illegal matrix element for identical orbits");
@@ -882,6 +886,42 @@ Tables build_any(const std::string& sps, const std::string* intfile, const std::
 }
 
 }  // namespace
+
+std::vector<ParamColumn> param_jacobian(const Tables& t0, const std::string& sps) {
+  const int norb = (int)t0.in.orbits.size();
+  const int np = norb + (int)t0.lines.size();
+  std::vector<ParamColumn> cols(np);
+  for (int p = 0; p < np; p++) {
+    std::ostringstream os;
+    os << t0.lines.size();
+    for (int a = 0; a < norb; a++) os << (p == a ? " 1" : " 0");
+    os << "\n";
+    for (size_t q = 0; q < t0.lines.size(); q++) {
+      const TbmeLine& L = t0.lines[q];
+      os << L.a << " " << L.b << " " << L.c << " " << L.d << " " << L.J << " " << L.T << " "
+         << ((int)q + norb == p ? "1" : "0") << "\n";
+    }
+    Tables t;
+    t.in = t0.in;
+    (void)sps;
+    Builder b(t);
+    std::istringstream is(os.str());
+    b.read_int_stream(is);
+    b.subsume();
+    b.make_sps();
+    b.uncouple_xx(0);
+    b.uncouple_xx(1);
+    b.uncouple_pn();
+    const std::vector<float>* tab[3] = {&t.vpp, &t.vnn, &t.vpn};
+    for (int k = 0; k < 3; k++)
+      for (size_t i = 0; i < tab[k]->size(); i++)
+        if ((*tab[k])[i] != 0.0f) {
+          cols[p].idx[k].push_back((int32_t)i);
+          cols[p].coef[k].push_back((*tab[k])[i]);
+        }
+  }
+  return cols;
+}
 
 void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out) {
   const Species& S = t.sp[it];

@@ -70,8 +70,20 @@ struct TwoBodyOps {
   std::vector<int8_t> phase;
 };
 
+// One TBME line of the .int file, in file order (a b c d J T V).
+struct TbmeLine { int a, b, c, d, J, T; float v; };
+
+// d(uncoupled element)/d(parameter) for one parameter: sparse lists into the
+// dense pp / nn / pn element tables.
+struct ParamColumn {
+  std::vector<int32_t> idx[3];
+  std::vector<float> coef[3];
+};
+
 struct Tables {
   ModelInput in;
+  std::vector<float> spe_raw;          // SPEs as read (before spescale)
+  std::vector<TbmeLine> lines;         // TBME lines as read
   // single-particle states, combined label order (left m<0 first, then m>=0)
   int nsps = 0, nL = 0, nR = 0;
   std::vector<int> sorb, sj2, sm2;
@@ -110,6 +122,11 @@ struct Tables {
 void read_sps(const std::string& path, ModelInput& in);
 Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps = true);
+// Jacobian of the FP32 uncoupled elements with respect to the .int parameters
+// (SPEs first, then TBME lines in file order): built from unit interactions
+// through the same reading, subsumption and uncoupling steps.
+std::vector<ParamColumn> param_jacobian(const Tables& t, const std::string& sps);
+
 // Same as build_tables with the interaction given as the text of an iso .int
 // file (scaling 1 1 1 1).
 Tables build_tables_text(const std::string& sps, const std::string& inttext, int Z, int N, int jz2,

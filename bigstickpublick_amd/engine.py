@@ -52,6 +52,9 @@ def load_library():
     L.bs_autotune.argtypes = [vp, i32, ctypes.POINTER(i32)]
     L.bs_autotune_times.argtypes = [vp, i32, vp]
     L.bs_apply_h_host.argtypes = [vp, vp, vp, i32, i32]
+    L.bs_num_params.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.bs_density.argtypes = [vp, vp, i32, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+    L.bs_gradient.argtypes = [vp, vp, i32, vp]
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
@@ -200,6 +203,34 @@ class System:
         _check(load_library().bs_observable(self.h, 0 if which == "J2" else 1,
                                             ctypes.c_void_p(psi.data_ptr()), dt, ctypes.byref(v)))
         return v.value
+
+    def num_params(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(load_library().bs_num_params(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def density(self, psi):
+        """(rho_pp, rho_nn, rho_pn, <psi|psi>) for a CUDA vector psi."""
+        self.upload()
+        npx = self.me_table(0).shape[0]
+        npn = self.me_table(2).shape[0]
+        rpp = np.zeros((npx, npx))
+        rnn = np.zeros((npx, npx))
+        rpn = np.zeros((npn, npn))
+        n2 = ctypes.c_double()
+        dt = BS_F32 if psi.dtype.itemsize == 4 else BS_F64
+        _check(load_library().bs_density(self.h, ctypes.c_void_p(psi.data_ptr()), dt, rpp.ctypes.data,
+                                         rnn.ctypes.data, rpn.ctypes.data, ctypes.byref(n2)))
+        return rpp, rnn, rpn, n2.value
+
+    def gradient(self, psi):
+        """dE/dtheta for theta = (SPEs, TBME lines in .int order), psi a CUDA vector."""
+        self.upload()
+        nspe, ntb = self.num_params()
+        g = np.zeros(nspe + ntb)
+        dt = BS_F32 if psi.dtype.itemsize == 4 else BS_F64
+        _check(load_library().bs_gradient(self.h, ctypes.c_void_p(psi.data_ptr()), dt, g.ctypes.data))
+        return g
 
     def autotune_times(self, dtype="f32"):
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64

@@ -115,6 +115,26 @@ int bs_apply_h_host(bs_system *sys, const void *x, void *y, int dtype, int accum
  * sum_{i<j} 2 j_i.j_j (and likewise T^2) and applied with the same kernel. */
 int bs_observable(bs_system *sys, int which, const void *psi, int dtype, double *value);
 
+/* Parameters of the interaction: nspe single-particle energies (one per
+ * orbit) followed by ntbme two-body matrix elements, in .int file order. */
+int bs_num_params(const bs_system *sys, int *nspe, int *ntbme);
+
+/* Adjoint pass over the same jump tables as bs_apply_h (the analogue of
+ * applyh{PP,NN,PN}bundled_den, src/bdenslib4.f90:129-725):
+ * rho[me] = <psi| O_me |psi> for every uncoupled operator, with
+ * H = sum_me v_me O_me and v_me the dense tables of bs_me_table.  psi is a
+ * device vector (dtype, dim); rho_* are host FP64 arrays of npair*npair
+ * (any may be NULL); norm2 (may be NULL) receives <psi|psi>.  Not normalized. */
+int bs_density(bs_system *sys, const void *psi, int dtype, double *rho_pp, double *rho_nn,
+               double *rho_pn, double *norm2);
+
+/* Hellmann-Feynman gradient dE/dtheta = <psi|dH/dtheta|psi>/<psi|psi> for all
+ * interaction parameters (grad has nspe + ntbme entries, order of
+ * bs_num_params), from bs_density and the Jacobian of the FP32 elements with
+ * respect to the .int values (built through the same reading, subsumption and
+ * uncoupling steps; the scaling line is included). */
+int bs_gradient(bs_system *sys, const void *psi, int dtype, double *grad);
+
 typedef struct {
   int nkeep;        /* eigenpairs wanted */
   int niter;        /* Lanczos iterations (plain) or restart length (thick) */

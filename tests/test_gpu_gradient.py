@@ -1,0 +1,86 @@
+"""Hellmann-Feynman gradients from the adjoint pass against central finite
+differences (h = 1e-4 MeV) of FP64 eigenvalues."""
+import os
+
+import numpy as np
+import pytest
+
+from bigstickpublick_amd.inputs import f32, small_case, write_case
+from golden_cases import config_fixture, paths
+from oracle_api import Oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _energy_identity(s, psi):
+    """sum_me v_me rho_me must equal <psi|H|psi> (all three operator classes)."""
+    rpp, rnn, rpn, n2 = s.density(psi)
+    e = ((s.me_table(0) * rpp).sum() + (s.me_table(1) * rnn).sum() + (s.me_table(2) * rpn).sum()) / n2
+    hx = s.apply(psi)
+    return e, float((psi.double() @ hx.double()).item() / n2)
+
+
+def test_density_reproduces_energy(casedir):
+    case, d, fx = config_fixture(0, casedir)
+    from bigstickpublick_amd import System
+    s = System(*paths(case, d), case.Z, case.N, case.jz2, upload=True)
+    psi = torch.tensor(np.random.default_rng(4).standard_normal(s.dim), device="cuda")
+    e1, e2 = _energy_identity(s, psi)
+    assert abs(e1 - e2) < 1e-9 * max(1.0, abs(e2))
+    e1, e2 = _energy_identity(s, psi.float())
+    assert abs(e1 - e2) < 1e-4 * max(1.0, abs(e2))
+
+
+def _oracle_e0(case, d, theta, k=0):
+    norb = len(case.orbits)
+    it = case.interaction
+    case.interaction = type(it)(it.orbits, tuple(theta[:norb]), [l[:6] + (v,) for l, v in zip(it.tbmes, theta[norb:])])
+    sps, intf = write_case(case, d)
+    case.interaction = it
+    o = Oracle(os.path.join(d, sps + ".sps"), os.path.join(d, intf + ".int"), case.Z, case.N, case.jz2)
+    return np.linalg.eigvalsh(o.dense())[k]
+
+
+@pytest.mark.parametrize("space,Z,N,jz2", [("sd", 2, 2, 0), ("sd", 1, 2, 1), ("pf", 2, 1, 3), ("sd", 1, 0, 1)])
+def test_gradient_vs_finite_differences(space, Z, N, jz2, casedir):
+    from bigstickpublick_amd import System
+    case = small_case(space, Z, N, jz2, seed=91)
+    d = os.path.join(casedir, "grad_%s%d%d" % (space, Z, N))
+    write_case(case, d)
+    s = System(*paths(case, d), Z, N, jz2, upload=True)
+    r = s.lanczos(nkeep=1, niter=min(s.dim, 200), dtype="f64", polish=True, want_vectors=True)
+    g = s.gradient(r.evecs[0])
+    theta = list(case.interaction.spe) + [t[6] for t in case.interaction.tbmes]
+    nspe, ntb = s.num_params()
+    assert len(g) == nspe + ntb == len(theta)
+    rng = np.random.default_rng(0)
+    pick = list(range(nspe)) + sorted(rng.choice(np.arange(nspe, len(theta)), size=min(12, ntb), replace=False))
+    for p in pick:
+        tp, tm = list(theta), list(theta)
+        tp[p] = f32(theta[p] + 1e-4)
+        tm[p] = f32(theta[p] - 1e-4)
+        fd = (_oracle_e0(case, d, tp) - _oracle_e0(case, d, tm)) / (tp[p] - tm[p])
+        assert abs(fd - g[p]) < 2e-5, (p, fd, g[p])
+
+
+def test_autograd_function_config0(casedir):
+    """torch.autograd through EigenEnergies: the gradient of E0 + E1 equals the
+    sum of the adjoint gradients and matches finite differences of the GPU
+    FP64 energies for a few parameters."""
+    from bigstickpublick_amd.autograd import eigenenergies, parameters
+    case, d, fx = config_fixture(0, casedir)
+    theta = parameters(case.interaction).requires_grad_(True)
+    e = eigenenergies(theta, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
+    assert np.abs(e.detach().numpy() - fx["res"][:2, 0]).max() < 2e-5
+    (e[0] + e[1]).backward()
+    g = theta.grad.numpy()
+    for p in (0, 2, 5, 40):
+        tp = theta.detach().clone()
+        tm = theta.detach().clone()
+        tp[p] = f32(float(tp[p]) + 1e-4)
+        tm[p] = f32(float(tm[p]) - 1e-4)
+        ep = eigenenergies(tp, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
+        em = eigenenergies(tm, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
+        fd = float((ep.sum() - em.sum()) / (tp[p] - tm[p]))
+        assert abs(fd - g[p]) < 1e-4, (p, fd, g[p])
