@@ -517,6 +517,258 @@ __global__ __launch_bounds__(256) void k_density_pn(DevTables tb, DensTables dn,
   }
 }
 
+// ------------------------------------------------------ batched Hamiltonians
+// K Hamiltonians on one basis; vectors interleaved X[state*K + k].  One wave
+// per output basis state, lane = Hamiltonian: every jump-table read (uniform
+// across the wave, so scalar) serves all K vectors, and the x and element
+// loads are K-contiguous.
+struct BatchVals {
+  int K;
+  const float* pp32;   // [merged PP entry][K]
+  const double* pp64;
+  const float* nn32;   // [NN ELL slot][K]
+  const double* nn64;
+  const float* W;      // [pn block element][K]
+  const float* pspe;   // [proton SD][K]
+  const float* nspe;   // [neutron SD][K]
+};
+
+template <typename T> __device__ __forceinline__ T bval(const float* a, const double* b, int64_t i);
+template <> __device__ __forceinline__ float bval<float>(const float* a, const double*, int64_t i) { return a[i]; }
+template <> __device__ __forceinline__ double bval<double>(const float*, const double* b, int64_t i) { return b[i]; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals bv, const T* __restrict__ x,
+                                                       T* __restrict__ y) {
+  const int K = bv.K;
+  const int k = threadIdx.x & 63;
+  const int f = blockIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * 4 + (threadIdx.x >> 6)));
+  const int sp = tb.psec[f];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  if (c >= L) return;
+  const bool on = k < K;
+  const int kk = on ? k : 0;
+  const int64_t bf = tb.pstart[f];
+  const int nf0 = tb.nfirst[cn];
+  T acc = T(0);
+  if (tb.call_spe)
+    acc += ((T)bv.pspe[(int64_t)f * K + kk] + (T)bv.nspe[(int64_t)(nf0 + c) * K + kk]) * x[(bf + c) * K + kk];
+  for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
+    const int64_t bi = tb.pstart[tb.pp_src[e]];
+    acc += bval<T>(bv.pp32, bv.pp64, e * K + kk) * x[(bi + c) * K + kk];
+  }
+  {
+    const int64_t base = tb.nn_base[cn];
+    const int E = tb.nn_emax[cn];
+    for (int e = 0; e < E; e++) {
+      const int64_t o = base + (int64_t)e * L + c;
+      acc += bval<T>(bv.nn32, bv.nn64, o * K + kk) * x[(bf + tb.nn_src[o]) * K + kk];
+    }
+  }
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int cl = 0; cl < nd; cl++) {
+    const int p0 = tb.pcls[(int64_t)f * nd1 + cl], p1 = tb.pcls[(int64_t)f * nd1 + cl + 1];
+    if (p0 == p1) continue;
+    const int kn = tb.negk[cl];
+    const int ws = tb.wstride[cl];
+    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
+    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
+    for (int pj = p0; pj < p1; pj++) {
+      const int64_t bi = tb.pstart[tb.p1_src[pj]];
+      const int meta = tb.p1_meta[pj];
+      const int64_t wrow = tb.woff[cl] + (int64_t)(meta & 0xff) * ws;
+      T a = T(0);
+      for (int e = 0; e < E; e++) {
+        const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
+        const int nl = (pk >> 16) & 0xff;
+        if (nl >= ws) continue;  // padding slot
+        const T t = (T)bv.W[(wrow + nl) * K + kk] * x[(bi + (pk & 0xffff)) * K + kk];
+        a += ((pk >> 24) & 1) ? -t : t;
+      }
+      acc += (meta >> 8) ? -a : a;
+    }
+  }
+  if (on) y[(bf + c) * K + k] = acc;
+}
+
+// part[b*m*K + j*K + k] = partial sum over this block's states of V_j[s][k] * w[s][k]
+template <typename T>
+__global__ __launch_bounds__(256) void k_bmultidot(const T* __restrict__ V, int64_t ldv, int m, const T* __restrict__ w,
+                                                   int64_t D, int K, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int k = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t chunk = (D + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, D);
+  for (int j = 0; j < m; j++) {
+    double a = 0.0;
+    if (k < K)
+      for (int64_t st = lo + wv; st < hi; st += 4) a += (double)V[(int64_t)j * ldv + st * K + k] * (double)w[st * K + k];
+    red[wv][k] = a;
+    __syncthreads();
+    if (wv == 0 && k < K)
+      part[((int64_t)blockIdx.x * m + j) * K + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    __syncthreads();
+  }
+}
+
+// out[j*K + k] = sum_b part[...]; mode 1: sqrt
+__global__ void k_breduce(const double* __restrict__ part, int nb, int m, int K, double* __restrict__ out, int mode) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * K) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(int64_t)b * m * K + idx];
+  out[idx] = mode == 1 ? sqrt(s) : s;
+}
+
+template <typename T>
+__global__ void k_bsub_scaled(T* __restrict__ w, const T* __restrict__ v, const double* __restrict__ a, int64_t D, int K) {
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] - (T)(a[i % K] * (double)v[i]);
+}
+
+template <typename T>
+__global__ void k_bmultiaxpy(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m, const double* __restrict__ d,
+                             int64_t D, int K) {
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % K);
+    double s = 0.0;
+    for (int j = 0; j < m; j++) s += d[j * K + k] * (double)V[(int64_t)j * ldv + i];
+    w[i] = w[i] - (T)s;
+  }
+}
+
+template <typename T>
+__global__ void k_bnormalize(T* __restrict__ v, const T* __restrict__ w, const double* __restrict__ beta, int64_t D, int K) {
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (T)((double)w[i] * (1.0 / beta[i % K]));
+}
+
+// Lowest nkeep eigenpairs of K symmetric tridiagonal matrices (alpha[i*K+k],
+// beta[i*K+k] = coupling of i and i+1), FP64, one thread per matrix: Sturm
+// bisection for the eigenvalues, inverse iteration (tridiagonal LU with
+// partial pivoting) for the vectors.  Y[(k*m + i)*nkeep + n].
+__global__ void k_tridiag_ritz(const double* __restrict__ alpha, const double* __restrict__ beta, int m, int K,
+                               int nkeep, double* __restrict__ evals, double* __restrict__ Y, double* __restrict__ work) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  auto a = [&](int i) { return alpha[(int64_t)i * K + k]; };
+  auto b = [&](int i) { return beta[(int64_t)i * K + k]; };
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; i++) {
+    const double r = (i > 0 ? fabs(b(i - 1)) : 0.0) + (i + 1 < m ? fabs(b(i)) : 0.0);
+    lo = fmin(lo, a(i) - r);
+    hi = fmax(hi, a(i) + r);
+  }
+  double* wk = work + (int64_t)k * 6 * m;
+  double* dd = wk;         // LU diagonal
+  double* du = wk + m;     // first superdiagonal
+  double* du2 = wk + 2 * m;  // second superdiagonal (pivoting)
+  double* dl = wk + 3 * m;   // multipliers
+  int* piv = (int*)(wk + 4 * m);
+  double* v = wk + 5 * m;
+  for (int n = 0; n < nkeep; n++) {
+    // count of eigenvalues below x (Sturm sequence)
+    double l = lo, h = hi;
+    for (int it = 0; it < 200 && h - l > 1e-15 * fmax(1.0, fabs(l) + fabs(h)); it++) {
+      const double x = 0.5 * (l + h);
+      int cnt = 0;
+      double q = 1.0;
+      for (int i = 0; i < m; i++) {
+        const double bb = i > 0 ? b(i - 1) : 0.0;
+        q = a(i) - x - (i > 0 ? bb * bb / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0.0) cnt++;
+      }
+      if (cnt > n) h = x;
+      else l = x;
+    }
+    const double lam = 0.5 * (l + h);
+    evals[(int64_t)k * nkeep + n] = lam;
+    // LU of (T - lam) with partial pivoting (tridiagonal -> bandwidth 2)
+    const double shift = lam + 1e-13 * fmax(1.0, fabs(lam));
+    for (int i = 0; i < m; i++) {
+      dd[i] = a(i) - shift;
+      du[i] = i + 1 < m ? b(i) : 0.0;
+      du2[i] = 0.0;
+    }
+    double sub = m > 1 ? b(0) : 0.0;
+    for (int i = 0; i < m - 1; i++) {
+      const double nxt_sub = i + 2 < m ? b(i + 1) : 0.0;
+      if (fabs(dd[i]) >= fabs(sub)) {
+        piv[i] = 0;
+        const double mult = sub / (dd[i] == 0.0 ? 1e-300 : dd[i]);
+        dl[i] = mult;
+        dd[i + 1] -= mult * du[i];
+      } else {
+        piv[i] = 1;
+        const double mult = dd[i] / sub;
+        dl[i] = mult;
+        dd[i] = sub;
+        double t0 = du[i];
+        du[i] = dd[i + 1];
+        dd[i + 1] = t0 - mult * du[i];
+        du2[i] = du[i + 1];
+        du[i + 1] = -mult * du[i + 1];
+      }
+      sub = nxt_sub;
+    }
+    for (int i = 0; i < m; i++) v[i] = 1.0 / sqrt((double)m) * (1.0 + 0.01 * (double)((i * 7919) % 13));
+    for (int rep = 0; rep < 4; rep++) {
+      // forward: apply the row operations
+      for (int i = 0; i < m - 1; i++) {
+        if (piv[i]) {
+          const double t0 = v[i];
+          v[i] = v[i + 1];
+          v[i + 1] = t0 - dl[i] * v[i];
+        } else {
+          v[i + 1] -= dl[i] * v[i];
+        }
+      }
+      // back substitution
+      for (int i = m - 1; i >= 0; i--) {
+        double t0 = v[i];
+        if (i + 1 < m) t0 -= du[i] * v[i + 1];
+        if (i + 2 < m) t0 -= du2[i] * v[i + 2];
+        v[i] = t0 / (dd[i] == 0.0 ? 1e-300 : dd[i]);
+      }
+      // orthogonalize against the previous vectors of this matrix, normalize
+      for (int p = 0; p < n; p++) {
+        double dot = 0.0;
+        for (int i = 0; i < m; i++) dot += v[i] * Y[((int64_t)k * m + i) * nkeep + p];
+        for (int i = 0; i < m; i++) v[i] -= dot * Y[((int64_t)k * m + i) * nkeep + p];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < m; i++) nrm += v[i] * v[i];
+      nrm = 1.0 / sqrt(nrm);
+      for (int i = 0; i < m; i++) v[i] *= nrm;
+    }
+    for (int i = 0; i < m; i++) Y[((int64_t)k * m + i) * nkeep + n] = v[i];
+  }
+}
+
+// X_n[s][k] = sum_j Y[(k*m + j)*nkeep + n] V_j[s][k]
+template <typename T, typename U>
+__global__ void k_bcombine(U* __restrict__ X, const T* __restrict__ V, int64_t ldv, int m, const double* __restrict__ Y,
+                           int nkeep, int64_t D, int K) {
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % K);
+    double acc[16];
+    for (int q = 0; q < nkeep; q++) acc[q] = 0.0;
+    for (int j = 0; j < m; j++) {
+      const double vj = (double)V[(int64_t)j * ldv + i];
+      const double* yk = Y + ((int64_t)k * m + j) * nkeep;
+      for (int q = 0; q < nkeep; q++) acc[q] += yk[q] * vj;
+    }
+    for (int q = 0; q < nkeep; q++) X[(int64_t)q * n + i] = (U)acc[q];
+  }
+}
+
 // ---------------------------------------------------------------- helpers
 template <typename T>
 struct DevBuf {
@@ -602,6 +854,7 @@ struct bs_system {
   std::string sps_path;
   std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
   DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
+  std::vector<int64_t> nn_slot;        // ELL slot of each merged NN entry (batched mode)
   // adjoint tables (built on first bs_density)
   bool dens_ready = false;
   DensTables dn{};
@@ -695,6 +948,7 @@ void upload_tables(bs_system* s) {
     tot += (int64_t)em * N.seccount[c];
   }
   std::vector<int32_t> nn_src(std::max<int64_t>(tot, 1), 0);
+  s->nn_slot.assign(t.tb[1].src.size(), -1);
   std::vector<float> nnv32(std::max<int64_t>(tot, 1), 0.0f);
   std::vector<double> nnv64(std::max<int64_t>(tot, 1), 0.0);
   for (int c = 0; c < N.nsec; c++) {
@@ -706,6 +960,7 @@ void upload_tables(bs_system* s) {
         int32_t src = t.tb[1].src[q];
         if (N.sdsec[src] != c) throw std::runtime_error("NN jump leaves its sector");
         const int64_t o = nn_base[c] + (int64_t)e * L + col;
+        s->nn_slot[q] = o;
         nn_src[o] = (int32_t)(src - N.secfirst[c]);
         nnv64[o] = t.tb[1].val[q];
         nnv32[o] = (float)t.tb[1].val[q];

@@ -719,7 +719,7 @@ struct Builder {
                 uint64_t sm = m2 | (1ULL << k) | (1ULL << l);
                 // the reference stores and counts jumps whatever their element value
                 t.tb_unmerged[it][f] += (sm == fm) ? 2 : 1;
-                if (me == 0.0f) continue;
+                if (me == 0.0f && !t.keep_structural) continue;
                 // sign of a+_i a+_j a_l a_k acting on |s>
                 int sg = fsign(sm, k);
                 uint64_t s1 = sm & ~(1ULL << k);
@@ -742,7 +742,7 @@ struct Builder {
           size_t r = q;
           double s = 0.0;
           while (r < acc.size() && acc[r].first == acc[q].first) { s += acc[r].second; r++; }
-          if (s != 0.0) { T.src.push_back(acc[q].first); T.val.push_back(s); }
+          if (s != 0.0 || t.keep_structural) { T.src.push_back(acc[q].first); T.val.push_back(s); }
           q = r;
         }
         T.rowptr[f + 1] = (int64_t)T.src.size();
@@ -799,7 +799,12 @@ struct Builder {
 
 namespace {
 Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
-                 int jz2, float spscale, float a0, float a, float x, bool with_jumps);
+                 int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural = false);
+}
+
+Tables build_tables_structural(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
+                               float spscale, float a0, float a, float x) {
+  return build_any(sps, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, true, true);
 }
 
 Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
@@ -845,8 +850,9 @@ std::string observable_int_text(const std::vector<Orbit>& orbits, int which) {
 
 namespace {
 Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
-                 int jz2, float spscale, float a0, float a, float x, bool with_jumps) {
+                 int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural) {
   Tables t;
+  t.keep_structural = structural;
   read_sps(sps, t.in);
   for (auto& o : t.in.orbits) {
     if (o.w != t.in.orbits[0].w) fail("W truncation / mixed-W spaces are not supported");
@@ -886,6 +892,87 @@ Tables build_any(const std::string& sps, const std::string* intfile, const std::
 }
 
 }  // namespace
+
+MeTables build_me_tables(const Tables& base, const std::string& intfile) {
+  Tables t;
+  t.in = base.in;
+  Builder b(t);
+  b.read_int(intfile);
+  if ((int)t.pspe.size() != (int)base.in.orbits.size()) fail("interaction has a different number of orbits");
+  b.subsume();
+  b.make_sps();
+  b.uncouple_xx(0);
+  b.uncouple_xx(1);
+  b.uncouple_pn();
+  MeTables m;
+  m.vpp.swap(t.vpp);
+  m.vnn.swap(t.vnn);
+  m.vpn.swap(t.vpn);
+  m.pspe = t.pspe;
+  m.nspe = t.nspe;
+  m.call_spe = t.call_spe;
+  // pn elements in base's class layout
+  const int ns = base.nsps, npn = ns * ns;
+  m.wpn.assign(base.wpn.size(), 0.0f);
+  for (int a = 0; a < ns; a++)
+    for (int c = 0; c < ns; c++) {
+      const int k = base.class_of_op[a * ns + c], pl = base.local_of_op[a * ns + c];
+      const int kn = (int)(std::lower_bound(base.delta_of_class.begin(), base.delta_of_class.end(),
+                                            -base.delta_of_class[k]) - base.delta_of_class.begin());
+      for (int bb = 0; bb < ns; bb++)
+        for (int d = 0; d < ns; d++) {
+          if (base.class_of_op[bb * ns + d] != kn) continue;
+          const int nl = base.local_of_op[bb * ns + d];
+          m.wpn[base.class_offset[k] + pl * base.class_size[kn] + nl] = m.vpn[(size_t)(a * ns + bb) * npn + (c * ns + d)];
+        }
+    }
+  return m;
+}
+
+std::vector<double> merged_values(const Tables& t, int it, const std::vector<float>& v) {
+  if (!t.keep_structural) fail("merged_values needs structural tables");
+  const Species& S = t.sp[it];
+  const TwoBody& T = t.tb[it];
+  std::vector<double> out(T.src.size(), 0.0);
+  if (S.np < 2) return out;
+  std::unordered_map<uint64_t, int32_t> idx;
+  for (size_t i = 0; i < S.mask.size(); i++) idx[S.mask[i]] = (int32_t)i;
+  const int ns = t.nsps, np = t.npairxx;
+  for (int64_t f = 0; f < (int64_t)S.mask.size(); f++) {
+    const uint64_t fm = S.mask[f];
+    const int64_t r0 = T.rowptr[f], r1 = T.rowptr[f + 1];
+    for (uint64_t ti = fm; ti; ti &= ti - 1) {
+      const int i = __builtin_ctzll(ti);
+      for (uint64_t tj = fm & ((1ULL << i) - 1); tj; tj &= tj - 1) {
+        const int j = __builtin_ctzll(tj);
+        const uint64_t m2 = fm & ~(1ULL << i) & ~(1ULL << j);
+        const int mij = t.sm2[i] + t.sm2[j];
+        const int cidx = i * (i - 1) / 2 + j;
+        for (int k = 1; k < ns; k++) {
+          if ((m2 >> k) & 1) continue;
+          for (int l = 0; l < k; l++) {
+            if ((m2 >> l) & 1) continue;
+            if (t.sm2[k] + t.sm2[l] != mij) continue;
+            const uint64_t sm = m2 | (1ULL << k) | (1ULL << l);
+            auto itr = idx.find(sm);
+            if (itr == idx.end()) continue;
+            int sg = Builder::fsign(sm, k);
+            const uint64_t s1 = sm & ~(1ULL << k);
+            sg *= Builder::fsign(s1, l);
+            const uint64_t s2 = s1 & ~(1ULL << l);
+            sg *= Builder::fsign(s2, j);
+            sg *= Builder::fsign(s2 | (1ULL << j), i);
+            const int32_t src = itr->second;
+            const auto pos = std::lower_bound(T.src.begin() + r0, T.src.begin() + r1, src);
+            if (pos == T.src.begin() + r1 || *pos != src) fail("structural two-body entry missing");
+            out[pos - T.src.begin()] += (double)v[(size_t)cidx * np + (k * (k - 1) / 2 + l)] * (double)sg;
+          }
+        }
+      }
+    }
+  }
+  return out;
+}
 
 std::vector<ParamColumn> param_jacobian(const Tables& t0, const std::string& sps) {
   const int norb = (int)t0.in.orbits.size();

@@ -115,6 +115,9 @@ struct Tables {
   // the reference's own operation count (src/bigstick_main.f90:1150-1192):
   // two-body operator applications with diagonal pair terms counted twice
   int64_t ref_pp = 0, ref_nn = 0;
+  // keep every structurally present two-body (target, source) pair, whatever
+  // its summed value (needed when several interactions share the tables)
+  bool keep_structural = false;
   std::vector<int64_t> tb_unmerged[2];  // per SD: unmerged two-body ops into it (+ diagonal again)
 };
 
@@ -122,6 +125,20 @@ struct Tables {
 void read_sps(const std::string& path, ModelInput& in);
 Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps = true);
+// Element tables of another interaction on the same orbits, particle numbers
+// and scaling (no basis, no jumps), with the pn elements in the Delta-class
+// block layout of `base`.
+struct MeTables {
+  std::vector<float> vpp, vnn, vpn, wpn, pspe, nspe;
+  bool call_spe = false;
+};
+MeTables build_me_tables(const Tables& base, const std::string& intfile);
+// Merged two-body values of element table v (dense pair-pair) in the entry
+// order of base.tb[it]; base must have been built with keep_structural.
+std::vector<double> merged_values(const Tables& base, int it, const std::vector<float>& v);
+Tables build_tables_structural(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
+                               float spscale, float a0, float a, float x);
+
 // Jacobian of the FP32 uncoupled elements with respect to the .int parameters
 // (SPEs first, then TBME lines in file order): built from unit interactions
 // through the same reading, subsumption and uncoupling steps.
