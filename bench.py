@@ -34,6 +34,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", type=int, default=4, help="benchmark configuration (4 = pf 6p6n)")
+    ap.add_argument("--solve", action="store_true",
+                    help="also time the whole eigensolve (thick-restart Lanczos, keep 10, restart at 40, "
+                         "lowest 5, FP64 polish) in FP64 and in mixed precision, and report eigenpairs/s "
+                         "at |E - E_FP64| <= 1e-4 MeV")
+    ap.add_argument("--max-restarts", type=int, default=40)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, copy x into y once (a plain copy kernel of 8*D bytes) "
                          "so hardware-counter runs can calibrate FETCH_SIZE/WRITE_SIZE")
@@ -92,6 +97,40 @@ def cpu_baseline(sps, intf, case, target_s=15.0):
             "sample": "factorized FP32 OpenMP port of the reference apply (oracle/bsoracle.c) on "
                       "proton rows %d..%d of %d = %.2f%% of the %d updates of one H x, %.2f s; "
                       "scaled by the update fraction" % (lo, hi2, p.nP, 100 * frac, total, dt)}
+
+
+def solve_leg(s, case, max_restarts):
+    """Eigenpairs/s of the whole solve (tables resident, Lanczos, restarts,
+    polish) for the plain FP64 path and the mixed path (FP32 vectors and
+    elements, FP64 alpha/beta, reorthogonalization, Ritz solve and polish).
+    The FP64 run, stopped when no Ritz value moves by 1e-7 MeV over a restart,
+    is the reference the mixed energies are scored against."""
+    import numpy as np
+    import torch
+    thick, niter, nkeep = case.thick, case.niter, case.nkeep
+    out = {"nkeep": nkeep, "niter": niter, "thick": thick}
+    runs = {}
+    for dt, tol in (("f64", 1e-7), ("f32", 1e-5)):
+        s.autotune(dt)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        r = s.lanczos(nkeep=nkeep, niter=niter, thick=thick, max_restarts=max_restarts, dtype=dt,
+                      tol=tol, polish=True)
+        torch.cuda.synchronize()
+        runs[dt] = (r, time.time() - t0)
+    r64, t64 = runs["f64"]
+    r32, t32 = runs["f32"]
+    err = np.abs(r32.evals - r64.evals)
+    ok32 = int((err <= 1e-4).sum())
+    out.update({
+        "fp64": {"seconds": t64, "matvecs": r64.matvecs, "evals": r64.evals.tolist(),
+                 "resid": r64.resid.tolist(), "eigenpairs_per_s": nkeep / t64},
+        "mixed": {"seconds": t32, "matvecs": r32.matvecs, "evals": r32.evals.tolist(),
+                  "resid": r32.resid.tolist(), "abs_err_vs_fp64": err.tolist(),
+                  "within_1e-4": ok32, "eigenpairs_per_s": ok32 / t32},
+        "speedup_mixed_vs_fp64": (ok32 / t32) / (nkeep / t64),
+    })
+    return out
 
 
 def main():
@@ -177,6 +216,11 @@ def main():
                            "staged_rows": System.variants()[variant][1],
                            "xcd_map": System.variants()[variant][2]},
     }
+    if a.solve:
+        del x, y
+        x = y = None
+        torch.cuda.empty_cache()
+        out["solve"] = solve_leg(s, case, a.max_restarts)
     if not a.no_cpu:
         try:
             del x, y

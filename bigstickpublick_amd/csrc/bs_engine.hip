@@ -523,14 +523,16 @@ __global__ __launch_bounds__(256) void k_density_pn(DevTables tb, DensTables dn,
 // across the wave, so scalar) serves all K vectors, and the x and element
 // loads are K-contiguous.
 struct BatchVals {
-  int K;
-  const float* pp32;   // [merged PP entry][K]
+  int K;       // vectors per state (interleaved)
+  int vs;      // value stride: K (one Hamiltonian per vector) or 1 (one shared Hamiltonian)
+  int vk;      // 1: value index adds the lane, 0: shared values
+  const float* pp32;   // [merged PP entry][vs]
   const double* pp64;
-  const float* nn32;   // [NN ELL slot][K]
+  const float* nn32;   // [NN ELL slot][vs]
   const double* nn64;
-  const float* W;      // [pn block element][K]
-  const float* pspe;   // [proton SD][K]
-  const float* nspe;   // [neutron SD][K]
+  const float* W;      // [pn block element][vs]
+  const float* pspe;   // [proton SD][vs]
+  const float* nspe;   // [neutron SD][vs]
 };
 
 template <typename T> __device__ __forceinline__ T bval(const float* a, const double* b, int64_t i);
@@ -550,21 +552,22 @@ __global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals b
   if (c >= L) return;
   const bool on = k < K;
   const int kk = on ? k : 0;
+  const int vs = bv.vs, vk = bv.vk ? kk : 0;
   const int64_t bf = tb.pstart[f];
   const int nf0 = tb.nfirst[cn];
   T acc = T(0);
   if (tb.call_spe)
-    acc += ((T)bv.pspe[(int64_t)f * K + kk] + (T)bv.nspe[(int64_t)(nf0 + c) * K + kk]) * x[(bf + c) * K + kk];
+    acc += ((T)bv.pspe[(int64_t)f * vs + vk] + (T)bv.nspe[(int64_t)(nf0 + c) * vs + vk]) * x[(bf + c) * K + kk];
   for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
     const int64_t bi = tb.pstart[tb.pp_src[e]];
-    acc += bval<T>(bv.pp32, bv.pp64, e * K + kk) * x[(bi + c) * K + kk];
+    acc += bval<T>(bv.pp32, bv.pp64, e * vs + vk) * x[(bi + c) * K + kk];
   }
   {
     const int64_t base = tb.nn_base[cn];
     const int E = tb.nn_emax[cn];
     for (int e = 0; e < E; e++) {
       const int64_t o = base + (int64_t)e * L + c;
-      acc += bval<T>(bv.nn32, bv.nn64, o * K + kk) * x[(bf + tb.nn_src[o]) * K + kk];
+      acc += bval<T>(bv.nn32, bv.nn64, o * vs + vk) * x[(bf + tb.nn_src[o]) * K + kk];
     }
   }
   const int nd = tb.ndelta, nd1 = nd + 1;
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals b
         const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
         const int nl = (pk >> 16) & 0xff;
         if (nl >= ws) continue;  // padding slot
-        const T t = (T)bv.W[(wrow + nl) * K + kk] * x[(bi + (pk & 0xffff)) * K + kk];
+        const T t = (T)bv.W[(wrow + nl) * vs + vk] * x[(bi + (pk & 0xffff)) * K + kk];
         a += ((pk >> 24) & 1) ? -t : t;
       }
       acc += (meta >> 8) ? -a : a;
@@ -1645,6 +1648,271 @@ void run_density(bs_system* s, const T* psi, double* rho_pp, double* rho_nn, dou
 
 }  // namespace
 
+// ------------------------------------------------------------ batched mode
+// K Hamiltonians on one basis (one interaction file each, same orbits,
+// particle numbers and scaling).  The base system holds the structural jump
+// tables (every structurally present two-body pair kept, whatever its value
+// in any one interaction); per-Hamiltonian values are interleaved
+// value[entry][k] so that each jump read serves all K vectors.
+struct bs_batch {
+  std::unique_ptr<bs_system> base;
+  int K = 0;
+  std::vector<std::string> intfiles;
+  bool same_params = true;  // every interaction lists the same TBMEs as the first
+  DevBuf<float> pp32, nn32, W, pspe, nspe;
+  DevBuf<double> pp64, nn64;
+  BatchVals bv{};
+  // Lanczos workspace and its captured graph
+  struct Work {
+    int dtype = 0, niter = 0, nkeep = 0, polish = 0;
+    DevBuf<unsigned char> V, w, X64, HX;
+    DevBuf<double> part, dots, alpha, beta, evals, Y, tdwork, rq, res;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    ~Work() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (stream) (void)hipStreamDestroy(stream);
+    }
+  };
+  std::unique_ptr<Work> work;
+};
+
+namespace {
+
+constexpr int kMaxBatch = 64;
+
+int batch_blocks(int64_t D) { return (int)std::min<int64_t>(std::max<int64_t>((D + 63) / 64, 1), 512); }
+
+template <typename T>
+void launch_batched(const bs_system* s, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  const DevTables& d = s->dt;
+  if (bv.K < 1 || bv.K > kMaxBatch) throw std::runtime_error("batch size must be in 1..64");
+  const int ny = (d.lmax + 3) / 4;
+  hipLaunchKernelGGL(k_apply_batched<T>, dim3(d.nP, ny), dim3(256), 0, st, d, bv, x, y);
+  HIP_OK(hipGetLastError());
+}
+
+void batch_upload_values(bs_batch* b) {
+  const bs::Tables& t = b->base->t;
+  const int K = b->K;
+  const int64_t npp = (int64_t)t.tb[0].val.size();
+  const int64_t nslot = (int64_t)std::max<size_t>(b->base->nn_val32.n, 1);
+  const int64_t nw = (int64_t)t.wpn.size();
+  const int64_t nP = (int64_t)t.sp[0].mask.size(), nN = (int64_t)t.sp[1].mask.size();
+  std::vector<float> pp32((size_t)std::max<int64_t>(npp, 1) * K, 0.0f), nn32((size_t)nslot * K, 0.0f);
+  std::vector<double> pp64(pp32.size(), 0.0), nn64(nn32.size(), 0.0);
+  std::vector<float> W((size_t)std::max<int64_t>(nw, 1) * K, 0.0f);
+  std::vector<float> ps((size_t)std::max<int64_t>(nP, 1) * K, 0.0f), ns((size_t)std::max<int64_t>(nN, 1) * K, 0.0f);
+  for (int k = 0; k < K; k++) {
+    const bs::MeTables me = bs::build_me_tables(t, b->intfiles[k]);
+    if (me.call_spe != t.call_spe) throw std::runtime_error("interactions disagree on the single-particle term");
+    if (!me.same_lines) b->same_params = false;
+    const std::vector<double> vp = bs::merged_values(t, 0, me.vpp);
+    const std::vector<double> vn = bs::merged_values(t, 1, me.vnn);
+    for (int64_t e = 0; e < npp; e++) {
+      pp64[(size_t)e * K + k] = vp[e];
+      pp32[(size_t)e * K + k] = (float)vp[e];
+    }
+    for (size_t q = 0; q < vn.size(); q++) {
+      const int64_t o = b->base->nn_slot[q];
+      nn64[(size_t)o * K + k] = vn[q];
+      nn32[(size_t)o * K + k] = (float)vn[q];
+    }
+    if ((int64_t)me.wpn.size() != nw) throw std::runtime_error("pn element blocks differ between interactions");
+    for (int64_t i = 0; i < nw; i++) W[(size_t)i * K + k] = me.wpn[i];
+    for (int it = 0; it < 2; it++) {
+      const bs::Species& S = t.sp[it];
+      std::vector<float>& out = it == 0 ? ps : ns;
+      const std::vector<float>& e = it == 0 ? me.pspe : me.nspe;
+      for (size_t i = 0; i < S.mask.size(); i++) {
+        float acc = 0.0f;
+        for (uint64_t q = S.mask[i]; q; q &= q - 1) acc += e[t.sorb[__builtin_ctzll(q)]];
+        out[i * K + k] = acc;
+      }
+    }
+  }
+  b->pp32.upload(pp32);
+  b->pp64.upload(pp64);
+  b->nn32.upload(nn32);
+  b->nn64.upload(nn64);
+  b->W.upload(W);
+  b->pspe.upload(ps);
+  b->nspe.upload(ns);
+  BatchVals& v = b->bv;
+  v.K = K;
+  v.vs = K;
+  v.vk = 1;
+  v.pp32 = b->pp32.p;
+  v.pp64 = b->pp64.p;
+  v.nn32 = b->nn32.p;
+  v.nn64 = b->nn64.p;
+  v.W = b->W.p;
+  v.pspe = b->pspe.p;
+  v.nspe = b->nspe.p;
+}
+
+BatchVals shared_vals(const bs_system* s, int K) {
+  BatchVals v{};
+  v.K = K;
+  v.vs = 1;
+  v.vk = 0;
+  v.pp32 = s->dt.pp_val32;
+  v.pp64 = s->dt.pp_val64;
+  v.nn32 = s->dt.nn_val32;
+  v.nn64 = s->dt.nn_val64;
+  v.W = s->dt.W;
+  v.pspe = s->dt.pspe_sd;
+  v.nspe = s->dt.nspe_sd;
+  return v;
+}
+
+// Enqueue one whole batched Lanczos solve on stream st (no host
+// synchronisation, no memset: it is captured into a graph).  Every
+// Hamiltonian runs the same number of iterations in lockstep; alpha/beta and
+// the Gram-Schmidt dots are FP64 per Hamiltonian.
+template <typename T>
+void enqueue_batch_lanczos(bs_batch* b, bs_batch::Work& w, hipStream_t st) {
+  const bs_system* s = b->base.get();
+  const int K = b->K;
+  const int64_t D = s->t.dim, DK = D * K;
+  const int niter = w.niter, nkeep = w.nkeep;
+  const int nb = batch_blocks(D);
+  const int g = grid_for(DK);
+  T* V = (T*)w.V.p;
+  T* wv = (T*)w.w.p;
+  k_fill<T><<<g, kBlock, 0, st>>>(V, (T)(1.0 / std::sqrt((double)D)), DK);
+  for (int i = 0; i < niter; i++) {
+    T* vi = V + (int64_t)i * DK;
+    launch_batched<T>(s, b->bv, vi, wv, st);
+    k_bmultidot<T><<<nb, 256, 0, st>>>(vi, DK, 1, wv, D, K, w.part.p);
+    k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.alpha.p + (int64_t)i * K, 0);
+    k_bsub_scaled<T><<<g, kBlock, 0, st>>>(wv, vi, w.alpha.p + (int64_t)i * K, D, K);
+    if (i > 0) {
+      k_bmultidot<T><<<nb, 256, 0, st>>>(V, DK, i, wv, D, K, w.part.p);
+      k_breduce<<<(i * K + 255) / 256, 256, 0, st>>>(w.part.p, nb, i, K, w.dots.p, 0);
+      k_bmultiaxpy<T><<<g, kBlock, 0, st>>>(wv, V, DK, i, w.dots.p, D, K);
+    }
+    k_bmultidot<T><<<nb, 256, 0, st>>>(wv, DK, 1, wv, D, K, w.part.p);
+    k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.beta.p + (int64_t)i * K, 1);
+    k_bnormalize<T><<<g, kBlock, 0, st>>>(V + (int64_t)(i + 1) * DK, wv, w.beta.p + (int64_t)i * K, D, K);
+  }
+  HIP_OK(hipGetLastError());
+  k_tridiag_ritz<<<(K + 63) / 64, 64, 0, st>>>(w.alpha.p, w.beta.p, niter, K, nkeep, w.evals.p, w.Y.p, w.tdwork.p);
+  k_bcombine<T, double><<<g, kBlock, 0, st>>>((double*)w.X64.p, V, DK, niter, w.Y.p, nkeep, D, K);
+  HIP_OK(hipGetLastError());
+  double* X = (double*)w.X64.p;
+  double* HX = (double*)w.HX.p;
+  for (int q = 0; q < nkeep; q++) {
+    double* xq = X + (int64_t)q * DK;
+    k_bmultidot<double><<<nb, 256, 0, st>>>(xq, DK, 1, xq, D, K, w.part.p);
+    k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.dots.p, 1);
+    k_bnormalize<double><<<g, kBlock, 0, st>>>(xq, xq, w.dots.p, D, K);
+    if (w.polish) {
+      launch_batched<double>(s, b->bv, xq, HX, st);
+      k_bmultidot<double><<<nb, 256, 0, st>>>(xq, DK, 1, HX, D, K, w.part.p);
+      k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.rq.p + (int64_t)q * K, 0);
+      k_bsub_scaled<double><<<g, kBlock, 0, st>>>(HX, xq, w.rq.p + (int64_t)q * K, D, K);
+      k_bmultidot<double><<<nb, 256, 0, st>>>(HX, DK, 1, HX, D, K, w.part.p);
+      k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.res.p + (int64_t)q * K, 1);
+    }
+  }
+  HIP_OK(hipGetLastError());
+}
+
+template <typename T>
+void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,
+                       void* evecs, double* resid, int* iters) {
+  const int K = b->K;
+  const int64_t D = b->base->t.dim, DK = D * K;
+  const int niter = (int)std::min<int64_t>(p->niter, D);
+  const int nkeep = p->nkeep;
+  if (nkeep < 1 || nkeep > 16) throw std::runtime_error("nkeep must be in 1..16");
+  if (niter < nkeep) throw std::runtime_error("niter must be >= nkeep");
+  if (p->thick != 0) throw std::runtime_error("batched Lanczos runs a fixed number of iterations (thick must be 0)");
+  const int dt = sizeof(T) == 4 ? BS_F32 : BS_F64;
+  std::unique_ptr<bs_batch::Work>& wp = b->work;
+  if (!wp || wp->dtype != dt || wp->niter != niter || wp->nkeep != nkeep || wp->polish != (p->polish ? 1 : 0)) {
+    wp.reset(new bs_batch::Work());
+    bs_batch::Work& w = *wp;
+    w.dtype = dt;
+    w.niter = niter;
+    w.nkeep = nkeep;
+    w.polish = p->polish ? 1 : 0;
+    const int nb = batch_blocks(D);
+    w.V.alloc((size_t)(niter + 1) * DK * sizeof(T));
+    w.w.alloc((size_t)DK * sizeof(T));
+    w.X64.alloc((size_t)nkeep * DK * sizeof(double));
+    w.HX.alloc((size_t)DK * sizeof(double));
+    w.part.alloc((size_t)nb * niter * K);
+    w.dots.alloc((size_t)niter * K);
+    w.alpha.alloc((size_t)niter * K);
+    w.beta.alloc((size_t)niter * K);
+    w.evals.alloc((size_t)K * nkeep);
+    w.Y.alloc((size_t)K * niter * nkeep);
+    w.tdwork.alloc((size_t)K * 6 * niter);
+    w.rq.alloc((size_t)nkeep * K);
+    w.res.alloc((size_t)nkeep * K);
+    HIP_OK(hipStreamCreate(&w.stream));
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_batch_lanczos<T>(b, w, w.stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(w.stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(w.stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_OK(ie);
+  }
+  bs_batch::Work& w = *wp;
+  HIP_OK(hipGraphLaunch(w.exec, w.stream));
+  HIP_OK(hipStreamSynchronize(w.stream));
+  std::vector<double> ev((size_t)K * nkeep), rq((size_t)nkeep * K), rs((size_t)nkeep * K);
+  HIP_OK(hipMemcpy(ev.data(), w.evals.p, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
+  if (w.polish) {
+    HIP_OK(hipMemcpy(rq.data(), w.rq.p, sizeof(double) * rq.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(rs.data(), w.res.p, sizeof(double) * rs.size(), hipMemcpyDeviceToHost));
+  }
+  if (alpha || beta) {
+    std::vector<double> a((size_t)niter * K), bb((size_t)niter * K);
+    HIP_OK(hipMemcpy(a.data(), w.alpha.p, sizeof(double) * a.size(), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(bb.data(), w.beta.p, sizeof(double) * bb.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; k++)
+      for (int i = 0; i < niter; i++) {
+        if (alpha) alpha[(size_t)k * niter + i] = a[(size_t)i * K + k];
+        if (beta) beta[(size_t)k * niter + i] = bb[(size_t)i * K + k];
+      }
+  }
+  for (int k = 0; k < K; k++)
+    for (int q = 0; q < nkeep; q++) {
+      evals[(size_t)k * nkeep + q] = w.polish ? rq[(size_t)q * K + k] : ev[(size_t)k * nkeep + q];
+      if (resid) resid[(size_t)k * nkeep + q] = w.polish ? rs[(size_t)q * K + k] : -1.0;
+    }
+  if (evecs) {
+    if (p->dtype == BS_F64)
+      HIP_OK(hipMemcpy(evecs, w.X64.p, sizeof(double) * (size_t)nkeep * DK, hipMemcpyDeviceToDevice));
+    else {
+      const int64_t n = (int64_t)nkeep * DK;
+      k_convert<double, float><<<grid_for(n), kBlock, 0, w.stream>>>((float*)evecs, (const double*)w.X64.p, n);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(w.stream));
+    }
+  }
+  if (iters) *iters = niter + (w.polish ? nkeep : 0);
+}
+
+// lane k of an interleaved set -> contiguous FP64 vector
+template <typename T>
+__global__ void k_gather_lane(double* __restrict__ out, const T* __restrict__ in, int64_t D, int K, int k) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i * K + k];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* bs_last_error(void) { return g_err.c_str(); }
@@ -1932,6 +2200,111 @@ int bs_gradient(bs_system* s, const void* psi, int dtype, double* grad) {
         }
       }
       for (int a = 0; a < norb; a++) grad[a] += (double)t.in.spscale * occ[a] / n2;
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_apply_block(bs_system* s, const void* x, void* y, int K, int dtype) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (!x || !y) return set_err("null vector");
+    if (K < 1 || K > kMaxBatch) return set_err("K must be in 1..64");
+    const BatchVals v = shared_vals(s, K);
+    if (dtype == BS_F32) launch_batched<float>(s, v, (const float*)x, (float*)y, nullptr);
+    else if (dtype == BS_F64) launch_batched<double>(s, v, (const double*)x, (double*)y, nullptr);
+    else return set_err("dtype must be BS_F32 or BS_F64");
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_batch_init(const char* sps, const char* const* intfiles, int K, int Z, int N, int jz2, float spscale,
+                  float a0, float a, float x, bs_batch** out) {
+  try {
+    if (!out || !sps || !intfiles) return set_err("null argument");
+    *out = nullptr;
+    if (K < 1 || K > kMaxBatch) return set_err("K must be in 1..64");
+    for (int k = 0; k < K; k++)
+      if (!intfiles[k]) return set_err("null interaction file name");
+    std::unique_ptr<bs_batch> b(new bs_batch());
+    b->K = K;
+    for (int k = 0; k < K; k++) b->intfiles.push_back(intfiles[k]);
+    b->base.reset(new bs_system());
+    b->base->t = bs::build_tables_structural(sps, intfiles[0], Z, N, jz2, spscale, a0, a, x);
+    b->base->sps_path = sps;
+    upload_tables(b->base.get());
+    batch_upload_values(b.get());
+    HIP_OK(hipDeviceSynchronize());
+    *out = b.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_batch_finalize(bs_batch* b) {
+  delete b;
+  return 0;
+}
+
+int bs_batch_size(const bs_batch* b, int* K, int64_t* dim) {
+  if (!b) return set_err("null batch");
+  if (K) *K = b->K;
+  if (dim) *dim = b->base->t.dim;
+  return 0;
+}
+
+bs_system* bs_batch_system(bs_batch* b) { return b ? b->base.get() : nullptr; }
+
+int bs_batch_apply(bs_batch* b, const void* x, void* y, int dtype) {
+  try {
+    if (!b) return set_err("null batch");
+    if (!x || !y) return set_err("null vector");
+    if (dtype == BS_F32) launch_batched<float>(b->base.get(), b->bv, (const float*)x, (float*)y, nullptr);
+    else if (dtype == BS_F64) launch_batched<double>(b->base.get(), b->bv, (const double*)x, (double*)y, nullptr);
+    else return set_err("dtype must be BS_F32 or BS_F64");
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,
+                     void* evecs, double* resid, int* iters) {
+  try {
+    if (!b || !p || !evals) return set_err("null argument");
+    if (p->dtype == BS_F32) run_batch_lanczos<float>(b, p, alpha, beta, evals, evecs, resid, iters);
+    else if (p->dtype == BS_F64) run_batch_lanczos<double>(b, p, alpha, beta, evals, evecs, resid, iters);
+    else return set_err("dtype must be BS_F32 or BS_F64");
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_batch_gradient(bs_batch* b, const void* psi, int dtype, double* grad) {
+  try {
+    if (!b || !psi || !grad) return set_err("null argument");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (!b->same_params) return set_err("interactions list different two-body elements; gradients need one list");
+    bs_system* s = b->base.get();
+    const int64_t D = s->t.dim;
+    const int np = (int)(s->t.in.orbits.size() + s->t.lines.size());
+    DevBuf<double> lane;
+    lane.alloc(D);
+    for (int k = 0; k < b->K; k++) {
+      if (dtype == BS_F32)
+        k_gather_lane<float><<<grid_for(D), kBlock>>>(lane.p, (const float*)psi, D, b->K, k);
+      else
+        k_gather_lane<double><<<grid_for(D), kBlock>>>(lane.p, (const double*)psi, D, b->K, k);
+      HIP_OK(hipGetLastError());
+      const int rc = bs_gradient(s, lane.p, BS_F64, grad + (size_t)k * np);
+      if (rc) return rc;
     }
     return 0;
   } catch (const std::exception& e) {

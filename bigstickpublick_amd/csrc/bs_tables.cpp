@@ -911,6 +911,11 @@ MeTables build_me_tables(const Tables& base, const std::string& intfile) {
   m.pspe = t.pspe;
   m.nspe = t.nspe;
   m.call_spe = t.call_spe;
+  m.same_lines = t.lines.size() == base.lines.size();
+  for (size_t i = 0; m.same_lines && i < t.lines.size(); i++) {
+    const TbmeLine &u = t.lines[i], &w = base.lines[i];
+    m.same_lines = u.a == w.a && u.b == w.b && u.c == w.c && u.d == w.d && u.J == w.J && u.T == w.T;
+  }
   // pn elements in base's class layout
   const int ns = base.nsps, npn = ns * ns;
   m.wpn.assign(base.wpn.size(), 0.0f);

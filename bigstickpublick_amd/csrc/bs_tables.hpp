@@ -131,6 +131,7 @@ Tables build_tables(const std::string& sps, const std::string& intfile, int Z, i
 struct MeTables {
   std::vector<float> vpp, vnn, vpn, wpn, pspe, nspe;
   bool call_spe = false;
+  bool same_lines = false;  // same TBME list (a b c d J T, file order) as base
 };
 MeTables build_me_tables(const Tables& base, const std::string& intfile);
 // Merged two-body values of element table v (dense pair-pair) in the entry

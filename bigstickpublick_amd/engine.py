@@ -58,6 +58,16 @@ def load_library():
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
+    L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
+    L.bs_batch_init.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), i32, i32, i32, i32,
+                                f32, f32, f32, f32, ctypes.POINTER(vp)]
+    L.bs_batch_finalize.argtypes = [vp]
+    L.bs_batch_size.argtypes = [vp, ctypes.POINTER(i32), i64p]
+    L.bs_batch_system.argtypes = [vp]
+    L.bs_batch_system.restype = vp
+    L.bs_batch_apply.argtypes = [vp, vp, vp, i32]
+    L.bs_batch_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp, ctypes.POINTER(i32)]
+    L.bs_batch_gradient.argtypes = [vp, vp, i32, vp]
     _lib = L
     return L
 
@@ -260,3 +270,106 @@ class System:
                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
+
+    def apply_block(self, x, y=None):
+        """y[i, k] = (H x)[i, k] for a (dim, K) CUDA tensor of K vectors, K <= 64."""
+        import torch
+        self.upload()
+        if x.dim() != 2 or x.shape[0] != self.dim or x.dtype not in (torch.float32, torch.float64):
+            raise ValueError("x must be a (dim, K) CUDA float32/float64 tensor")
+        x = x.contiguous()
+        if y is None:
+            y = torch.empty_like(x)
+        dt = BS_F32 if x.dtype == torch.float32 else BS_F64
+        _check(load_library().bs_apply_block(self.h, ctypes.c_void_p(x.data_ptr()),
+                                             ctypes.c_void_p(y.data_ptr()), x.shape[1], dt))
+        return y
+
+
+@dataclass
+class BatchResult:
+    evals: np.ndarray      # (K, nkeep)
+    alpha: np.ndarray      # (K, niter)
+    beta: np.ndarray       # (K, niter)
+    resid: np.ndarray      # (K, nkeep)
+    matvecs: int
+    evecs: object = None   # (nkeep, dim, K) CUDA tensor
+
+
+class Batch:
+    """K Hamiltonians (one .int file each) on one basis, K <= 64: shared jump
+    tables, interleaved (dim, K) vectors."""
+
+    def __init__(self, sps, intfiles, Z, N, jz2=0, scaling=(1.0, 1.0, 1.0, 1.0)):
+        L = load_library()
+        names = (ctypes.c_char_p * len(intfiles))(*[str(f).encode() for f in intfiles])
+        h = ctypes.c_void_p()
+        _check(L.bs_batch_init(str(sps).encode(), names, len(intfiles), Z, N, jz2,
+                               *[float(v) for v in scaling], ctypes.byref(h)))
+        self.h = h
+        k, d = ctypes.c_int(), ctypes.c_int64()
+        _check(L.bs_batch_size(h, ctypes.byref(k), ctypes.byref(d)))
+        self.K, self.dim = k.value, d.value
+        self.intfiles = list(intfiles)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().bs_batch_finalize(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def num_params(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        sysh = load_library().bs_batch_system(self.h)
+        _check(load_library().bs_num_params(sysh, ctypes.byref(a), ctypes.byref(b)))
+        return a.value + b.value
+
+    def apply(self, x, y=None):
+        """y[:, k] = H_k x[:, k] for a (dim, K) CUDA tensor."""
+        import torch
+        if tuple(x.shape) != (self.dim, self.K) or x.dtype not in (torch.float32, torch.float64):
+            raise ValueError("x must be a (dim, K) CUDA float32/float64 tensor")
+        x = x.contiguous()
+        if y is None:
+            y = torch.empty_like(x)
+        dt = BS_F32 if x.dtype == torch.float32 else BS_F64
+        torch.cuda.synchronize()
+        _check(load_library().bs_batch_apply(self.h, ctypes.c_void_p(x.data_ptr()),
+                                             ctypes.c_void_p(y.data_ptr()), dt))
+        torch.cuda.synchronize()
+        return y
+
+    def lanczos(self, nkeep, niter, dtype="f32", polish=True, want_vectors=False):
+        import torch
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        p = _Params(nkeep, niter, 0, 0, dt, 0.0, int(polish))
+        ev = np.zeros((self.K, nkeep))
+        res = np.zeros((self.K, nkeep))
+        m = min(niter, self.dim)
+        alpha = np.zeros((self.K, m))
+        beta = np.zeros((self.K, m))
+        it = ctypes.c_int()
+        vec = None
+        if want_vectors:
+            vec = torch.empty((nkeep, self.dim, self.K), device="cuda",
+                              dtype=torch.float32 if dt == BS_F32 else torch.float64)
+        torch.cuda.synchronize()
+        _check(load_library().bs_batch_lanczos(self.h, ctypes.byref(p), alpha.ctypes.data, beta.ctypes.data,
+                                               ev.ctypes.data,
+                                               ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
+                                               res.ctypes.data, ctypes.byref(it)))
+        return BatchResult(ev, alpha, beta, res, it.value, vec)
+
+    def gradient(self, psi):
+        """dE_k/dtheta for each Hamiltonian: psi is a (dim, K) CUDA tensor, one
+        eigenvector per Hamiltonian; returns (K, nparams)."""
+        import torch
+        if tuple(psi.shape) != (self.dim, self.K):
+            raise ValueError("psi must be (dim, K)")
+        psi = psi.contiguous()
+        g = np.zeros((self.K, self.num_params()))
+        dt = BS_F32 if psi.dtype == torch.float32 else BS_F64
+        torch.cuda.synchronize()
+        _check(load_library().bs_batch_gradient(self.h, ctypes.c_void_p(psi.data_ptr()), dt, g.ctypes.data))
+        return g

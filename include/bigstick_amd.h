@@ -156,6 +156,42 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* ---------------------------------------------------------------- block and batched
+ * Interleaved multi-vector layout: element k of basis state i at [i*K + k],
+ * K <= 64. */
+
+/* y = H x for K vectors of the same Hamiltonian (device pointers, y is
+ * overwritten).  Replaces the block apply of the reference
+ * (src/bapplyhlibOMP_block.f90:43-887, vectors block(ivec,istate)). */
+int bs_apply_block(bs_system *sys, const void *x, void *y, int K, int dtype);
+
+/* K Hamiltonians on one basis: one iso .int file each, same orbits, particle
+ * numbers and scaling line (src/binput_tbme.f90:898-1250 per file).  Jump
+ * tables are built once; values are stored per Hamiltonian.  No counterpart
+ * in the reference, which diagonalizes one Hamiltonian per run. */
+typedef struct bs_batch bs_batch;
+int bs_batch_init(const char *sps, const char *const *intfiles, int K, int Z, int N, int jz2,
+                  float spscale, float a0, float a, float x, bs_batch **out);
+int bs_batch_finalize(bs_batch *b);
+int bs_batch_size(const bs_batch *b, int *K, int64_t *dim);
+/* The system holding the shared basis and jump tables (values of the first
+ * interaction); owned by the batch. */
+bs_system *bs_batch_system(bs_batch *b);
+/* y_k = H_k x_k, interleaved device vectors. */
+int bs_batch_apply(bs_batch *b, const void *x, void *y, int dtype);
+/* Lanczos on all K Hamiltonians in lockstep (the recurrence of bs_lanczos,
+ * thick must be 0), captured once into a HIP graph and replayed on later
+ * calls with the same parameters; Ritz values and vectors by FP64 Sturm
+ * bisection and inverse iteration on the device.  alpha/beta (may be NULL):
+ * [k*min(niter,dim) + i]; evals[k*nkeep + n];
+ * evecs (device, may be NULL): nkeep interleaved sets [n][i*K + k] in dtype;
+ * resid[k*nkeep + n] (may be NULL; -1 without polish). */
+int bs_batch_lanczos(bs_batch *b, const bs_lanczos_params *p, double *alpha, double *beta, double *evals,
+                     void *evecs, double *resid, int *iters);
+/* Hellmann-Feynman gradients of <psi_k|H_k|psi_k>/<psi_k|psi_k> for every
+ * Hamiltonian: grad[k*nparams + p], parameters as in bs_gradient. */
+int bs_batch_gradient(bs_batch *b, const void *psi, int dtype, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

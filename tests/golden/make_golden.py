@@ -209,6 +209,28 @@ def config(cfg, with_basis=True, niter=None):
         shutil.rmtree(wd)
 
 
+def ensemble(members=8):
+    """Batched ensemble: the reference run on each of the first `members`
+    Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
+    coefficients of the fixed 150-step run)."""
+    res, al, be = [], [], []
+    for m in range(members):
+        case = config_case(2, m)
+        wd = tempfile.mkdtemp(prefix="bsgold_")
+        try:
+            write_case(case, wd)
+            run_ref(wd, menu_script(case, "n", "lf"), timeout=1800)
+            res.append(read_res(os.path.join(wd, case.name + ".res")))
+            a, b = read_lcoef(os.path.join(wd, case.name + ".lcoef"))
+            al.append(a)
+            be.append(b)
+            print("ensemble member", m, "E=", res[-1][:, 0])
+        finally:
+            shutil.rmtree(wd)
+    np.savez_compressed(os.path.join(HERE, "config2.npz"), res=np.array(res), alpha=np.array(al),
+                        beta=np.array(be), members=members)
+
+
 if __name__ == "__main__":
     want = sys.argv[1:] or [c[0] for c in SMALL] + ["config0", "config1"]
     for c in SMALL:
@@ -217,5 +239,7 @@ if __name__ == "__main__":
     for cfg in (0, 1):
         if "config%d" % cfg in want:
             config(cfg, with_basis=True)
+    if "config2" in want:
+        ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps
         config(3, with_basis=True, niter=20)

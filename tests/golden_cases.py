@@ -30,6 +30,19 @@ def config_fixture(cfg, workdir):
     return case, d, fx
 
 
+def ensemble_fixture(workdir, members):
+    """The first `members` Hamiltonians of the sd 4p4n ensemble, written into
+    one directory, and the reference's results for them."""
+    d = os.path.join(workdir, "cfg2")
+    cases = []
+    for m in range(members):
+        case = config_case(2, m)
+        write_case(case, d)
+        cases.append(case)
+    fx = dict(np.load(os.path.join(GOLD, "config2.npz")))
+    return cases, d, fx
+
+
 def paths(case, d):
     return os.path.join(d, case.space + ".sps"), os.path.join(d, case.name + ".int")
 
