@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -596,6 +598,156 @@ __global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals b
   if (on) y[(bf + c) * K + k] = acc;
 }
 
+// K interleaved vectors per basis state on large bases: the tile kernel's
+// structure (one workgroup per proton row, CPT columns per thread) with every
+// jump-table and index read serving K vectors.  LDS: staged rows [col][G][K]
+// followed by Wb[nl][G][K].  Values per Hamiltonian (vs = K, vk = 1) or
+// shared (vs = 1, vk = 0).
+template <typename T, int BLOCK, int CPT, int K, int G>
+__global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals bv, const T* __restrict__ x,
+                                                       T* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int GK = G * K;
+  T* xs = reinterpret_cast<T*>(smem);
+  float* wb = reinterpret_cast<float*>(smem + (size_t)GK * tb.lmax * sizeof(T));
+  const int tid = threadIdx.x;
+  const int tile = tb.tile_map[blockIdx.x];
+  if (tile < 0) return;
+  const int f = tb.tile_f0[tile];
+  const int sp = tb.psec[f];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  const int nf0 = tb.nfirst[cn];
+  const int vs = bv.vs, vk = bv.vk;
+  const int64_t bf = tb.pstart[f];
+  T acc[CPT][K];
+#pragma unroll
+  for (int q = 0; q < CPT; q++)
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[q][k] = T(0);
+  using XV = VecG<T, K>;
+  // ---- SPE
+  if (tb.call_spe) {
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int c = tid + q * BLOCK;
+      if (c < L) {
+        const XV xv = *reinterpret_cast<const XV*>(x + (bf + c) * K);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+          acc[q][k] += ((T)bv.pspe[(int64_t)f * vs + k * vk] + (T)bv.nspe[(int64_t)(nf0 + c) * vs + k * vk]) * xv.v[k];
+      }
+    }
+  }
+  // ---- PP: row AXPYs, K-wide coalesced reads
+  for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
+    const int64_t bi = tb.pstart[tb.pp_src[e]];
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = bval<T>(bv.pp32, bv.pp64, e * vs + k * vk);
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int c = tid + q * BLOCK;
+      if (c < L) {
+        const XV xv = *reinterpret_cast<const XV*>(x + (bi + c) * K);
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[q][k] += v[k] * xv.v[k];
+      }
+    }
+  }
+  // ---- NN: the row staged [col][K]
+  {
+    for (int i = tid; i < L * K; i += BLOCK) xs[i] = x[bf * K + i];
+    __syncthreads();
+    const XV* xr = reinterpret_cast<const XV*>(xs);
+    const int64_t base = tb.nn_base[cn];
+    const int E = tb.nn_emax[cn];
+    for (int e = 0; e < E; e++) {
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * BLOCK;
+        if (c < L) {
+          const int64_t o = base + (int64_t)e * L + c;
+          const XV xv = xr[tb.nn_src[o]];
+#pragma unroll
+          for (int k = 0; k < K; k++) acc[q][k] += bval<T>(bv.nn32, bv.nn64, o * vs + k * vk) * xv.v[k];
+        }
+      }
+    }
+  }
+  // ---- PN: per Delta class, batches of G source rows staged [col][G][K]
+  using GV = VecG<T, GK>;
+  using WV = VecG<float, GK>;
+  const GV* xg = reinterpret_cast<const GV*>(xs);
+  const WV* wg = reinterpret_cast<const WV*>(wb);
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int k0 = 0; k0 < nd; k0++) {
+    const int p0 = tb.pcls[(int64_t)f * nd1 + k0], p1 = tb.pcls[(int64_t)f * nd1 + k0 + 1];
+    if (p0 == p1) continue;
+    const int kn = tb.negk[k0];
+    const int si = tb.srcsec[sp * nd + k0];
+    const int Li = tb.nlen[tb.pconj[si]];
+    const int wsd = tb.wstride[k0];
+    const int64_t woff = tb.woff[k0];
+    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
+    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
+    for (int b0 = p0; b0 < p1; b0 += G) {
+      const int nb = min(G, p1 - b0);
+      __syncthreads();
+      for (int g = 0; g < G; g++) {
+        const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
+        for (int i = tid; i < Li * K; i += BLOCK) {
+          const int c = i / K, k = i - c * K;
+          xs[(c * G + g) * K + k] = g < nb ? x[bi * K + i] : T(0);
+        }
+      }
+      for (int i = tid; i < (wsd + 1) * GK; i += BLOCK) {
+        const int nl = i / GK, r = i - nl * GK, g = r / K, k = r - g * K;
+        float w = 0.0f;
+        if (g < nb && nl < wsd) {
+          const int meta = tb.p1_meta[b0 + g];
+          w = bv.W[(woff + (int64_t)(meta & 0xff) * wsd + nl) * vs + k * vk];
+          if (meta >> 8) w = -w;
+        }
+        wb[i] = w;
+      }
+      __syncthreads();
+      const uint32_t pad = (uint32_t)wsd << 16;
+      for (int e = 0; e < E; e++) {
+        uint32_t pk[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int c = tid + q * BLOCK;
+          pk[q] = c < L ? tb.n1_pk[ebase + (int64_t)e * L + c] : pad;
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const GV xv = xg[pk[q] & 0xffff];
+          const WV wv = wg[(pk[q] >> 16) & 0xff];
+          const bool neg = (pk[q] >> 24) & 1;
+#pragma unroll
+          for (int k = 0; k < K; k++) {
+            T d = T(0);
+#pragma unroll
+            for (int g = 0; g < G; g++) d += (T)wv.v[g * K + k] * xv.v[g * K + k];
+            acc[q][k] += neg ? -d : d;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int c = tid + q * BLOCK;
+    if (c < L) {
+      XV o;
+#pragma unroll
+      for (int k = 0; k < K; k++) o.v[k] = acc[q][k];
+      *reinterpret_cast<XV*>(y + (bf + c) * K) = o;
+    }
+  }
+}
+
 // part[b*m*K + j*K + k] = partial sum over this block's states of V_j[s][k] * w[s][k]
 template <typename T>
 __global__ __launch_bounds__(256) void k_bmultidot(const T* __restrict__ V, int64_t ldv, int m, const T* __restrict__ w,
@@ -754,21 +906,21 @@ __global__ void k_tridiag_ritz(const double* __restrict__ alpha, const double* _
   }
 }
 
-// X_n[s][k] = sum_j Y[(k*m + j)*nkeep + n] V_j[s][k]
+// X_n[s][k] = sum_j Y[(k*m + j)*ys + n] V_j[s][k] for n < ncol (<= 16)
 template <typename T, typename U>
 __global__ void k_bcombine(U* __restrict__ X, const T* __restrict__ V, int64_t ldv, int m, const double* __restrict__ Y,
-                           int nkeep, int64_t D, int K) {
+                           int ys, int ncol, int64_t D, int K) {
   const int64_t n = D * K;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int k = (int)(i % K);
     double acc[16];
-    for (int q = 0; q < nkeep; q++) acc[q] = 0.0;
+    for (int q = 0; q < ncol; q++) acc[q] = 0.0;
     for (int j = 0; j < m; j++) {
       const double vj = (double)V[(int64_t)j * ldv + i];
-      const double* yk = Y + ((int64_t)k * m + j) * nkeep;
-      for (int q = 0; q < nkeep; q++) acc[q] += yk[q] * vj;
+      const double* yk = Y + ((int64_t)k * m + j) * ys;
+      for (int q = 0; q < ncol; q++) acc[q] += yk[q] * vj;
     }
-    for (int q = 0; q < nkeep; q++) X[(int64_t)q * n + i] = (U)acc[q];
+    for (int q = 0; q < ncol; q++) X[(int64_t)q * n + i] = (U)acc[q];
   }
 }
 
@@ -858,6 +1010,8 @@ struct bs_system {
   std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
   DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
   std::vector<int64_t> nn_slot;        // ELL slot of each merged NN entry (batched mode)
+  std::map<int, int> multi_choice;     // multi-vector kernel variant per (dtype, shared values, K)
+  std::map<int, std::array<float, 8>> multi_ms;  // its autotuner timings
   // adjoint tables (built on first bs_density)
   bool dens_ready = false;
   DensTables dn{};
@@ -1662,15 +1816,17 @@ struct bs_batch {
   DevBuf<float> pp32, nn32, W, pspe, nspe;
   DevBuf<double> pp64, nn64;
   BatchVals bv{};
-  // Lanczos workspace and its captured graph
+  // Lanczos workspace and its captured graphs: [0] the first Krylov cycle,
+  // [1] a restarted cycle, [2] the thick restart, [3] Ritz vectors + polish
   struct Work {
-    int dtype = 0, niter = 0, nkeep = 0, polish = 0;
-    DevBuf<unsigned char> V, w, X64, HX;
-    DevBuf<double> part, dots, alpha, beta, evals, Y, tdwork, rq, res;
-    hipGraphExec_t exec = nullptr;
+    int dtype = 0, niter = 0, nkeep = 0, polish = 0, thick = 0;
+    DevBuf<unsigned char> V, w, X64, HX, tmp;
+    DevBuf<double> part, dots, alpha, beta, evals, Y, tdwork, rq, res, A, arrow;
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
     hipStream_t stream = nullptr;
     ~Work() {
-      if (exec) (void)hipGraphExecDestroy(exec);
+      for (auto& e : exec)
+        if (e) (void)hipGraphExecDestroy(e);
       if (stream) (void)hipStreamDestroy(stream);
     }
   };
@@ -1683,13 +1839,122 @@ constexpr int kMaxBatch = 64;
 
 int batch_blocks(int64_t D) { return (int)std::min<int64_t>(std::max<int64_t>((D + 63) / 64, 1), 512); }
 
+// Multi-vector kernels: variant 0 = one wave per basis state (lane = vector;
+// small bases), 1 + 2*gi + map = k_apply_multi with G = 1 << gi staged rows
+// and identity (map 0) or XCD-aware (map 1) workgroup order.
+constexpr int kNumMulti = 5;
+
+template <typename T, int CPT, int K, int G>
+void launch_multi_kg(const bs_system* s, int map, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  DevTables d = s->dt;
+  d.tile_f0 = s->tile_f0[0].p;
+  d.tile_nr = s->tile_nr[0].p;
+  d.tile_map = s->tile_map[0][map].p;
+  const size_t lds = (size_t)G * K * d.lmax * sizeof(T) + (size_t)d.wcols * G * K * sizeof(float);
+  if (lds > kLdsLimit) throw std::runtime_error("multi-vector variant does not fit in LDS");
+  auto fn = k_apply_multi<T, kApplyBlock, CPT, K, G>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kLdsLimit));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, dim3(s->nslots[0][map]), dim3(kApplyBlock), lds, st, d, bv, x, y);
+  HIP_OK(hipGetLastError());
+}
+
+template <typename T, int CPT, int K>
+void launch_multi_k(const bs_system* s, int v, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  const int gi = (v - 1) / 2, map = (v - 1) % 2;
+  if (gi == 0) launch_multi_kg<T, CPT, K, 1>(s, map, bv, x, y, st);
+  else launch_multi_kg<T, CPT, K, 2>(s, map, bv, x, y, st);
+}
+
+template <typename T, int CPT>
+void launch_multi_cpt(const bs_system* s, int v, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  switch (bv.K) {
+    case 1: launch_multi_k<T, CPT, 1>(s, v, bv, x, y, st); break;
+    case 2: launch_multi_k<T, CPT, 2>(s, v, bv, x, y, st); break;
+    case 4: launch_multi_k<T, CPT, 4>(s, v, bv, x, y, st); break;
+    case 8: launch_multi_k<T, CPT, 8>(s, v, bv, x, y, st); break;
+    default: throw std::runtime_error("multi-vector tile kernels take K = 1, 2, 4 or 8");
+  }
+}
+
+bool multi_fits(const bs_system* s, int v, int K, size_t tsize) {
+  if (v == 0) return true;
+  if (K != 1 && K != 2 && K != 4 && K != 8) return false;
+  const int G = (v - 1) / 2 == 0 ? 1 : 2;
+  return (size_t)G * K * s->dt.lmax * tsize + (size_t)s->dt.wcols * G * K * sizeof(float) <= kLdsLimit;
+}
+
 template <typename T>
-void launch_batched(const bs_system* s, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+void launch_batched_v(const bs_system* s, int v, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
   const DevTables& d = s->dt;
   if (bv.K < 1 || bv.K > kMaxBatch) throw std::runtime_error("batch size must be in 1..64");
-  const int ny = (d.lmax + 3) / 4;
-  hipLaunchKernelGGL(k_apply_batched<T>, dim3(d.nP, ny), dim3(256), 0, st, d, bv, x, y);
+  if (v == 0) {
+    const int ny = (d.lmax + 3) / 4;
+    hipLaunchKernelGGL(k_apply_batched<T>, dim3(d.nP, ny), dim3(256), 0, st, d, bv, x, y);
+    HIP_OK(hipGetLastError());
+    return;
+  }
+  switch (s->cpt) {
+    case 1: launch_multi_cpt<T, 1>(s, v, bv, x, y, st); break;
+    case 2: launch_multi_cpt<T, 2>(s, v, bv, x, y, st); break;
+    case 4: launch_multi_cpt<T, 4>(s, v, bv, x, y, st); break;
+    default: launch_multi_cpt<T, 8>(s, v, bv, x, y, st); break;
+  }
+}
+
+// The multi-vector variant for (dtype, K, shared values): timed on first use
+// (two launches each on scratch vectors) and kept.  Never called under
+// stream capture.
+template <typename T>
+int batch_variant(bs_system* s, const BatchVals& bv) {
+  const int key = (sizeof(T) == 8 ? 1000 : 0) + (bv.vk ? 0 : 100) + bv.K;
+  auto it = s->multi_choice.find(key);
+  if (it != s->multi_choice.end()) return it->second;
+  const int64_t n = s->t.dim * bv.K;
+  DevBuf<T> x, y;
+  x.alloc(n);
+  y.alloc(n);
+  k_fill<T><<<grid_for(n), kBlock>>>(x.p, (T)0.5, n);
   HIP_OK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestv = 0;
+  for (int v = 0; v < kNumMulti; v++) {
+    if (!multi_fits(s, v, bv.K, sizeof(T))) continue;
+    launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);
+    HIP_OK(hipEventRecord(e0, nullptr));
+    launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);
+    HIP_OK(hipEventRecord(e1, nullptr));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    s->multi_ms[key][v] = ms;
+    if (ms < best) { best = ms; bestv = v; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->multi_choice[key] = bestv;
+  return bestv;
+}
+
+template <typename T>
+void launch_batched(bs_system* s, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  launch_batched_v<T>(s, batch_variant<T>(s, bv), bv, x, y, st);
+}
+
+// the chosen variant, for launches recorded under stream capture
+template <typename T>
+void launch_batched_chosen(const bs_system* s, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
+  const int key = (sizeof(T) == 8 ? 1000 : 0) + (bv.vk ? 0 : 100) + bv.K;
+  auto it = s->multi_choice.find(key);
+  if (it == s->multi_choice.end()) throw std::runtime_error("multi-vector variant not chosen before capture");
+  launch_batched_v<T>(s, it->second, bv, x, y, st);
 }
 
 void batch_upload_values(bs_batch* b) {
@@ -1766,24 +2031,22 @@ BatchVals shared_vals(const bs_system* s, int K) {
   return v;
 }
 
-// Enqueue one whole batched Lanczos solve on stream st (no host
-// synchronisation, no memset: it is captured into a graph).  Every
-// Hamiltonian runs the same number of iterations in lockstep; alpha/beta and
-// the Gram-Schmidt dots are FP64 per Hamiltonian.
+// Lanczos steps i0..i1-1 for all K Hamiltonians in lockstep (FP64
+// alpha/beta and Gram-Schmidt dots per Hamiltonian), enqueued on st with no
+// host synchronisation and no memset, so that it can be captured.
 template <typename T>
-void enqueue_batch_lanczos(bs_batch* b, bs_batch::Work& w, hipStream_t st) {
+void enqueue_batch_steps(bs_batch* b, bs_batch::Work& w, int i0, int i1, hipStream_t st) {
   const bs_system* s = b->base.get();
   const int K = b->K;
   const int64_t D = s->t.dim, DK = D * K;
-  const int niter = w.niter, nkeep = w.nkeep;
   const int nb = batch_blocks(D);
   const int g = grid_for(DK);
   T* V = (T*)w.V.p;
   T* wv = (T*)w.w.p;
-  k_fill<T><<<g, kBlock, 0, st>>>(V, (T)(1.0 / std::sqrt((double)D)), DK);
-  for (int i = 0; i < niter; i++) {
+  if (i0 == 0) k_fill<T><<<g, kBlock, 0, st>>>(V, (T)(1.0 / std::sqrt((double)D)), DK);
+  for (int i = i0; i < i1; i++) {
     T* vi = V + (int64_t)i * DK;
-    launch_batched<T>(s, b->bv, vi, wv, st);
+    launch_batched_chosen<T>(s, b->bv, vi, wv, st);
     k_bmultidot<T><<<nb, 256, 0, st>>>(vi, DK, 1, wv, D, K, w.part.p);
     k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.alpha.p + (int64_t)i * K, 0);
     k_bsub_scaled<T><<<g, kBlock, 0, st>>>(wv, vi, w.alpha.p + (int64_t)i * K, D, K);
@@ -1797,18 +2060,149 @@ void enqueue_batch_lanczos(bs_batch* b, bs_batch::Work& w, hipStream_t st) {
     k_bnormalize<T><<<g, kBlock, 0, st>>>(V + (int64_t)(i + 1) * DK, wv, w.beta.p + (int64_t)i * K, D, K);
   }
   HIP_OK(hipGetLastError());
-  k_tridiag_ritz<<<(K + 63) / 64, 64, 0, st>>>(w.alpha.p, w.beta.p, niter, K, nkeep, w.evals.p, w.Y.p, w.tdwork.p);
-  k_bcombine<T, double><<<g, kBlock, 0, st>>>((double*)w.X64.p, V, DK, niter, w.Y.p, nkeep, D, K);
+}
+
+// Projected matrix of a (restarted) Krylov cycle for each Hamiltonian: the
+// locked Ritz values on the diagonal with their couplings in column kstart
+// (src/blanczoslib.f90:143-222), then the tridiagonal part.  A[k][m][m].
+__global__ void k_projected(const double* __restrict__ alpha, const double* __restrict__ beta,
+                            const double* __restrict__ arrow, int kstart, int m, int K, double* __restrict__ A) {
+  const int k = blockIdx.y;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < m * m; idx += gridDim.x * blockDim.x) {
+    const int i = idx / m, j = idx - i * m;
+    double v = 0.0;
+    if (i == j) v = alpha[(int64_t)i * K + k];
+    else if (i >= kstart && j >= kstart && (j == i + 1 || i == j + 1)) v = beta[(int64_t)min(i, j) * K + k];
+    else if (j == kstart && i < kstart) v = arrow[(int64_t)i * K + k];
+    else if (i == kstart && j < kstart) v = arrow[(int64_t)j * K + k];
+    A[((int64_t)k * m + i) * m + j] = v;
+  }
+}
+
+// Lowest nout eigenpairs of each A[k] (cyclic Jacobi, FP64, one thread per
+// matrix; work holds m*m per matrix).  evals[k*nout + n], Y[(k*m + i)*nout + n].
+__global__ void k_jacobi_ritz(double* __restrict__ A, int m, int K, int nout, double* __restrict__ evals,
+                              double* __restrict__ Y, double* __restrict__ work) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double* a = A + (int64_t)k * m * m;
+  double* v = work + (int64_t)k * m * m;
+  for (int i = 0; i < m * m; i++) v[i] = 0.0;
+  for (int i = 0; i < m; i++) v[i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) {
+        const double t = a[i * m + j] * a[i * m + j];
+        tot += t;
+        if (i != j) off += t;
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < m - 1; p++)
+      for (int q = p + 1; q < m; q++) {
+        const double apq = a[p * m + q];
+        if (fabs(apq) < 1e-300) continue;
+        const double theta = (a[q * m + q] - a[p * m + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < m; r++) {
+          const double arp = a[r * m + p], arq = a[r * m + q];
+          a[r * m + p] = c * arp - sn * arq;
+          a[r * m + q] = sn * arp + c * arq;
+        }
+        for (int r = 0; r < m; r++) {
+          const double apr = a[p * m + r], aqr = a[q * m + r];
+          a[p * m + r] = c * apr - sn * aqr;
+          a[q * m + r] = sn * apr + c * aqr;
+        }
+        for (int r = 0; r < m; r++) {
+          const double vrp = v[r * m + p], vrq = v[r * m + q];
+          v[r * m + p] = c * vrp - sn * vrq;
+          v[r * m + q] = sn * vrp + c * vrq;
+        }
+      }
+  }
+  // the nout smallest diagonal entries, ascending (selection)
+  unsigned long long used[4] = {0, 0, 0, 0};  // m <= 256
+  for (int n = 0; n < nout; n++) {
+    int best = -1;
+    for (int i = 0; i < m; i++) {
+      if ((used[i >> 6] >> (i & 63)) & 1ull) continue;
+      if (best < 0 || a[i * m + i] < a[best * m + best]) best = i;
+    }
+    used[best >> 6] |= 1ull << (best & 63);
+    evals[(int64_t)k * nout + n] = a[best * m + best];
+    for (int i = 0; i < m; i++) Y[((int64_t)k * m + i) * nout + n] = v[i * m + best];
+  }
+}
+
+// After a thick restart: locked Ritz values become alpha[0..thick), their
+// couplings to the carried-over Krylov vector arrow[j] = beta_{m-1} Y[m-1, j].
+__global__ void k_restart_coeffs(double* __restrict__ alpha, const double* __restrict__ beta,
+                                 double* __restrict__ arrow, const double* __restrict__ evals,
+                                 const double* __restrict__ Y, int m, int K, int nout, int thick) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= thick * K) return;
+  const int j = idx / K, k = idx - j * K;
+  alpha[(int64_t)j * K + k] = evals[(int64_t)k * nout + j];
+  arrow[(int64_t)j * K + k] = beta[(int64_t)(m - 1) * K + k] * Y[((int64_t)k * m + m - 1) * nout + j];
+}
+
+template <typename T>
+__global__ void k_copy(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Ritz pairs of the current projected matrices (Sturm bisection / inverse
+// iteration for a plain tridiagonal cycle, Jacobi for an arrowhead one).
+void enqueue_batch_ritz(bs_batch::Work& w, int K, int kstart, int nout, hipStream_t st) {
+  const int m = w.niter;
+  if (kstart == 0 && w.thick == 0) {
+    k_tridiag_ritz<<<(K + 63) / 64, 64, 0, st>>>(w.alpha.p, w.beta.p, m, K, nout, w.evals.p, w.Y.p, w.tdwork.p);
+  } else {
+    k_projected<<<dim3((m * m + 255) / 256, K), 256, 0, st>>>(w.alpha.p, w.beta.p, w.arrow.p, kstart, m, K, w.A.p);
+    k_jacobi_ritz<<<(K + 63) / 64, 64, 0, st>>>(w.A.p, m, K, nout, w.evals.p, w.Y.p, w.tdwork.p);
+  }
   HIP_OK(hipGetLastError());
+}
+
+// thick restart: V[0:thick] = V[0:m] Y[:, 0:thick], V[thick] = V[m]
+template <typename T>
+void enqueue_batch_restart(bs_batch* b, bs_batch::Work& w, int nout, hipStream_t st) {
+  const int K = b->K, m = w.niter, thick = w.thick;
+  const int64_t DK = b->base->t.dim * K;
+  const int g = grid_for(DK);
+  T* V = (T*)w.V.p;
+  T* tmp = (T*)w.tmp.p;
+  k_bcombine<T, T><<<g, kBlock, 0, st>>>(tmp, V, DK, m, w.Y.p, nout, thick, b->base->t.dim, K);
+  k_copy<T><<<grid_for((int64_t)thick * DK), kBlock, 0, st>>>(V, tmp, (int64_t)thick * DK);
+  k_copy<T><<<g, kBlock, 0, st>>>(V + (int64_t)thick * DK, V + (int64_t)m * DK, DK);
+  k_restart_coeffs<<<(thick * K + 255) / 256, 256, 0, st>>>(w.alpha.p, w.beta.p, w.arrow.p, w.evals.p, w.Y.p, m, K,
+                                                            nout, thick);
+  HIP_OK(hipGetLastError());
+}
+
+// Ritz vectors in FP64 from the last Ritz solve, normalised, then the FP64
+// Rayleigh quotient and residual (polish)
+template <typename T>
+void enqueue_batch_finish(bs_batch* b, bs_batch::Work& w, int nout, hipStream_t st) {
+  const bs_system* s = b->base.get();
+  const int K = b->K, nkeep = w.nkeep;
+  const int64_t D = s->t.dim, DK = D * K;
+  const int nb = batch_blocks(D);
+  const int g = grid_for(DK);
   double* X = (double*)w.X64.p;
   double* HX = (double*)w.HX.p;
+  k_bcombine<T, double><<<g, kBlock, 0, st>>>(X, (const T*)w.V.p, DK, w.niter, w.Y.p, nout, nkeep, D, K);
+  HIP_OK(hipGetLastError());
   for (int q = 0; q < nkeep; q++) {
     double* xq = X + (int64_t)q * DK;
     k_bmultidot<double><<<nb, 256, 0, st>>>(xq, DK, 1, xq, D, K, w.part.p);
     k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.dots.p, 1);
     k_bnormalize<double><<<g, kBlock, 0, st>>>(xq, xq, w.dots.p, D, K);
     if (w.polish) {
-      launch_batched<double>(s, b->bv, xq, HX, st);
+      launch_batched_chosen<double>(s, b->bv, xq, HX, st);
       k_bmultidot<double><<<nb, 256, 0, st>>>(xq, DK, 1, HX, D, K, w.part.p);
       k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.rq.p + (int64_t)q * K, 0);
       k_bsub_scaled<double><<<g, kBlock, 0, st>>>(HX, xq, w.rq.p + (int64_t)q * K, D, K);
@@ -1819,6 +2213,25 @@ void enqueue_batch_lanczos(bs_batch* b, bs_batch::Work& w, hipStream_t st) {
   HIP_OK(hipGetLastError());
 }
 
+template <typename F>
+hipGraphExec_t capture(hipStream_t st, F&& body) {
+  hipGraph_t graph = nullptr;
+  HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIP_OK(hipStreamEndCapture(st, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_OK(ie);
+  return exec;
+}
+
 template <typename T>
 void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,
                        void* evecs, double* resid, int* iters) {
@@ -1826,52 +2239,85 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
   const int64_t D = b->base->t.dim, DK = D * K;
   const int niter = (int)std::min<int64_t>(p->niter, D);
   const int nkeep = p->nkeep;
+  const int thick = p->thick;
   if (nkeep < 1 || nkeep > 16) throw std::runtime_error("nkeep must be in 1..16");
   if (niter < nkeep) throw std::runtime_error("niter must be >= nkeep");
-  if (p->thick != 0) throw std::runtime_error("batched Lanczos runs a fixed number of iterations (thick must be 0)");
+  if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter || thick > 16)))
+    throw std::runtime_error("thick must be 0 or in [nkeep, min(niter, 17))");
+  if (thick > 0 && niter > 256) throw std::runtime_error("thick restart takes niter <= 256");
+  const int nout = std::max(nkeep, thick);
   const int dt = sizeof(T) == 4 ? BS_F32 : BS_F64;
   std::unique_ptr<bs_batch::Work>& wp = b->work;
-  if (!wp || wp->dtype != dt || wp->niter != niter || wp->nkeep != nkeep || wp->polish != (p->polish ? 1 : 0)) {
+  if (!wp || wp->dtype != dt || wp->niter != niter || wp->nkeep != nkeep || wp->polish != (p->polish ? 1 : 0) ||
+      wp->thick != thick) {
     wp.reset(new bs_batch::Work());
     bs_batch::Work& w = *wp;
     w.dtype = dt;
     w.niter = niter;
     w.nkeep = nkeep;
+    w.thick = thick;
     w.polish = p->polish ? 1 : 0;
     const int nb = batch_blocks(D);
     w.V.alloc((size_t)(niter + 1) * DK * sizeof(T));
     w.w.alloc((size_t)DK * sizeof(T));
     w.X64.alloc((size_t)nkeep * DK * sizeof(double));
     w.HX.alloc((size_t)DK * sizeof(double));
+    if (thick) w.tmp.alloc((size_t)thick * DK * sizeof(T));
     w.part.alloc((size_t)nb * niter * K);
     w.dots.alloc((size_t)niter * K);
     w.alpha.alloc((size_t)niter * K);
     w.beta.alloc((size_t)niter * K);
-    w.evals.alloc((size_t)K * nkeep);
-    w.Y.alloc((size_t)K * niter * nkeep);
-    w.tdwork.alloc((size_t)K * 6 * niter);
+    w.evals.alloc((size_t)K * nout);
+    w.Y.alloc((size_t)K * niter * nout);
+    w.tdwork.alloc((size_t)K * std::max(6 * niter, thick ? niter * niter : 0));
     w.rq.alloc((size_t)nkeep * K);
     w.res.alloc((size_t)nkeep * K);
-    HIP_OK(hipStreamCreate(&w.stream));
-    hipGraph_t graph = nullptr;
-    HIP_OK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
-    try {
-      enqueue_batch_lanczos<T>(b, w, w.stream);
-    } catch (...) {
-      (void)hipStreamEndCapture(w.stream, &graph);
-      if (graph) (void)hipGraphDestroy(graph);
-      throw;
+    if (thick) {
+      w.A.alloc((size_t)K * niter * niter);
+      w.arrow.alloc((size_t)std::max(thick, 1) * K);
     }
-    HIP_OK(hipStreamEndCapture(w.stream, &graph));
-    const hipError_t ie = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIP_OK(ie);
+    HIP_OK(hipStreamCreate(&w.stream));
+    batch_variant<T>(b->base.get(), b->bv);
+    if (w.polish) batch_variant<double>(b->base.get(), b->bv);
+    const hipStream_t st = w.stream;
+    w.exec[0] = capture(st, [&] {
+      enqueue_batch_steps<T>(b, w, 0, niter, st);
+      enqueue_batch_ritz(w, K, 0, nout, st);
+    });
+    if (thick) {
+      w.exec[1] = capture(st, [&] {
+        enqueue_batch_steps<T>(b, w, thick, niter, st);
+        enqueue_batch_ritz(w, K, thick, nout, st);
+      });
+      w.exec[2] = capture(st, [&] { enqueue_batch_restart<T>(b, w, nout, st); });
+    }
+    w.exec[3] = capture(st, [&] { enqueue_batch_finish<T>(b, w, nout, st); });
   }
   bs_batch::Work& w = *wp;
-  HIP_OK(hipGraphLaunch(w.exec, w.stream));
-  HIP_OK(hipStreamSynchronize(w.stream));
-  std::vector<double> ev((size_t)K * nkeep), rq((size_t)nkeep * K), rs((size_t)nkeep * K);
+  const hipStream_t st = w.stream;
+  int matvecs = niter;
+  HIP_OK(hipGraphLaunch(w.exec[0], st));
+  std::vector<double> ev((size_t)K * nout), prev;
+  HIP_OK(hipStreamSynchronize(st));
   HIP_OK(hipMemcpy(ev.data(), w.evals.p, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
+  for (int restarts = 0; thick > 0 && restarts < p->max_restarts; restarts++) {
+    if (p->tol > 0.0 && !prev.empty()) {
+      double dmax = 0.0;
+      for (int k = 0; k < K; k++)
+        for (int q = 0; q < nkeep; q++)
+          dmax = std::max(dmax, std::fabs(ev[(size_t)k * nout + q] - prev[(size_t)k * nout + q]));
+      if (dmax < p->tol) break;
+    }
+    prev = ev;
+    HIP_OK(hipGraphLaunch(w.exec[2], st));
+    HIP_OK(hipGraphLaunch(w.exec[1], st));
+    matvecs += niter - thick;
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipMemcpy(ev.data(), w.evals.p, sizeof(double) * ev.size(), hipMemcpyDeviceToHost));
+  }
+  HIP_OK(hipGraphLaunch(w.exec[3], st));
+  HIP_OK(hipStreamSynchronize(st));
+  std::vector<double> rq((size_t)nkeep * K), rs((size_t)nkeep * K);
   if (w.polish) {
     HIP_OK(hipMemcpy(rq.data(), w.rq.p, sizeof(double) * rq.size(), hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(rs.data(), w.res.p, sizeof(double) * rs.size(), hipMemcpyDeviceToHost));
@@ -1888,7 +2334,7 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
   }
   for (int k = 0; k < K; k++)
     for (int q = 0; q < nkeep; q++) {
-      evals[(size_t)k * nkeep + q] = w.polish ? rq[(size_t)q * K + k] : ev[(size_t)k * nkeep + q];
+      evals[(size_t)k * nkeep + q] = w.polish ? rq[(size_t)q * K + k] : ev[(size_t)k * nout + q];
       if (resid) resid[(size_t)k * nkeep + q] = w.polish ? rs[(size_t)q * K + k] : -1.0;
     }
   if (evecs) {
@@ -1896,12 +2342,12 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
       HIP_OK(hipMemcpy(evecs, w.X64.p, sizeof(double) * (size_t)nkeep * DK, hipMemcpyDeviceToDevice));
     else {
       const int64_t n = (int64_t)nkeep * DK;
-      k_convert<double, float><<<grid_for(n), kBlock, 0, w.stream>>>((float*)evecs, (const double*)w.X64.p, n);
+      k_convert<double, float><<<grid_for(n), kBlock, 0, st>>>((float*)evecs, (const double*)w.X64.p, n);
       HIP_OK(hipGetLastError());
-      HIP_OK(hipStreamSynchronize(w.stream));
+      HIP_OK(hipStreamSynchronize(st));
     }
   }
-  if (iters) *iters = niter + (w.polish ? nkeep : 0);
+  if (iters) *iters = matvecs + (w.polish ? nkeep : 0);
 }
 
 // lane k of an interleaved set -> contiguous FP64 vector
@@ -2221,6 +2667,28 @@ int bs_apply_block(bs_system* s, const void* x, void* y, int K, int dtype) {
   } catch (const std::exception& e) {
     return set_err(e.what());
   }
+}
+
+int bs_multi_variant(bs_system* s, int dtype, int K, int shared, int v) {
+  try {
+    checked(s);
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (K < 1 || K > kMaxBatch) return set_err("K must be in 1..64");
+    if (v < 0 || v >= kNumMulti) return set_err("no such multi-vector variant");
+    if (!multi_fits(s, v, K, dtype == BS_F32 ? 4 : 8)) return set_err("variant does not fit for this system and K");
+    const int key = (dtype == BS_F64 ? 1000 : 0) + (shared ? 100 : 0) + K;
+    s->multi_choice[key] = v;
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_batch_set_variant(bs_batch* b, int dtype, int v) {
+  if (!b) return set_err("null batch");
+  b->work.reset();  // a captured graph holds the old choice
+  return bs_multi_variant(b->base.get(), dtype, b->K, 0, v);
 }
 
 int bs_batch_init(const char* sps, const char* const* intfiles, int K, int Z, int N, int jz2, float spscale,

@@ -59,6 +59,8 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
+    L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
+    L.bs_batch_set_variant.argtypes = [vp, i32, i32]
     L.bs_batch_init.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), i32, i32, i32, i32,
                                 f32, f32, f32, f32, ctypes.POINTER(vp)]
     L.bs_batch_finalize.argtypes = [vp]
@@ -271,6 +273,13 @@ class System:
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
 
+    def set_block_variant(self, dtype, K, v):
+        """Force multi-vector kernel variant v (0..4) for apply_block with K
+        vectors; False if it does not fit."""
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        return load_library().bs_multi_variant(self.h, dt, K, 1, v) == 0
+
     def apply_block(self, x, y=None):
         """y[i, k] = (H x)[i, k] for a (dim, K) CUDA tensor of K vectors, K <= 64."""
         import torch
@@ -325,6 +334,11 @@ class Batch:
         _check(load_library().bs_num_params(sysh, ctypes.byref(a), ctypes.byref(b)))
         return a.value + b.value
 
+    def set_variant(self, dtype, v):
+        """Force multi-vector kernel variant v (0..4); False if it does not fit."""
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        return load_library().bs_batch_set_variant(self.h, dt, v) == 0
+
     def apply(self, x, y=None):
         """y[:, k] = H_k x[:, k] for a (dim, K) CUDA tensor."""
         import torch
@@ -340,10 +354,11 @@ class Batch:
         torch.cuda.synchronize()
         return y
 
-    def lanczos(self, nkeep, niter, dtype="f32", polish=True, want_vectors=False):
+    def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
+                want_vectors=False):
         import torch
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
-        p = _Params(nkeep, niter, 0, 0, dt, 0.0, int(polish))
+        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish))
         ev = np.zeros((self.K, nkeep))
         res = np.zeros((self.K, nkeep))
         m = min(niter, self.dim)

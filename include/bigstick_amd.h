@@ -165,6 +165,13 @@ int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double
  * (src/bapplyhlibOMP_block.f90:43-887, vectors block(ivec,istate)). */
 int bs_apply_block(bs_system *sys, const void *x, void *y, int K, int dtype);
 
+/* Multi-vector kernel variant for (dtype, K, shared: 1 = one Hamiltonian as
+ * in bs_apply_block, 0 = one per vector as in bs_batch_apply): 0 = one wave
+ * per basis state, 1..4 = row tiles with G = 1 (1, 2) or 2 (3, 4) staged
+ * source rows, identity (odd) or XCD-aware (even) order.  Without a call the
+ * fastest is timed and kept on first use.  Nonzero if it does not fit. */
+int bs_multi_variant(bs_system *sys, int dtype, int K, int shared, int v);
+
 /* K Hamiltonians on one basis: one iso .int file each, same orbits, particle
  * numbers and scaling line (src/binput_tbme.f90:898-1250 per file).  Jump
  * tables are built once; values are stored per Hamiltonian.  No counterpart
@@ -177,6 +184,8 @@ int bs_batch_size(const bs_batch *b, int *K, int64_t *dim);
 /* The system holding the shared basis and jump tables (values of the first
  * interaction); owned by the batch. */
 bs_system *bs_batch_system(bs_batch *b);
+/* bs_multi_variant for the batch's own K (drops a captured Lanczos graph). */
+int bs_batch_set_variant(bs_batch *b, int dtype, int v);
 /* y_k = H_k x_k, interleaved device vectors. */
 int bs_batch_apply(bs_batch *b, const void *x, void *y, int dtype);
 /* Lanczos on all K Hamiltonians in lockstep (the recurrence of bs_lanczos,

@@ -75,12 +75,15 @@ def test_autograd_function_config0(casedir):
     assert np.abs(e.detach().numpy() - fx["res"][:2, 0]).max() < 2e-5
     (e[0] + e[1]).backward()
     g = theta.grad.numpy()
+    # FP32 matrix elements put rounding steps of ~1e-8 MeV into E(theta):
+    # up to ~1e-3 of noise in the quotient at h = 1e-4, ~1e-6 at h = 1e-2
     for p in (0, 2, 5, 40):
-        tp = theta.detach().clone()
-        tm = theta.detach().clone()
-        tp[p] = f32(float(tp[p]) + 1e-4)
-        tm[p] = f32(float(tm[p]) - 1e-4)
-        ep = eigenenergies(tp, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
-        em = eigenenergies(tm, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
-        fd = float((ep.sum() - em.sum()) / (tp[p] - tm[p]))
-        assert abs(fd - g[p]) < 1e-4, (p, fd, g[p])
+        for h, tol in ((1e-4, 2e-3), (1e-2, 5e-5)):
+            tp = theta.detach().clone()
+            tm = theta.detach().clone()
+            tp[p] = f32(float(tp[p]) + h)
+            tm[p] = f32(float(tm[p]) - h)
+            ep = eigenenergies(tp, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
+            em = eigenenergies(tm, case.interaction, case.Z, case.N, case.jz2, nkeep=2, niter=150, dtype="f64")
+            fd = float((ep.sum() - em.sum()) / (tp[p] - tm[p]))
+            assert abs(fd - g[p]) < tol, (p, h, fd, g[p])
