@@ -117,13 +117,17 @@ def test_observables_and_host_apply(casedir):
     (.res of config 0); the host-pointer apply equals the device apply."""
     case, d, fx = config_fixture(0, casedir)
     s = _sys(case, d)
-    r = s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True, want_vectors=True)
+    # converged vectors; J^2 and T^2 are built, as in the reference, from FP32
+    # elements (|elements| up to ~10), so <J^2> carries ~1e-6 relative error
+    r = s.lanczos(nkeep=5, niter=40, thick=10, max_restarts=30, dtype="f64", tol=1e-10, polish=True,
+                  want_vectors=True)
+    assert r.resid[:3].max() < 1e-8
     for k in range(3):
         j2 = s.observable("J2", r.evecs[k])
         t2 = s.observable("T2", r.evecs[k])
         J, T = fx["res"][k, 1], fx["res"][k, 2]
-        assert abs(j2 - J * (J + 1)) < 1e-6
-        assert abs(t2 - T * (T + 1)) < 1e-6
+        assert abs(j2 - J * (J + 1)) < 5e-5, (k, j2, J)
+        assert abs(t2 - T * (T + 1)) < 5e-5, (k, t2, T)
     x = np.random.default_rng(9).standard_normal(s.dim)
     yd = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
     yh = s.apply_host(x)
