@@ -44,20 +44,60 @@ def parse():
                     help="also time K Hamiltonians on the same basis (the workload's interaction and K-1 "
                          "more seeded ones) as one multi-vector matvec; with --solve, the batched "
                          "mixed-precision eigensolve with gradients, scored against FP64")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="force this kernel variant (index into System.variants()) instead of autotuning")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="compute HBM bytes per launch of the H x kernel from two rocprofv3 --pmc result "
+                         "databases (FETCH_SIZE, WRITE_SIZE; runs made with --copy-probe --variant V), "
+                         "write profiles/k_apply_tiles_pf6p6n_traffic.json and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, copy x into y once (a plain copy kernel of 8*D bytes) "
                          "so hardware-counter runs can calibrate FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), scaled by a copy-kernel calibration; committed under profiles/."""
-    p = os.path.join(ROOT, "profiles", "apply_rows_pf6p6n_traffic.json")
-    if not os.path.exists(p):
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "k_apply_tiles_pf6p6n_traffic.json")
+
+
+def traffic_from_profiles(variant):
+    """HBM bytes per launch of the given kernel variant, measured with
+    rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and scaled by a copy-kernel
+    calibration; committed under profiles/."""
+    if not os.path.exists(TRAFFIC_JSON):
         return None
-    with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+    with open(TRAFFIC_JSON) as f:
+        t = json.load(f)
+    if t.get("variant") != list(variant):
+        return None
+    return t.get("bytes_per_launch")
+
+
+def traffic_from_dbs(fetch_db, write_db, D):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the H x kernel, scaled by
+    what the same counters read for the copy probe (one plain copy of 4*D
+    bytes read and 4*D written, the last copy dispatch of the run)."""
+    import sqlite3
+    out = {}
+    for key, db in (("fetch", fetch_db), ("write", write_db)):
+        c = sqlite3.connect(db)
+        rows = list(c.execute("select dispatch_id, kernel_name, value, duration from counters_collection "
+                              "order by dispatch_id"))
+        kern = [r for r in rows if "k_apply_tiles<float" in r[1]]
+        copies = [r for r in rows if "copy" in r[1].lower() and r[2] > 1e5]
+        if not kern or not copies:
+            raise RuntimeError("no H x or copy-probe dispatch in " + db)
+        names = sorted(set(r[1] for r in kern))
+        if len(names) != 1:
+            raise RuntimeError("several H x kernels in %s (run with --variant)" % db)
+        probe = copies[-1]
+        scale = 4.0 * D / (probe[2] * 1024.0)
+        vals = [r[2] for r in kern]
+        out[key] = {"kib_raw": vals, "probe_kib": probe[2], "probe_kernel": probe[1], "scale": scale,
+                    "bytes_per_launch": sum(vals) / len(vals) * 1024.0 * scale,
+                    "mean_duration_ns": sum(r[3] for r in kern) / len(kern)}
+        out["kernel"] = names[0]
+    out["bytes_per_launch"] = out["fetch"]["bytes_per_launch"] + out["write"]["bytes_per_launch"]
+    return out
 
 
 def cpu_baseline(sps, intf, case, target_s=15.0):
@@ -138,7 +178,7 @@ def solve_leg(s, case, max_restarts):
     return out
 
 
-def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work):
+def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work, n_ops):
     """K Hamiltonians on one basis: interleaved multi-vector matvec (every
     jump read serves K vectors) and, with solve, the lockstep thick-restart
     Lanczos for all K in mixed precision plus Hellmann-Feynman gradients of
@@ -179,7 +219,14 @@ def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work):
     ev1.record()
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / steps
-    out = {"K": K, "build_s": t_build, "kernel_ms": ms, "hv_per_s": K / (ms * 1e-3)}
+    chosen, tms = b.variant_info("f32")
+    out = {"K": K, "build_s": t_build, "kernel_ms": ms, "hv_per_s": K / (ms * 1e-3),
+           "variant": chosen, "variant_ms": [round(float(v), 3) for v in tms]}
+    # explicit-CSR model with K value columns: 4 B index + 4K B values per
+    # stored (half) element, read x / read-write y for K vectors
+    b_alg = n_ops / 2 * (4 + 4 * K) + 3 * 4 * D * K
+    out["b_alg_bytes"] = b_alg
+    out["hbm_gbs_alg"] = b_alg / (ms * 1e-3) / 1e9
     del x, y
     torch.cuda.empty_cache()
     if solve:
@@ -217,6 +264,26 @@ def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work):
 
 def main():
     a = parse()
+    if a.traffic:
+        from bigstickpublick_amd.inputs import config_case
+        from bigstickpublick_amd import System
+        import re
+        case = config_case(a.config)
+        work = tempfile.mkdtemp(prefix="bsbench_")
+        from bigstickpublick_amd.inputs import write_case
+        write_case(case, work)
+        s = System(os.path.join(work, case.space + ".sps"), os.path.join(work, case.name + ".int"),
+                   case.Z, case.N, case.jz2)
+        t = traffic_from_dbs(a.traffic[0], a.traffic[1], s.dim)
+        m = re.search(r"k_apply_tiles<float, 512, (\d+), (\d+), (\d+)>", t["kernel"])
+        t["workload"] = "%s M=0 D=%d FP32" % (case.name, s.dim)
+        t["variant"] = [int(m.group(2)), int(m.group(3)), a.variant if a.variant >= 0 else None]
+        if a.variant >= 0:
+            t["variant"] = list(System.variants()[a.variant])
+        with open(TRAFFIC_JSON, "w") as f:
+            json.dump(t, f, indent=1)
+        print(json.dumps({"bytes_per_launch": t["bytes_per_launch"], "kernel": t["kernel"]}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
@@ -238,7 +305,12 @@ def main():
     torch.cuda.synchronize()
     t_upload = time.time() - t0
     t0 = time.time()
-    variant = s.autotune("f32")
+    if a.variant >= 0:
+        if not s.set_variant("f32", a.variant):
+            raise SystemExit("kernel variant %d does not fit this system" % a.variant)
+        variant = a.variant
+    else:
+        variant = s.autotune("f32")
     t_tune = time.time() - t0
     D = s.dim
     ops = s.op_counts()
@@ -268,7 +340,7 @@ def main():
     ms_per_step = wall * 1000.0 / a.steps
     b_alg = n_ops / 2 * 8 + 3 * 4 * D
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles(System.variants()[variant])
     out = {
         "metric": METRIC,
         "value": a.steps / wall,
@@ -294,6 +366,7 @@ def main():
         "setup_s": {"host_tables": t_host, "upload": t_upload, "autotune": t_tune},
         "autotune_ms": {"R%d_G%d_xcd%d" % v: round(float(t), 3)
                         for v, t in zip(System.variants(), s.autotune_times("f32")) if t > 0},
+        "traffic_source": os.path.relpath(TRAFFIC_JSON, ROOT) if traffic else None,
         "kernel_variant": {"rows_per_tile": System.variants()[variant][0],
                            "staged_rows": System.variants()[variant][1],
                            "xcd_map": System.variants()[variant][2]},
@@ -307,7 +380,7 @@ def main():
     if a.batch:
         s.close()
         torch.cuda.empty_cache()
-        out["batch"] = batch_leg(sps, case, a.batch, a.steps, a.warmup, a.solve, a.max_restarts, work)
+        out["batch"] = batch_leg(sps, case, a.batch, a.steps, a.warmup, a.solve, a.max_restarts, work, n_ops)
         if a.solve and "solve" in out:
             out["batch"]["speedup_vs_fp64_single"] = (out["batch"]["solve"]["mixed_batched"]["eigenpairs_per_s"]
                                                       / out["solve"]["fp64"]["eigenpairs_per_s"])

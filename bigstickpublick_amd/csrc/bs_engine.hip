@@ -90,7 +90,8 @@ struct DevTables {
   // point at the zero column nn_local = size of the class
   const int64_t* n1_base;
   const int32_t* n1_emax;
-  const uint32_t* n1_pk;
+  const uint32_t* n1_pk;      // src | nl << 16 | sign << 24
+  const uint32_t* n1_pkv[3][2];  // per G (1, 2, 4) and T (f32, f64): LDS byte offsets, see k_apply_tiles
   // pn element blocks: class k block is [size_k][size_-k] at woff[k]
   const float* W;
   const int32_t* woff;
@@ -103,6 +104,18 @@ template <typename T> __device__ __forceinline__ T vload(const float* a, const d
 template <> __device__ __forceinline__ float vload<float>(const float* a, const double*, int64_t i) { return a[i]; }
 template <> __device__ __forceinline__ double vload<double>(const float*, const double* b, int64_t i) { return b[i]; }
 
+// Raw buffer loads: a uniform (scalar) descriptor and byte offset plus a
+// per-lane 32-bit byte offset, so table walks cost no vector address math.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // G values of one LDS column, read with one wide LDS load.
 template <typename T, int G>
 struct alignas(sizeof(T) * G) VecG {
@@ -110,18 +123,28 @@ struct alignas(sizeof(T) * G) VecG {
 };
 
 // One workgroup per tile of up to R proton rows of one sector; BLOCK threads,
-// CPT columns per thread (column = tid + q*BLOCK).  Dynamic LDS holds staged
-// x rows stored transposed -- column-major [lmax][max(R,G)], so one wide load
-// fetches every staged row at a source column -- followed by the batch's
-// coefficient table Wb[nl][g] = phase_g * W_k[pp_g][nl] (zero row at the
-// padding index).
+// CPT columns per thread (column = tid + q*BLOCK, clamped to the row for
+// loads; only columns < L are stored).  Dynamic LDS, in bytes:
+//   [0, kWbBytes*G)    Wb[2*nl + s][g] = (-1)^s * phase_g * W_k[pp_g][nl],
+//                      the batch's coefficients with the neutron jump's sign
+//                      folded in (rows 2*wsd, 2*wsd + 1: zero, for padding)
+//   [kWbBytes*G, ...)  staged x rows, column-major [lmax][max(R, G)]
+// The neutron one-body entries of the PN part are pre-scaled LDS byte
+// offsets for this G and T: x offset | (Wb offset << 17), so an entry costs
+// one AND, one shift, two LDS reads and G FMAs.
+constexpr int kWbRows = 128;                    // 2 * (max class size + 1) <= 128
+constexpr int kWbBytes = kWbRows * (int)sizeof(float);
+constexpr int kMaxCols = 4096;                  // longest neutron sector (staged row length)
+
 template <typename T, int BLOCK, int CPT, int R, int G>
 __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __restrict__ x,
                                                        T* __restrict__ y, int accumulate) {
-  extern __shared__ __align__(16) unsigned char smem[];
   constexpr int S = R > G ? R : G;
-  T* xs = reinterpret_cast<T*>(smem);
-  float* wb = reinterpret_cast<float*>(smem + (size_t)S * tb.lmax * sizeof(T));
+  constexpr int XS = kWbBytes * G;  // byte offset of the staged rows
+  // static, so that every LDS address is a register offset plus a constant
+  __shared__ __align__(16) unsigned char smem[XS + S * kMaxCols * sizeof(T)];
+  T* xs = reinterpret_cast<T*>(smem + XS);
+  float* wb = reinterpret_cast<float*>(smem);
   const int tid = threadIdx.x;
   const int tile = tb.tile_map[blockIdx.x];
   if (tile < 0) return;  // padding slot of the XCD map (uniform per workgroup)
@@ -131,6 +154,9 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   const int cn = tb.pconj[sp];
   const int L = tb.nlen[cn];
   const int nf0 = tb.nfirst[cn];
+  uint32_t cl[CPT];  // clamped columns: loads never leave the row
+#pragma unroll
+  for (int q = 0; q < CPT; q++) cl[q] = (uint32_t)min(tid + q * BLOCK, L - 1);
   T acc[R][CPT];
 #pragma unroll
   for (int r = 0; r < R; r++)
@@ -145,10 +171,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
         const T ep = (T)tb.pspe_sd[f0 + r];
         const int64_t bf = tb.pstart[f0 + r];
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int c = tid + q * BLOCK;
-          if (c < L) acc[r][q] += (ep + (T)tb.nspe_sd[nf0 + c]) * x[bf + c];
-        }
+        for (int q = 0; q < CPT; q++) acc[r][q] = fma(ep + (T)tb.nspe_sd[nf0 + cl[q]], x[bf + cl[q]], acc[r][q]);
       }
     }
   }
@@ -158,13 +181,10 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     if (r < nr) {
       const int f = f0 + r;
       for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
-        const int64_t bi = tb.pstart[tb.pp_src[e]];
+        const T* xi = x + tb.pstart[tb.pp_src[e]];
         const T v = vload<T>(tb.pp_val32, tb.pp_val64, e);
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int c = tid + q * BLOCK;
-          if (c < L) acc[r][q] += v * x[bi + c];
-        }
+        for (int q = 0; q < CPT; q++) acc[r][q] = fma(v, xi[cl[q]], acc[r][q]);
       }
     }
   }
@@ -178,30 +198,32 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
       for (int c = tid; c < L; c += BLOCK) xs[c * R + r] = r < nr ? x[bf + c] : T(0);
     }
     __syncthreads();
-    const int64_t base = tb.nn_base[cn];
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(tb.nn_src + tb.nn_base[cn]);
+    const __amdgpu_buffer_rsrc_t rv =
+        sizeof(T) == 4 ? buf_rsrc(tb.nn_val32 + tb.nn_base[cn]) : buf_rsrc(tb.nn_val64 + tb.nn_base[cn]);
     const int E = tb.nn_emax[cn];
     // entry-major over the thread's columns: CPT independent loads in flight
     for (int e = 0; e < E; e++) {
+      const uint32_t so = (uint32_t)e * (uint32_t)L * 4u;
       int sl[CPT];
       T v[CPT];
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
-        const int c = tid + q * BLOCK;
-        const int64_t o = base + (int64_t)e * L + c;
-        sl[q] = c < L ? tb.nn_src[o] : 0;
-        v[q] = c < L ? vload<T>(tb.nn_val32, tb.nn_val64, o) : T(0);
+        sl[q] = (int)buf_u32(rs, cl[q] * 4u, so);
+        if constexpr (sizeof(T) == 4) v[q] = __uint_as_float(buf_u32(rv, cl[q] * 4u, so));
+        else v[q] = buf_f64(rv, cl[q] * 8u, so * 2u);
       }
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
         const VecG<T, R> xv = xr[sl[q]];
 #pragma unroll
-        for (int r = 0; r < R; r++) acc[r][q] += v[q] * xv.v[r];
+        for (int r = 0; r < R; r++) acc[r][q] = fma(v[q], xv.v[r], acc[r][q]);
       }
     }
   }
   // ---- PN: per Delta class, batches of G source rows staged transposed
-  VecG<T, G>* xg = reinterpret_cast<VecG<T, G>*>(xs);
-  const VecG<float, G>* wg = reinterpret_cast<const VecG<float, G>*>(wb);
+  const int gsel = G == 1 ? 0 : (G == 2 ? 1 : 2);
+  const uint32_t* pkt = sizeof(T) == 4 ? tb.n1_pkv[gsel][0] : tb.n1_pkv[gsel][1];
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k = 0; k < nd; k++) {
     int p0[R], p1[R];
@@ -221,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     const int Li = tb.nlen[tb.pconj[si]];
     const int wsd = tb.wstride[k];
     const float* Wk = tb.W + tb.woff[k];
-    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
+    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1_base[(int64_t)cn * nd + kn]);
     const int E = tb.n1_emax[(int64_t)cn * nd + kn];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -232,35 +254,41 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
           for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = g < nb ? x[bi + c] : T(0);
         }
-        for (int i = tid; i < (wsd + 1) * G; i += BLOCK) {
-          const int nl = i / G, g = i - nl * G;
+        for (int i = tid; i < 2 * (wsd + 1) * G; i += BLOCK) {
+          const int row = i / G, g = i - row * G, nl = row >> 1;
           float w = 0.0f;
           if (g < nb && nl < wsd) {
             const int meta = tb.p1_meta[b0 + g];
             w = Wk[(meta & 0xff) * wsd + nl];
-            if (meta >> 8) w = -w;
+            if ((meta >> 8) ^ (row & 1)) w = -w;
           }
           wb[i] = w;
         }
         __syncthreads();
-        // entry-major over the thread's columns: CPT independent loads in flight
-        const uint32_t pad = (uint32_t)wsd << 16;  // zero row of Wb
+        // entry-major over the thread's columns, CPT loads in flight, the
+        // next entry's loads issued before this entry's arithmetic
+        const uint32_t step = (uint32_t)L * 4u;
+        uint32_t pk[CPT];
+        if (E > 0) {
+#pragma unroll
+          for (int q = 0; q < CPT; q++) pk[q] = buf_u32(rpk, cl[q] * 4u, 0u);
+        }
         for (int e = 0; e < E; e++) {
-          uint32_t pk[CPT];
+          uint32_t nx[CPT];
+          if (e + 1 < E) {
+            const uint32_t so = (uint32_t)(e + 1) * step;
 #pragma unroll
-          for (int q = 0; q < CPT; q++) {
-            const int c = tid + q * BLOCK;
-            pk[q] = c < L ? tb.n1_pk[ebase + (int64_t)e * L + c] : pad;
+            for (int q = 0; q < CPT; q++) nx[q] = buf_u32(rpk, cl[q] * 4u, so);
           }
 #pragma unroll
           for (int q = 0; q < CPT; q++) {
-            const VecG<T, G> xv = xg[pk[q] & 0xffff];
-            const VecG<float, G> wv = wg[(pk[q] >> 16) & 0xff];
-            T d = T(0);
+            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[q] & 0x1ffffu));
+            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[q] >> 17));
 #pragma unroll
-            for (int g = 0; g < G; g++) d += (T)wv.v[g] * xv.v[g];
-            acc[r][q] += ((pk[q] >> 24) & 1) ? -d : d;
+            for (int g = 0; g < G; g++) acc[r][q] = fma((T)wv.v[g], xv.v[g], acc[r][q]);
           }
+#pragma unroll
+          for (int q = 0; q < CPT; q++) pk[q] = nx[q];
         }
       }
     }
@@ -1034,7 +1062,7 @@ struct bs_system {
   DevBuf<int32_t> psec, pconj, nlen, nfirst, srcsec, pp_src, nn_emax, nn_src, pcls, p1_src, n1_emax;
   DevBuf<int32_t> woff, wrows, wstride, negk;
   DevBuf<int16_t> p1_meta;
-  DevBuf<uint32_t> n1_pk;
+  DevBuf<uint32_t> n1_pk, n1_pkv[3][2];
   DevBuf<float> pspe_sd, nspe_sd, pp_val32, nn_val32, W;
   DevBuf<double> pp_val64, nn_val64;
 };
@@ -1066,7 +1094,7 @@ void upload_tables(bs_system* s) {
     nfirst[c] = (int32_t)N.secfirst[c];
     lmax = std::max(lmax, nlen[c]);
   }
-  if (lmax > 8 * kApplyBlock) throw std::runtime_error("neutron sector longer than 4096 SDs is not supported");
+  if (lmax > kMaxCols) throw std::runtime_error("neutron sector longer than 4096 SDs is not supported");
   if (lmax > 0xffff) throw std::runtime_error("neutron sector too long for packed jumps");
   std::vector<int32_t> srcsec((size_t)P.nsec * nd, -1);
   for (int sp = 0; sp < P.nsec; sp++)
@@ -1296,6 +1324,22 @@ void upload_tables(bs_system* s) {
   s->n1_base.upload(n1_base);
   s->n1_emax.upload(n1_emax);
   s->n1_pk.upload(n1_pk);
+  // the same entries as LDS byte offsets for each staged-row count G and
+  // element type: x offset src*G*sizeof(T) (< 2^17) | Wb offset
+  // (2*nl + sign)*G*4 << 17; padding (nl = class size) lands on a zero row
+  if (2 * (t.max_class + 1) > kWbRows) throw std::runtime_error("operator class too large for the Wb table");
+  for (int gi = 0; gi < 3; gi++)
+    for (int ti = 0; ti < 2; ti++) {
+      const uint32_t G = 1u << gi, ts = ti == 0 ? 4u : 8u;
+      std::vector<uint32_t> v(n1_pk.size());
+      for (size_t i = 0; i < n1_pk.size(); i++) {
+        const uint32_t pk = n1_pk[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
+        const uint32_t xo = sl * G * ts, wo = (2u * nl + sg) * G * 4u;
+        if (xo >= (1u << 17) || wo >= (1u << 15)) throw std::runtime_error("neutron jump offsets do not pack");
+        v[i] = xo | (wo << 17);
+      }
+      s->n1_pkv[gi][ti].upload(v);
+    }
   s->W.upload(t.wpn);
   s->woff.upload(woff);
   s->wrows.upload(wrows);
@@ -1333,6 +1377,8 @@ void upload_tables(bs_system* s) {
   d.n1_base = s->n1_base.p;
   d.n1_emax = s->n1_emax.p;
   d.n1_pk = s->n1_pk.p;
+  for (int gi = 0; gi < 3; gi++)
+    for (int ti = 0; ti < 2; ti++) d.n1_pkv[gi][ti] = s->n1_pkv[gi][ti].p;
   d.W = s->W.p;
   d.woff = s->woff.p;
   d.wrows = s->wrows.p;
@@ -1350,7 +1396,7 @@ constexpr int kVarG[kNumShapes] = {1, 2, 1, 2, 4, 2, 4};
 template <typename T>
 size_t lds_bytes_rg(const bs_system* s, int R, int G) {
   const int S = R > G ? R : G;
-  return (size_t)S * s->dt.lmax * sizeof(T) + (size_t)s->dt.wcols * G * sizeof(float);
+  return (size_t)S * kMaxCols * sizeof(T) + (size_t)kWbBytes * G;
 }
 template <typename T>
 size_t lds_bytes(const bs_system* s, int v) {
@@ -1366,15 +1412,8 @@ void launch_variant(bs_system* s, int map, const T* x, T* y, int accumulate, hip
   d.tile_f0 = s->tile_f0[ri].p;
   d.tile_nr = s->tile_nr[ri].p;
   d.tile_map = s->tile_map[ri][map].p;
-  const size_t lds = lds_bytes_rg<T>(s, R, G);
-  auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kLdsLimit));
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), lds, st, d, x, y, accumulate);
+  auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;  // static LDS (lds_bytes_rg)
+  hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), 0, st, d, x, y, accumulate);
   HIP_OK(hipGetLastError());
 }
 
@@ -2683,6 +2722,18 @@ int bs_multi_variant(bs_system* s, int dtype, int K, int shared, int v) {
   } catch (const std::exception& e) {
     return set_err(e.what());
   }
+}
+
+int bs_multi_info(const bs_system* s, int dtype, int K, int shared, int* chosen, float* ms) {
+  if (!s) return set_err("null system");
+  const int key = (dtype == BS_F64 ? 1000 : 0) + (shared ? 100 : 0) + K;
+  auto it = s->multi_choice.find(key);
+  if (chosen) *chosen = it == s->multi_choice.end() ? -1 : it->second;
+  if (ms) {
+    auto jt = s->multi_ms.find(key);
+    for (int v = 0; v < kNumMulti; v++) ms[v] = jt == s->multi_ms.end() ? 0.0f : jt->second[v];
+  }
+  return 0;
 }
 
 int bs_batch_set_variant(bs_batch* b, int dtype, int v) {

@@ -61,6 +61,7 @@ def load_library():
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
     L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
     L.bs_batch_set_variant.argtypes = [vp, i32, i32]
+    L.bs_multi_info.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), vp]
     L.bs_batch_init.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), i32, i32, i32, i32,
                                 f32, f32, f32, f32, ctypes.POINTER(vp)]
     L.bs_batch_finalize.argtypes = [vp]
@@ -338,6 +339,15 @@ class Batch:
         """Force multi-vector kernel variant v (0..4); False if it does not fit."""
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         return load_library().bs_batch_set_variant(self.h, dt, v) == 0
+
+    def variant_info(self, dtype="f32"):
+        """(chosen multi-vector variant or -1, autotuner ms per variant)."""
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        L = load_library()
+        c = ctypes.c_int()
+        ms = np.zeros(5, np.float32)
+        _check(L.bs_multi_info(L.bs_batch_system(self.h), dt, self.K, 0, ctypes.byref(c), ms.ctypes.data))
+        return c.value, ms
 
     def apply(self, x, y=None):
         """y[:, k] = H_k x[:, k] for a (dim, K) CUDA tensor."""

@@ -172,6 +172,10 @@ int bs_apply_block(bs_system *sys, const void *x, void *y, int K, int dtype);
  * fastest is timed and kept on first use.  Nonzero if it does not fit. */
 int bs_multi_variant(bs_system *sys, int dtype, int K, int shared, int v);
 
+/* The variant in use for (dtype, K, shared) (-1 if none yet) and the
+ * autotuner's timings in ms for variants 0..4 (0 = not timed). */
+int bs_multi_info(const bs_system *sys, int dtype, int K, int shared, int *chosen, float *ms);
+
 /* K Hamiltonians on one basis: one iso .int file each, same orbits, particle
  * numbers and scaling line (src/binput_tbme.f90:898-1250 per file).  Jump
  * tables are built once; values are stored per Hamiltonian.  No counterpart
