@@ -92,6 +92,7 @@ struct DevTables {
   const int32_t* n1_emax;
   const uint32_t* n1_pk;      // src | nl << 16 | sign << 24
   const uint32_t* n1_pkv[3][2];  // per G (1, 2, 4) and T (f32, f64): LDS byte offsets, see k_apply_tiles
+  const uint8_t* n1_chunk;       // [(c*ndelta + k)*64 + j]: entries used by columns 64j..64j+63
   // pn element blocks: class k block is [size_k][size_-k] at woff[k]
   const float* W;
   const int32_t* woff;
@@ -124,11 +125,12 @@ struct alignas(sizeof(T) * G) VecG {
 
 // One workgroup per tile of up to R proton rows of one sector; BLOCK threads,
 // CPT columns per thread (column = tid + q*BLOCK, clamped to the row for
-// loads; only columns < L are stored).  Dynamic LDS, in bytes:
+// loads; only columns < L are stored).  Static LDS, in bytes:
 //   [0, kWbBytes*G)    Wb[2*nl + s][g] = (-1)^s * phase_g * W_k[pp_g][nl],
 //                      the batch's coefficients with the neutron jump's sign
 //                      folded in (rows 2*wsd, 2*wsd + 1: zero, for padding)
-//   [kWbBytes*G, ...)  staged x rows, column-major [lmax][max(R, G)]
+//   [kWbBytes*G, ...)  staged x rows, column-major: [col][R] for the NN part,
+//                      [col][G] for each PN batch
 // The neutron one-body entries of the PN part are pre-scaled LDS byte
 // offsets for this G and T: x offset | (Wb offset << 17), so an entry costs
 // one AND, one shift, two LDS reads and G FMAs.
@@ -144,7 +146,6 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   // static, so that every LDS address is a register offset plus a constant
   __shared__ __align__(16) unsigned char smem[XS + S * kMaxCols * sizeof(T)];
   T* xs = reinterpret_cast<T*>(smem + XS);
-  float* wb = reinterpret_cast<float*>(smem);
   const int tid = threadIdx.x;
   const int tile = tb.tile_map[blockIdx.x];
   if (tile < 0) return;  // padding slot of the XCD map (uniform per workgroup)
@@ -225,6 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   const int gsel = G == 1 ? 0 : (G == 2 ? 1 : 2);
   const uint32_t* pkt = sizeof(T) == 4 ? tb.n1_pkv[gsel][0] : tb.n1_pkv[gsel][1];
   const int nd = tb.ndelta, nd1 = nd + 1;
+  float* wb = reinterpret_cast<float*>(smem);
   for (int k = 0; k < nd; k++) {
     int p0[R], p1[R];
     int any = 0;
@@ -244,7 +246,12 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     const int wsd = tb.wstride[k];
     const float* Wk = tb.W + tb.woff[k];
     const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1_base[(int64_t)cn * nd + kn]);
-    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
+    // per 64-column chunk entry counts: the chunk of column tid + q*BLOCK is
+    // q*(BLOCK/64) + wave, uniform across the wave
+    const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
+    int Eq[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) Eq[q] = ch[q * (BLOCK / 64)];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       for (int b0 = p0[r]; b0 < p1[r]; b0 += G) {
@@ -265,30 +272,39 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           wb[i] = w;
         }
         __syncthreads();
-        // entry-major over the thread's columns, CPT loads in flight, the
-        // next entry's loads issued before this entry's arithmetic
+        // column-major over the thread's columns: each (wave, q) runs to its
+        // own chunk's entry count (wave-uniform), four entries per step with
+        // their loads in flight together
         const uint32_t step = (uint32_t)L * 4u;
-        uint32_t pk[CPT];
-        if (E > 0) {
 #pragma unroll
-          for (int q = 0; q < CPT; q++) pk[q] = buf_u32(rpk, cl[q] * 4u, 0u);
-        }
-        for (int e = 0; e < E; e++) {
-          uint32_t nx[CPT];
-          if (e + 1 < E) {
-            const uint32_t so = (uint32_t)(e + 1) * step;
+        for (int q = 0; q < CPT; q++) {
+          const int Ee = Eq[q];
+          const uint32_t vo = cl[q] * 4u;
+          T a0 = T(0), a1 = T(0);
+          int e = 0;
+          for (; e + 4 <= Ee; e += 4) {
+            uint32_t pk[4];
 #pragma unroll
-            for (int q = 0; q < CPT; q++) nx[q] = buf_u32(rpk, cl[q] * 4u, so);
+            for (int v = 0; v < 4; v++) pk[v] = buf_u32(rpk, vo, (uint32_t)(e + v) * step);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[v] & 0x1ffffu));
+              const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[v] >> 17));
+#pragma unroll
+              for (int g = 0; g < G; g++) {
+                if (v & 1) a1 = fma((T)wv.v[g], xv.v[g], a1);
+                else a0 = fma((T)wv.v[g], xv.v[g], a0);
+              }
+            }
           }
+          for (; e < Ee; e++) {
+            const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
+            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
+            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
 #pragma unroll
-          for (int q = 0; q < CPT; q++) {
-            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[q] & 0x1ffffu));
-            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[q] >> 17));
-#pragma unroll
-            for (int g = 0; g < G; g++) acc[r][q] = fma((T)wv.v[g], xv.v[g], acc[r][q]);
+            for (int g = 0; g < G; g++) a0 = fma((T)wv.v[g], xv.v[g], a0);
           }
-#pragma unroll
-          for (int q = 0; q < CPT; q++) pk[q] = nx[q];
+          acc[r][q] += a0 + a1;
         }
       }
     }
@@ -310,25 +326,53 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
 }
 
 // ----------------------------------------------------------- vector kernels
+// 16-byte vector of T for streaming passes
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
+__device__ __forceinline__ double dot16(float4 a, float4 b) {
+  return (double)a.x * (double)b.x + (double)a.y * (double)b.y + (double)a.z * (double)b.z + (double)a.w * (double)b.w;
+}
+__device__ __forceinline__ double dot16(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+
 // Partial sums of <V_j, w> for j in [0, m): block b writes part[b*m + j].
+// 16-byte loads over the first D - D % n elements of each vector (V_j starts
+// 16-byte aligned: ld is a multiple of 64 elements), the tail by block 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_multidot(const T* __restrict__ V, int64_t ld, int m,
                                                      const T* __restrict__ w, int64_t D,
                                                      double* __restrict__ part) {
+  using V16 = typename Vec16<T>::type;
+  constexpr int VW = Vec16<T>::n;
   __shared__ double red[8][kBlock / 64];
-  const int64_t chunk = (D + gridDim.x - 1) / gridDim.x;
+  // 16-byte path only when every vector starts 16-byte aligned (uniform test)
+  const bool vec = ((((uintptr_t)V | (uintptr_t)w) & 15) == 0) && (ld % VW == 0);
+  const int64_t nv = vec ? D / VW : 0;
+  const int64_t chunk = (nv + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * chunk;
-  const int64_t hi = lo + chunk < D ? lo + chunk : D;
+  const int64_t hi = lo + chunk < nv ? lo + chunk : nv;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const V16* w16 = reinterpret_cast<const V16*>(w);
   for (int j0 = 0; j0 < m; j0 += 8) {
     double a[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) a[u] = 0.0;
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-      const double wi = (double)w[i];
+      const V16 wi = w16[i];
 #pragma unroll
       for (int u = 0; u < 8; u++)
-        if (j0 + u < m) a[u] += (double)V[(int64_t)(j0 + u) * ld + i] * wi;
+        if (j0 + u < m) a[u] += dot16(reinterpret_cast<const V16*>(V + (int64_t)(j0 + u) * ld)[i], wi);
+    }
+    // scalar part: the tail, or everything when not aligned (block-strided)
+    {
+      const int64_t s0 = nv * VW;
+      const int64_t sc = (D - s0 + gridDim.x - 1) / gridDim.x;
+      const int64_t slo = s0 + (int64_t)blockIdx.x * sc, shi = min(slo + sc, D);
+      for (int64_t i = slo + threadIdx.x; i < shi; i += kBlock) {
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (j0 + u < m) a[u] += (double)V[(int64_t)(j0 + u) * ld + i] * (double)w[i];
+      }
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -372,11 +416,34 @@ __global__ void k_sub_scaled(T* __restrict__ w, const T* __restrict__ v, const d
     w[i] = w[i] - (T)(s * (double)v[i]);
 }
 
-// w -= T(sum_j d_j V_j)   (src/breorthog.f90:880-889)
+// w -= T(sum_j d_j V_j)   (src/breorthog.f90:880-889); 16-byte loads, V_j
+// 16-byte aligned (ld a multiple of 64 elements), scalar tail
 template <typename T>
 __global__ void k_multiaxpy(T* __restrict__ w, const T* __restrict__ V, int64_t ld, int m,
                             const double* __restrict__ d, int64_t D) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+  using V16 = typename Vec16<T>::type;
+  constexpr int VW = Vec16<T>::n;
+  const bool vec = ((((uintptr_t)V | (uintptr_t)w) & 15) == 0) && (ld % VW == 0);
+  const int64_t nv = vec ? D / VW : 0;
+  V16* w16 = reinterpret_cast<V16*>(w);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    double s[VW];
+#pragma unroll
+    for (int e = 0; e < VW; e++) s[e] = 0.0;
+    for (int j = 0; j < m; j++) {
+      const V16 v = reinterpret_cast<const V16*>(V + (int64_t)j * ld)[i];
+      const T* pv = reinterpret_cast<const T*>(&v);
+#pragma unroll
+      for (int e = 0; e < VW; e++) s[e] += d[j] * (double)pv[e];
+    }
+    V16 o = w16[i];
+    T* po = reinterpret_cast<T*>(&o);
+#pragma unroll
+    for (int e = 0; e < VW; e++) po[e] = po[e] - (T)s[e];
+    w16[i] = o;
+  }
+  for (int64_t i = nv * VW + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D;
+       i += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
     for (int j = 0; j < m; j++) s += d[j] * (double)V[(int64_t)j * ld + i];
     w[i] = w[i] - (T)s;
@@ -1063,6 +1130,7 @@ struct bs_system {
   DevBuf<int32_t> woff, wrows, wstride, negk;
   DevBuf<int16_t> p1_meta;
   DevBuf<uint32_t> n1_pk, n1_pkv[3][2];
+  DevBuf<uint8_t> n1_chunk;
   DevBuf<float> pspe_sd, nspe_sd, pp_val32, nn_val32, W;
   DevBuf<double> pp_val64, nn_val64;
 };
@@ -1193,6 +1261,7 @@ void upload_tables(bs_system* s) {
       tot += (int64_t)em * N.seccount[c];
     }
   std::vector<uint32_t> n1_pk(std::max<int64_t>(tot, 1), 0);
+  std::vector<uint8_t> n1_chunk((size_t)N.nsec * nd * 64, 0);
   for (int c = 0; c < N.nsec; c++) {
     const int L = nlen[c];
     for (int k = 0; k < nd; k++) {
@@ -1215,6 +1284,9 @@ void upload_tables(bs_system* s) {
           n1_pk[b + (int64_t)e * L + col] = (uint32_t)sl | ((uint32_t)nl << 16) | (O1.phase[q] < 0 ? (1u << 24) : 0u);
           e++;
         }
+        uint8_t& cm = n1_chunk[((size_t)c * nd + k) * 64 + col / 64];
+        if (e > 255) throw std::runtime_error("too many neutron jumps into one column");
+        cm = (uint8_t)std::max<int>(cm, e);
         for (; e < em; e++) n1_pk[b + (int64_t)e * L + col] = pad;
       }
     }
@@ -1324,6 +1396,7 @@ void upload_tables(bs_system* s) {
   s->n1_base.upload(n1_base);
   s->n1_emax.upload(n1_emax);
   s->n1_pk.upload(n1_pk);
+  s->n1_chunk.upload(n1_chunk);
   // the same entries as LDS byte offsets for each staged-row count G and
   // element type: x offset src*G*sizeof(T) (< 2^17) | Wb offset
   // (2*nl + sign)*G*4 << 17; padding (nl = class size) lands on a zero row
@@ -1377,6 +1450,7 @@ void upload_tables(bs_system* s) {
   d.n1_base = s->n1_base.p;
   d.n1_emax = s->n1_emax.p;
   d.n1_pk = s->n1_pk.p;
+  d.n1_chunk = s->n1_chunk.p;
   for (int gi = 0; gi < 3; gi++)
     for (int ti = 0; ti < 2; ti++) d.n1_pkv[gi][ti] = s->n1_pkv[gi][ti].p;
   d.W = s->W.p;
@@ -1399,9 +1473,7 @@ size_t lds_bytes_rg(const bs_system* s, int R, int G) {
   return (size_t)S * kMaxCols * sizeof(T) + (size_t)kWbBytes * G;
 }
 template <typename T>
-size_t lds_bytes(const bs_system* s, int v) {
-  return lds_bytes_rg<T>(s, kVarR[v % kNumShapes], kVarG[v % kNumShapes]);
-}
+size_t lds_bytes(const bs_system* s, int v);
 
 constexpr size_t kLdsLimit = 160 * 1024;
 
@@ -1415,6 +1487,11 @@ void launch_variant(bs_system* s, int map, const T* x, T* y, int accumulate, hip
   auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;  // static LDS (lds_bytes_rg)
   hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), 0, st, d, x, y, accumulate);
   HIP_OK(hipGetLastError());
+}
+
+template <typename T>
+size_t lds_bytes(const bs_system* s, int v) {
+  return lds_bytes_rg<T>(s, kVarR[v % kNumShapes], kVarG[v % kNumShapes]);
 }
 
 template <typename T, int CPT>
@@ -1504,9 +1581,12 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter))) return set_err("thick must be 0 or in [nkeep, niter)");
   const int mtot = niter + 1;
   const int nb = grid_for(D) < 1024 ? grid_for(D) : 1024;
+  // Lanczos vectors at a stride of whole 64-element blocks: every v_i starts
+  // 256-byte aligned, so the streaming passes use 16-byte loads
+  const int64_t ld = (D + 63) / 64 * 64;
   DevBuf<T> V, w;
   DevBuf<double> part, dots, ab;
-  V.alloc((size_t)mtot * D);
+  V.alloc((size_t)mtot * ld);
   w.alloc(D);
   part.alloc((size_t)nb * mtot);
   dots.alloc(mtot);
@@ -1526,7 +1606,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   std::vector<double> e, Y;
   while (true) {
     for (int i = kstart; i < niter; i++) {
-      T* vi = V.p + (int64_t)i * D;
+      T* vi = V.p + (int64_t)i * ld;
       launch_apply<T>(s, vi, w.p, 0, st);
       total_mv++;
       // alpha = <v_i, w>; w -= alpha v_i
@@ -1535,13 +1615,13 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       k_sub_scaled<T><<<g, kBlock, 0, st>>>(w.p, vi, d_alpha + i, D);
       // classical Gram-Schmidt against v_0 .. v_{i-1}
       if (i > 0) {
-        k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, D, i, w.p, D, part.p);
+        k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, i, w.p, D, part.p);
         k_reduce<<<i, kBlock, 0, st>>>(part.p, nb, i, dots.p, 0);
-        k_multiaxpy<T><<<g, kBlock, 0, st>>>(w.p, V.p, D, i, dots.p, D);
+        k_multiaxpy<T><<<g, kBlock, 0, st>>>(w.p, V.p, ld, i, dots.p, D);
       }
       k_multidot<T><<<nb, kBlock, 0, st>>>(w.p, D, 1, w.p, D, part.p);
       k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
-      k_normalize<T><<<g, kBlock, 0, st>>>(V.p + (int64_t)(i + 1) * D, w.p, d_beta + i, D);
+      k_normalize<T><<<g, kBlock, 0, st>>>(V.p + (int64_t)(i + 1) * ld, w.p, d_beta + i, D);
       HIP_OK(hipGetLastError());
     }
     alpha_h.assign(mtot, 0.0);
@@ -1581,14 +1661,14 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
         for (int k = 0; k < thick; k++) Yk[(size_t)j * thick + k] = Y[(size_t)j * m + k];
       dY.upload(Yk);
       DevBuf<T> tmp;
-      tmp.alloc((size_t)thick * D);
+      tmp.alloc((size_t)thick * ld);
       for (int k0 = 0; k0 < thick; k0 += 16) {
         int nk = std::min(16, thick - k0);
-        k_combine<T, T><<<g, kBlock, 0, st>>>(tmp.p + (int64_t)k0 * D, D, V.p, D, m, dY.p + k0, thick, nk, D);
+        k_combine<T, T><<<g, kBlock, 0, st>>>(tmp.p + (int64_t)k0 * ld, ld, V.p, ld, m, dY.p + k0, thick, nk, D);
       }
-      HIP_OK(hipMemcpyAsync(V.p, tmp.p, sizeof(T) * (size_t)thick * D, hipMemcpyDeviceToDevice, st));
+      HIP_OK(hipMemcpyAsync(V.p, tmp.p, sizeof(T) * (size_t)thick * ld, hipMemcpyDeviceToDevice, st));
       // the last Krylov vector becomes v_thick
-      HIP_OK(hipMemcpyAsync(V.p + (int64_t)thick * D, V.p + (int64_t)m * D, sizeof(T) * D,
+      HIP_OK(hipMemcpyAsync(V.p + (int64_t)thick * ld, V.p + (int64_t)m * ld, sizeof(T) * D,
                             hipMemcpyDeviceToDevice, st));
       arrow.assign(thick, 0.0);
       std::vector<double> anew(thick);
@@ -1615,7 +1695,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     dY.upload(Yk);
     X64.alloc((size_t)nkeep * D);
     HX.alloc(D);
-    k_combine<T, double><<<g, kBlock, 0, st>>>(X64.p, D, V.p, D, m, dY.p, nkeep, nkeep, D);
+    k_combine<T, double><<<g, kBlock, 0, st>>>(X64.p, D, V.p, ld, m, dY.p, nkeep, nkeep, D);
     HIP_OK(hipGetLastError());
     for (int k = 0; k < nkeep; k++) {
       double* xk = X64.p + (int64_t)k * D;
