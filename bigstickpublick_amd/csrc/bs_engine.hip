@@ -2911,6 +2911,47 @@ int bs_batch_gradient(bs_batch* b, const void* psi, int dtype, double* grad) {
   }
 }
 
+int bs_write_wfn(const bs_system* s, const char* path, int nkeep, const float* energy, const float* xj,
+                 const float* xt2, const void* vecs, int dtype, int on_device) {
+  try {
+    if (!s || !path || !vecs || (nkeep > 0 && (!energy || !xj || !xt2))) return set_err("null argument");
+    if (nkeep < 0) return set_err("nkeep must be >= 0");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    const int64_t D = s->t.dim;
+    const std::string h = bs::wfn_header(s->t);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_err(std::string("cannot open ") + path);
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    const int32_t nk = nkeep;
+    ok = ok && std::fwrite(&nk, 4, 1, f) == 1;
+    std::vector<float> v32(D);
+    std::vector<double> v64(dtype == BS_F64 ? D : 0);
+    for (int k = 0; k < nkeep && ok; k++) {
+      // (src/bwfnlib.f90:124-125) magic, 1-based index, E, J, T(T+1), then
+      // the vector in single precision
+      const int32_t head[2] = {27182818, k + 1};
+      const float vals[3] = {energy[k], xj[k], xt2[k]};
+      ok = std::fwrite(head, 4, 2, f) == 2 && std::fwrite(vals, 4, 3, f) == 3;
+      const size_t es = dtype == BS_F32 ? 4 : 8;
+      const char* src = static_cast<const char*>(vecs) + (size_t)k * D * es;
+      if (dtype == BS_F32) {
+        if (on_device) HIP_OK(hipMemcpy(v32.data(), src, 4 * (size_t)D, hipMemcpyDeviceToHost));
+        else std::memcpy(v32.data(), src, 4 * (size_t)D);
+      } else {
+        if (on_device) HIP_OK(hipMemcpy(v64.data(), src, 8 * (size_t)D, hipMemcpyDeviceToHost));
+        else std::memcpy(v64.data(), src, 8 * (size_t)D);
+        for (int64_t i = 0; i < D; i++) v32[i] = (float)v64[i];
+      }
+      ok = ok && std::fwrite(v32.data(), 4, (size_t)D, f) == (size_t)D;
+    }
+    if (!ok) return set_err(std::string("write failed: ") + path);
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
 int bs_autotune_times(const bs_system* s, int dtype, float* ms) {
   if (!s || !ms || (dtype != BS_F32 && dtype != BS_F64)) return set_err("bad argument");
   for (int v = 0; v < kNumVariants; v++) ms[v] = s->tune_ms[dtype == BS_F32 ? 0 : 1][v];

@@ -332,6 +332,8 @@ struct Builder {
         if (std::abs(m[j]) < std::abs(mm)) { k = j; mm = m[j]; so = o[j]; sm = m[j]; }
       if (k != is) { o[k] = o[is]; m[k] = m[is]; o[is] = so; m[is] = sm; }
     }
+    t.ssorb = o;
+    t.ssm2 = m;
     std::vector<int> lo, lm, ro, rm;
     for (int i = 0; i < n; i++) {
       if (m[i] >= 0) { ro.push_back(o[i]); rm.push_back(m[i]); }
@@ -1061,6 +1063,65 @@ void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out) {
       out.rowptr[f + 1] = (int64_t)out.src.size();
     }
   }
+}
+
+std::string wfn_header(const Tables& t) {
+  // Layout (little-endian stream, no record markers), as written by
+  // write_wfn_header: magic, version, 0, offset of the first vector, 4 zeros;
+  // Z, N; isoflag, numorb(1), numorb(2); per species and orbit
+  // (nr, 2j, l, par, w); nsps(1), nsps(2); per species and s.p. state in the
+  // |m|-sorted order (nr, 2j, 2m, l, w, par, orbit, group); 2Jz, parity
+  // character, maxWtot, allsameparity, allsameW, spinless; dimension (int64).
+  // Single-parity, single-W spaces (the only ones this engine builds): parity
+  // flags are written as +1 and '+', W as 0, maxWtot 0.
+  std::string h;
+  auto i4 = [&](int32_t v) { h.append(reinterpret_cast<const char*>(&v), 4); };
+  const std::vector<Orbit>& orb = t.in.orbits;
+  const int norb = (int)orb.size();
+  i4(31415926);
+  i4(1);
+  i4(0);
+  i4(0);  // vector offset, patched below
+  for (int i = 0; i < 4; i++) i4(0);
+  i4(t.in.Z);
+  i4(t.in.N);
+  i4(1);
+  i4(norb);
+  i4(norb);
+  for (int it = 0; it < 2; it++)
+    for (const Orbit& o : orb) {
+      i4(o.n);
+      i4(o.j2);
+      i4(o.l);
+      i4(1);
+      i4(0);
+    }
+  const int ns = (int)t.ssorb.size();
+  i4(ns);
+  i4(ns);
+  for (int it = 0; it < 2; it++)
+    for (int i = 0; i < ns; i++) {
+      const Orbit& o = orb[t.ssorb[i]];
+      i4(o.n);
+      i4(o.j2);
+      i4(t.ssm2[i]);
+      i4(o.l);
+      i4(0);
+      i4(1);
+      i4(t.ssorb[i] + 1);
+      i4(0);
+    }
+  i4(t.in.jz2);
+  i4('+');
+  i4(0);
+  i4(1);
+  i4(1);
+  i4(0);
+  const int64_t d = t.dim;
+  h.append(reinterpret_cast<const char*>(&d), 8);
+  const int32_t off = (int32_t)h.size() + 4;
+  std::memcpy(&h[12], &off, 4);
+  return h;
 }
 
 }  // namespace bs

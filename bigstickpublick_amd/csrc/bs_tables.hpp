@@ -87,6 +87,7 @@ struct Tables {
   // single-particle states, combined label order (left m<0 first, then m>=0)
   int nsps = 0, nL = 0, nR = 0;
   std::vector<int> sorb, sj2, sm2;
+  std::vector<int> ssorb, ssm2;        // the same states before the left/right split (sorted on |m|)
   Species sp[2];
   int64_t dim = 0;
   bool call_spe = false;
@@ -152,6 +153,10 @@ Tables build_tables_text(const std::string& sps, const std::string& inttext, int
 // Iso interaction text of J^2 (which = 0) or T^2 (which = 1) on these orbits.
 std::string observable_int_text(const std::vector<Orbit>& orbits, int which);
 void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out);
+// Header of the reference's binary .wfn file for this basis, up to and
+// including the basis dimension (src/bwfnlib.f90:373-541); the vectors start
+// 4 bytes after it (the count of vectors sits in between).
+std::string wfn_header(const Tables& t);
 float cleb(int j1, int m1, int j2, int m2, int j3, int m3);
 
 }  // namespace bs

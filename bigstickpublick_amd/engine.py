@@ -59,6 +59,7 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
+    L.bs_write_wfn.argtypes = [vp, ctypes.c_char_p, i32, vp, vp, vp, vp, i32, i32]
     L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
     L.bs_batch_set_variant.argtypes = [vp, i32, i32]
     L.bs_multi_info.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), vp]
@@ -280,6 +281,26 @@ class System:
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         return load_library().bs_multi_variant(self.h, dt, K, 1, v) == 0
+
+    def write_wfn(self, path, energy, xj, xt2, vecs):
+        """Write the reference's binary .wfn: energies, J, T(T+1) (length
+        nkeep) and vecs, an (nkeep, dim) numpy array or CUDA tensor."""
+        e = np.ascontiguousarray(energy, dtype=np.float32)
+        j = np.ascontiguousarray(xj, dtype=np.float32)
+        t = np.ascontiguousarray(xt2, dtype=np.float32)
+        nk = len(e)
+        if isinstance(vecs, np.ndarray):
+            v = np.ascontiguousarray(vecs)
+            if v.dtype not in (np.float32, np.float64) or v.size != nk * self.dim:
+                raise ValueError("vecs must be float32/float64 of shape (nkeep, dim)")
+            ptr, dt, dev = v.ctypes.data, (BS_F32 if v.dtype == np.float32 else BS_F64), 0
+        else:
+            v = vecs.contiguous()
+            if v.numel() != nk * self.dim:
+                raise ValueError("vecs must have shape (nkeep, dim)")
+            ptr, dt, dev = v.data_ptr(), (BS_F32 if v.dtype.itemsize == 4 else BS_F64), 1
+        _check(load_library().bs_write_wfn(self.h, str(path).encode(), nk, e.ctypes.data, j.ctypes.data,
+                                           t.ctypes.data, ctypes.c_void_p(ptr), dt, dev))
 
     def apply_block(self, x, y=None):
         """y[i, k] = (H x)[i, k] for a (dim, K) CUDA tensor of K vectors, K <= 64."""

@@ -111,6 +111,15 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_observable
 
+    function bs_write_wfn(sys, path, nkeep, energy, xj, xt2, vecs, dtype, on_device) &
+        bind(C, name='bs_write_wfn') result(rc)
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: sys, energy, xj, xt2, vecs
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: nkeep, dtype, on_device
+      integer(c_int) :: rc
+    end function bs_write_wfn
+
     ! HIP runtime, for device buffers owned by the Fortran host
     function hipMalloc(ptr, bytes) bind(C, name='hipMalloc') result(rc)
       import :: c_ptr, c_size_t, c_int

@@ -23,14 +23,18 @@ program bs_driver
   real(c_double), allocatable, target :: alpha(:), beta(:), evals(:), resid(:)
   real(c_double) :: j2, t2
   real :: xj, xt
+  real(c_float), allocatable, target :: we(:), wj(:), wt2(:)
+  logical :: write_wfn
   character(len=16) :: env
 
   read(5, '(a)') opt
   opt = adjustl(opt)
   if (opt(1:1) /= 'n' .and. opt(1:1) /= 'N') then
-    write(6, *) ' bs_driver: only the normal diagonalization run (menu n) is supported'
+    write(6, *) ' bs_driver: only the normal diagonalization run (menu n / ns) is supported'
     stop 1
   end if
+  ! n writes <name>.wfn, ns suppresses it (src/bmenu.f90:680-687)
+  write_wfn = .not. (opt(2:2) == 's' .or. opt(2:2) == 'S')
   read(5, '(a)') outname
   read(5, '(a)') spsname
   read(5, *) z, n
@@ -103,6 +107,7 @@ program bs_driver
   write(iu, '(1x,i0,a)') iters, '  iterations '
   write(iu, '(a)') '  '
   write(iu, '(a)') '  State      E        Ex         J       T      par'
+  allocate(we(nkeep), wj(nkeep), wt2(nkeep))
   do k = 1, nkeep
     rc = bs_observable(sys, 0, ptr_offset(dvec, int(k - 1, c_int64_t) * dim * esize), dtype, j2)
     call bs_check(rc, 'bs_observable J^2')
@@ -111,9 +116,18 @@ program bs_driver
     xj = real(-0.5d0 + sqrt(0.25d0 + max(j2, 0.0d0)))
     xt = real(-0.5d0 + sqrt(0.25d0 + max(t2, 0.0d0)))
     write(iu, 100) k, evals(k), evals(k) - evals(1), xj, xt, 1
+    ! .wfn carries E, J and T(T+1) in single precision (src/blanczoslib.f90:1330-1350)
+    we(k) = real(evals(k), c_float)
+    wj(k) = xj
+    wt2(k) = real(t2, c_float)
   end do
 100 format(i5,3x,2f10.5,2x,2f8.3,3x,i2)
   close(iu)
+  if (write_wfn) then
+    rc = bs_write_wfn(sys, cstr(trim(first_word(outname)) // '.wfn'), nkeep, c_loc(we), c_loc(wj), c_loc(wt2), &
+                      dvec, dtype, 1_c_int)
+    call bs_check(rc, 'bs_write_wfn')
+  end if
   open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
   do k = 1, niter
     write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)

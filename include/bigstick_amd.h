@@ -156,6 +156,15 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Write the reference's binary .wfn file (src/bwfnlib.f90:77-134 vectors,
+ * :373-541 header; stream access, little-endian): the basis header, nkeep,
+ * then per state magic, index, energy, J, T(T+1) and the vector in single
+ * precision.  vecs holds nkeep vectors of dim elements in dtype, on the
+ * device (on_device = 1) or the host (0).  Replaces wfn_writeeigenvec +
+ * write_wfn_header for the single-file, one-fragment case. */
+int bs_write_wfn(const bs_system *sys, const char *path, int nkeep, const float *energy, const float *xj,
+                 const float *xt2, const void *vecs, int dtype, int on_device);
+
 /* ---------------------------------------------------------------- block and batched
  * Interleaved multi-vector layout: element k of basis state i at [i*K + k],
  * K <= 64. */

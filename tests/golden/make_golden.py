@@ -209,6 +209,29 @@ def config(cfg, with_basis=True, niter=None):
         shutil.rmtree(wd)
 
 
+WFN = [("sd2p2n", "sd", 2, 2, 0, 77), ("pf2p1n_m3", "pf", 2, 1, 3, 83)]
+
+
+def wfn(name, space, Z, N, jz2, seed):
+    """The reference's .wfn (menu n, 150 fixed Lanczos steps or D - 1) and
+    .res for a small system, kept byte for byte as <name>.wfn / <name>_wfn.res."""
+    case = small_case(space, Z, N, jz2, seed=seed)
+    case.name = name
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, menu_script(case, "wh"), env_extra={"OMP_NUM_THREADS": "1"})
+        D, _ = read_hamdat(os.path.join(wd, "ham.dat"))
+        case.niter = min(150, D - 1)
+        case.nkeep = min(5, case.niter)
+        run_ref(wd, menu_script(case, "n", "lf"))
+        shutil.copy(os.path.join(wd, name + ".wfn"), os.path.join(HERE, name + ".wfn"))
+        shutil.copy(os.path.join(wd, name + ".res"), os.path.join(HERE, name + "_wfn.res"))
+        print(name, "wfn", os.path.getsize(os.path.join(HERE, name + ".wfn")), "bytes, D=%d" % D)
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -239,6 +262,9 @@ if __name__ == "__main__":
     for cfg in (0, 1):
         if "config%d" % cfg in want:
             config(cfg, with_basis=True)
+    for c in WFN:
+        if "wfn_" + c[0] in want:
+            wfn(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

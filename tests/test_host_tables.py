@@ -78,3 +78,27 @@ def test_bad_inputs_raise(casedir):
     open(bad, "w").write("\n".join(lines) + "\n")
     with pytest.raises(RuntimeError):
         System(sps, bad, 2, 2, 0)
+
+
+@pytest.mark.parametrize("name,Z,N,jz2,seed", [("sd2p2n", 2, 2, 0, 77), ("pf2p1n_m3", 2, 1, 3, 83)])
+def test_wfn_writer_reproduces_reference_file(name, Z, N, jz2, seed, casedir, tmp_path):
+    """bs_write_wfn, given the reference's own energies, J, T(T+1) and vectors,
+    writes the reference's .wfn byte for byte (header from this engine's
+    tables: orbits, |m|-sorted s.p. states, 2Jz, dimension)."""
+    from bigstickpublick_amd import System
+    from bigstickpublick_amd.inputs import small_case, write_case
+    from wfn_io import read_wfn
+    ref = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".wfn")
+    r = read_wfn(ref)
+    assert r["end"] == r["size"] and all(s["magic"] == 27182818 for s in r["states"])
+    case = small_case("sd" if name.startswith("sd") else "pf", Z, N, jz2, seed=seed)
+    case.name = name
+    d = os.path.join(casedir, "wfn_" + name)
+    write_case(case, d)
+    s = System(os.path.join(d, case.space + ".sps"), os.path.join(d, name + ".int"), Z, N, jz2)
+    assert s.dim == r["dim"]
+    out = str(tmp_path / (name + ".wfn"))
+    st = r["states"]
+    s.write_wfn(out, [x["e"] for x in st], [x["xj"] for x in st], [x["xt2"] for x in st],
+                np.stack([x["vec"] for x in st]))
+    assert open(out, "rb").read() == open(ref, "rb").read()
