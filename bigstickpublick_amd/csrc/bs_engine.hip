@@ -480,6 +480,42 @@ __global__ void k_combine(U* __restrict__ out, int64_t ldo, const T* __restrict_
   }
 }
 
+// X[:, 0:ncol] = V[:, 0:m] Y[0:m, 0:ncol] (thick-restart rotation and Ritz
+// vectors, src/breorthog.f90:1211-1243) on the FP64 matrix cores: one wave per
+// strip of 16 rows, v_mfma_f64_16x16x4f64 over m in steps of 4, V converted to
+// FP64 as it is loaded, ncol <= 16.  Operand layouts of the 16x16x4 FP64
+// instruction: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and D[i][j]
+// in lane j + 16*(i % 4), register i / 4.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+template <typename T, typename U>
+__global__ __launch_bounds__(256) void k_combine_mfma(U* __restrict__ X, int64_t ldo, const T* __restrict__ V,
+                                                      int64_t ld, int m, const double* __restrict__ Y, int ys,
+                                                      int ncol, int64_t D) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t nstrip = (D + 15) / 16;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwave = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t st = wave; st < nstrip; st += nwave) {
+    const int64_t i0 = st * 16;
+    const int64_t row = i0 + li;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < m; k0 += 4) {
+      const int k = k0 + lk;
+      const double a = (k < m && row < D) ? (double)V[(int64_t)k * ld + row] : 0.0;
+      const double b = (k < m && li < ncol) ? Y[(int64_t)k * ys + li] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (li < ncol) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t i = i0 + lk + 4 * r;
+        if (i < D) X[(int64_t)li * ldo + i] = (U)acc[r];
+      }
+    }
+  }
+}
+
 template <typename T, typename U>
 __global__ void k_convert(U* __restrict__ out, const T* __restrict__ in, int64_t D) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x)
@@ -1092,6 +1128,12 @@ void jacobi_eigen(int m, std::vector<double> A, std::vector<double>& e, std::vec
   Y.swap(Ys);
 }
 
+// one wave per 16-row strip, at most 2048 workgroups of 4 waves
+int mfma_grid(int64_t D) {
+  const int64_t waves = (D + 15) / 16;
+  return (int)std::min<int64_t>(std::max<int64_t>((waves + 3) / 4, 1), 2048);
+}
+
 int grid_for(int64_t D) {
   int64_t g = (D + kBlock - 1) / kBlock;
   return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 4096);
@@ -1664,7 +1706,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       tmp.alloc((size_t)thick * ld);
       for (int k0 = 0; k0 < thick; k0 += 16) {
         int nk = std::min(16, thick - k0);
-        k_combine<T, T><<<g, kBlock, 0, st>>>(tmp.p + (int64_t)k0 * ld, ld, V.p, ld, m, dY.p + k0, thick, nk, D);
+        k_combine_mfma<T, T><<<mfma_grid(D), 256, 0, st>>>(tmp.p + (int64_t)k0 * ld, ld, V.p, ld, m, dY.p + k0,
+                                                            thick, nk, D);
       }
       HIP_OK(hipMemcpyAsync(V.p, tmp.p, sizeof(T) * (size_t)thick * ld, hipMemcpyDeviceToDevice, st));
       // the last Krylov vector becomes v_thick
@@ -1695,7 +1738,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     dY.upload(Yk);
     X64.alloc((size_t)nkeep * D);
     HX.alloc(D);
-    k_combine<T, double><<<g, kBlock, 0, st>>>(X64.p, D, V.p, ld, m, dY.p, nkeep, nkeep, D);
+    k_combine_mfma<T, double><<<mfma_grid(D), 256, 0, st>>>(X64.p, D, V.p, ld, m, dY.p, nkeep, nkeep, D);
     HIP_OK(hipGetLastError());
     for (int k = 0; k < nkeep; k++) {
       double* xk = X64.p + (int64_t)k * D;
