@@ -117,6 +117,11 @@ __device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t vof
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// 16-byte vector of T for streaming passes
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
+
 // G values of one LDS column, read with one wide LDS load.
 template <typename T, int G>
 struct alignas(sizeof(T) * G) VecG {
@@ -258,8 +263,29 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
         const int nb = min(G, p1[r] - b0);
         __syncthreads();
         for (int g = 0; g < G; g++) {
-          const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
-          for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = g < nb ? x[bi + c] : T(0);
+          if (g < nb) {
+            // 16-byte loads from the first 16-byte aligned column on
+            const int64_t bi = tb.pstart[tb.p1_src[b0 + g]];
+            constexpr int VW = 16 / (int)sizeof(T);
+            // columns before the first 16-byte boundary (from the address,
+            // so any element-aligned x works), at most the row length
+            const int head = min(Li, (int)(((16u - ((uint32_t)(uintptr_t)(x + bi) & 15u)) & 15u) / sizeof(T)));
+            const int nvec = (Li - head) / VW;
+            using V16 = typename Vec16<T>::type;
+            const V16* xv = reinterpret_cast<const V16*>(x + bi + head);
+            for (int v = tid; v < nvec; v += BLOCK) {
+              const V16 q = xv[v];
+              const T* pq = reinterpret_cast<const T*>(&q);
+              const int c = head + v * VW;
+#pragma unroll
+              for (int u = 0; u < VW; u++) xs[(c + u) * G + g] = pq[u];
+            }
+            const int tail0 = head + nvec * VW;
+            if (tid < head) xs[tid * G + g] = x[bi + tid];
+            for (int c = tail0 + tid; c < Li; c += BLOCK) xs[c * G + g] = x[bi + c];
+          } else {
+            for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = T(0);
+          }
         }
         for (int i = tid; i < 2 * (wsd + 1) * G; i += BLOCK) {
           const int row = i / G, g = i - row * G, nl = row >> 1;
@@ -326,10 +352,6 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
 }
 
 // ----------------------------------------------------------- vector kernels
-// 16-byte vector of T for streaming passes
-template <typename T> struct Vec16;
-template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
-template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
 __device__ __forceinline__ double dot16(float4 a, float4 b) {
   return (double)a.x * (double)b.x + (double)a.y * (double)b.y + (double)a.z * (double)b.z + (double)a.w * (double)b.w;
 }
@@ -500,11 +522,17 @@ __global__ __launch_bounds__(256) void k_combine_mfma(U* __restrict__ X, int64_t
     const int64_t i0 = st * 16;
     const int64_t row = i0 + li;
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < m; k0 += 4) {
-      const int k = k0 + lk;
-      const double a = (k < m && row < D) ? (double)V[(int64_t)k * ld + row] : 0.0;
-      const double b = (k < m && li < ncol) ? Y[(int64_t)k * ys + li] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    // four k-steps per pass: their loads are in flight together
+    for (int k0 = 0; k0 < m; k0 += 16) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = k0 + 4 * u + lk;
+        a[u] = (k < m && row < D) ? (double)V[(int64_t)k * ld + row] : 0.0;
+        b[u] = (k < m && li < ncol) ? Y[(int64_t)k * ys + li] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
     if (li < ncol) {
 #pragma unroll
