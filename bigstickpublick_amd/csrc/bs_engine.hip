@@ -1381,36 +1381,15 @@ void upload_tables(bs_system* s) {
     }
   for (int f = 0; f < nP; f++)
     if (P.start[f] < 0 || P.start[f] + nlen[P.conj[P.sdsec[f]]] > t.dim) throw std::runtime_error("row outside the vector");
-  // estimated work (updates) of each proton row, for balancing the XCD map
-  std::vector<double> rowwork(nP, 0.0);
-  {
-    std::vector<int64_t> ncnt((size_t)N.nsec * nd, 0);  // neutron jumps per (target sector, class)
-    for (int n = 0; n < nN; n++)
-      for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) ncnt[(size_t)N.sdsec[n] * nd + t.class_of_op[O1.op[q]]]++;
-    std::vector<int64_t> nnsec(N.nsec, 0);
-    for (int n = 0; n < nN; n++) nnsec[N.sdsec[n]] += t.tb[1].rowptr[n + 1] - t.tb[1].rowptr[n];
-    for (int f = 0; f < nP; f++) {
-      const int c = P.conj[P.sdsec[f]];
-      double w = (double)(t.tb[0].rowptr[f + 1] - t.tb[0].rowptr[f]) * nlen[c] + (double)nnsec[c];
-      for (int k = 0; k < nd; k++)
-        w += (double)(pcls[(size_t)f * (nd + 1) + k + 1] - pcls[(size_t)f * (nd + 1) + k]) * ncnt[(size_t)c * nd + negk_of(t, k)];
-      rowwork[f] = w + nlen[c];
-    }
-  }
-  // tiles for each R, and two workgroup->tile maps: identity, and XCD-aware
-  // (8 contiguous chunks of equal work; chunk j on the workgroups with
-  // id % 8 == j, which the dispatcher places on XCD j)
+  // tiles for each R, and two workgroup->tile maps: identity (sector order)
+  // and XCD-grouped (below)
   for (int v = 0; v < 3; v++) {
     std::vector<int32_t> f0, nr;
-    std::vector<double> tw;
     for (int sp = 0; sp < P.nsec; sp++)
       for (int64_t f = P.secfirst[sp]; f < P.secfirst[sp] + P.seccount[sp]; f += kTileRows[v]) {
         f0.push_back((int32_t)f);
         int n = (int)std::min<int64_t>(kTileRows[v], P.secfirst[sp] + P.seccount[sp] - f);
         nr.push_back(n);
-        double w = 0.0;
-        for (int r = 0; r < n; r++) w += rowwork[f + r];
-        tw.push_back(w);
       }
     const int nt = (int)f0.size();
     s->tile_f0[v].upload(f0);
@@ -1420,22 +1399,18 @@ void upload_tables(bs_system* s) {
     for (int i = 0; i < nt; i++) ident[i] = i;
     s->tile_map[v][0].upload(ident);
     s->nslots[v][0] = nt;
-    constexpr int kXcd = 8;
-    double total = 0.0;
-    for (double w : tw) total += w;
-    std::vector<int> cstart(kXcd + 1, nt);
-    cstart[0] = 0;
-    double acc = 0.0;
-    int j = 1;
-    for (int i = 0; i < nt && j < kXcd; i++) {
-      acc += tw[i];
-      while (j < kXcd && acc >= total * j / kXcd) cstart[j++] = i + 1;
+    // XCD-grouped order: workgroup slot s runs on XCD s % 8 (round-robin
+    // dispatch); slot 8j + x takes tile (j / kGroup) * 8 * kGroup + x * kGroup
+    // + j % kGroup, so each XCD walks runs of kGroup consecutive tiles (which
+    // share source rows in its L2) while all XCDs advance together
+    constexpr int kXcd = 8, kGroup = 32;
+    const int nsl = (nt + kXcd * kGroup - 1) / (kXcd * kGroup) * (kXcd * kGroup);
+    std::vector<int32_t> xmap((size_t)nsl, -1);
+    for (int sl = 0; sl < nsl; sl++) {
+      const int x = sl % kXcd, j = sl / kXcd;
+      const int tile = (j / kGroup) * kXcd * kGroup + x * kGroup + j % kGroup;
+      xmap[sl] = tile < nt ? tile : -1;
     }
-    int maxlen = 0;
-    for (int q = 0; q < kXcd; q++) maxlen = std::max(maxlen, cstart[q + 1] - cstart[q]);
-    std::vector<int32_t> xmap((size_t)maxlen * kXcd, -1);
-    for (int q = 0; q < kXcd; q++)
-      for (int i = cstart[q]; i < cstart[q + 1]; i++) xmap[(size_t)(i - cstart[q]) * kXcd + q] = i;
     s->tile_map[v][1].upload(xmap);
     s->nslots[v][1] = (int)xmap.size();
   }
