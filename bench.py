@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", type=int, default=4, help="benchmark configuration (4 = pf 6p6n)")
+    ap.add_argument("--config", type=int, default=4,
+                    help="benchmark configuration (4 = pf 6p6n; 2 = the 64-Hamiltonian sd 4p4n ensemble, batched)")
     ap.add_argument("--solve", action="store_true",
                     help="also time the whole eigensolve (thick-restart Lanczos, keep 10, restart at 40, "
                          "lowest 5, FP64 polish) in FP64 and in mixed precision, and report eigenpairs/s "
@@ -262,6 +263,106 @@ def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work, n_ops):
     return out
 
 
+def ensemble_bench(a):
+    """The batched ensemble (64 seeded Hamiltonians on the sd 4p4n basis,
+    D = 28,503): one interleaved 64-vector H x per step; with --solve, the
+    lockstep batched Lanczos (150 steps, lowest 5, FP64 polish) in mixed
+    precision plus Hellmann-Feynman gradients of all 5 x 64 eigenpairs, against
+    the plain FP64 single-Hamiltonian path solving the same 64 one by one."""
+    import numpy as np
+    import torch
+    from bigstickpublick_amd import Batch, System
+    from bigstickpublick_amd.inputs import config_case, write_case
+    K = 64
+    work = tempfile.mkdtemp(prefix="bsbench_")
+    cases = [config_case(2, k) for k in range(K)]
+    for c in cases:
+        write_case(c, work)
+    sps = os.path.join(work, "sd.sps")
+    files = [os.path.join(work, c.name + ".int") for c in cases]
+    t0 = time.time()
+    b = Batch(sps, files, 4, 4, 0)
+    t_build = time.time() - t0
+    D = b.dim
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randn((D, K), device="cuda", dtype=torch.float32, generator=g)
+    y = torch.empty_like(x)
+    import ctypes
+    from bigstickpublick_amd.engine import BS_F32, load_library
+    L = load_library()
+    for _ in range(max(a.warmup, 1)):
+        b.apply(x, y)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.time()
+    ev0.record()
+    for _ in range(a.steps):
+        rc = L.bs_batch_apply(b.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), BS_F32)
+        if rc:
+            raise RuntimeError(L.bs_last_error().decode())
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    ms = ev0.elapsed_time(ev1) / a.steps
+    chosen, tms = b.variant_info("f32")
+    out = {
+        "metric": "H·v/s of the 64-Hamiltonian batched matvec (sd 4p4n ensemble)",
+        "value": K * a.steps / wall, "unit": "H·v/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall * 1000.0 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "sd4p4n ensemble of %d seeded Hamiltonians, M=0 D=%d, interleaved FP32 H x" % (K, D),
+                   "dim": D, "hamiltonians": K},
+        "kernel_ms": ms, "batch_build_s": t_build, "variant": chosen,
+        "variant_ms": [round(float(v), 4) for v in tms],
+    }
+    del x, y
+    if a.solve:
+        res = {}
+        for dt in ("f64", "f32"):
+            b.lanczos(nkeep=5, niter=150, dtype=dt, polish=True, want_vectors=(dt == "f32"))  # capture graphs
+            torch.cuda.synchronize()
+            t0 = time.time()
+            r = b.lanczos(nkeep=5, niter=150, dtype=dt, polish=True, want_vectors=(dt == "f32"))
+            torch.cuda.synchronize()
+            ts = time.time() - t0
+            tg = 0.0
+            if dt == "f32":
+                t0 = time.time()
+                grads = [b.gradient(r.evecs[q]) for q in range(5)]
+                tg = time.time() - t0
+            res[dt] = (r, ts, tg)
+        r64, t64b, _ = res["f64"]
+        r32, t32, tg = res["f32"]
+        err = np.abs(r32.evals - r64.evals)
+        ok = int((err <= 1e-4).sum())
+        # plain FP64 path: each Hamiltonian on its own (tables built outside the timing)
+        systems = [System(sps, f, 4, 4, 0, upload=True) for f in files]
+        for s in systems:
+            s.autotune("f64")
+        torch.cuda.synchronize()
+        t0 = time.time()
+        single = [s.lanczos(nkeep=5, niter=150, dtype="f64", polish=True).evals for s in systems]
+        torch.cuda.synchronize()
+        t_single = time.time() - t0
+        err_single = np.abs(np.array(single) - r64.evals).max()
+        out["solve"] = {
+            "plain_fp64_single": {"seconds": t_single, "eigenpairs": 5 * K, "eigenpairs_per_s": 5 * K / t_single,
+                                  "max_abs_diff_vs_batched_fp64": float(err_single)},
+            "batched_fp64": {"seconds": t64b, "eigenpairs_per_s": 5 * K / t64b},
+            "batched_mixed_with_gradients": {"seconds_solve": t32, "seconds_gradients": tg,
+                                             "gradient_shape": [5] + list(grads[0].shape),
+                                             "within_1e-4": ok, "of": int(err.size),
+                                             "max_abs_err_vs_fp64": float(err.max()),
+                                             "eigenpairs_per_s": ok / (t32 + tg)},
+        }
+        out["solve"]["speedup_vs_plain_fp64"] = (out["solve"]["batched_mixed_with_gradients"]["eigenpairs_per_s"]
+                                                 / out["solve"]["plain_fp64_single"]["eigenpairs_per_s"])
+    b.close()
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.traffic:
@@ -288,6 +389,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.config == 2:
+        ensemble_bench(a)
+        return
     import numpy as np
     import torch
     from bigstickpublick_amd import System

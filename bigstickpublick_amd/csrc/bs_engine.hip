@@ -261,6 +261,22 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     for (int r = 0; r < R; r++) {
       for (int b0 = p0[r]; b0 < p1[r]; b0 += G) {
         const int nb = min(G, p1[r] - b0);
+        // this batch's signed coefficients, loaded first so that their
+        // latency overlaps the source-row staging (one or two per thread)
+        constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
+        float wreg[kWbPer];
+#pragma unroll
+        for (int u = 0; u < kWbPer; u++) {
+          const int i = tid + u * BLOCK;
+          const int row = i / G, g = i - row * G, nl = row >> 1;
+          float w = 0.0f;
+          if (i < 2 * (wsd + 1) * G && g < nb && nl < wsd) {
+            const int meta = tb.p1_meta[b0 + g];
+            w = Wk[(meta & 0xff) * wsd + nl];
+            if ((meta >> 8) ^ (row & 1)) w = -w;
+          }
+          wreg[u] = w;
+        }
         __syncthreads();
         for (int g = 0; g < G; g++) {
           if (g < nb) {
@@ -287,15 +303,10 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
             for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = T(0);
           }
         }
-        for (int i = tid; i < 2 * (wsd + 1) * G; i += BLOCK) {
-          const int row = i / G, g = i - row * G, nl = row >> 1;
-          float w = 0.0f;
-          if (g < nb && nl < wsd) {
-            const int meta = tb.p1_meta[b0 + g];
-            w = Wk[(meta & 0xff) * wsd + nl];
-            if ((meta >> 8) ^ (row & 1)) w = -w;
-          }
-          wb[i] = w;
+#pragma unroll
+        for (int u = 0; u < kWbPer; u++) {
+          const int i = tid + u * BLOCK;
+          if (i < 2 * (wsd + 1) * G) wb[i] = wreg[u];
         }
         __syncthreads();
         // column-major over the thread's columns: each (wave, q) runs to its
