@@ -93,6 +93,7 @@ struct DevTables {
   const uint32_t* n1_pk;      // src | nl << 16 | sign << 24
   const uint32_t* n1_pkv[3][2];  // per G (1, 2, 4) and T (f32, f64): LDS byte offsets, see k_apply_tiles
   const uint8_t* n1_chunk;       // [(c*ndelta + k)*64 + j]: entries used by columns 64j..64j+63
+  const uint32_t* n1_pkr;        // src | (2*op + sign) << 16 (multi-vector kernels)
   // pn element blocks: class k block is [size_k][size_-k] at woff[k]
   const float* W;
   const int32_t* woff;
@@ -769,17 +770,23 @@ __global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals b
 }
 
 // K interleaved vectors per basis state on large bases: the tile kernel's
-// structure (one workgroup per proton row, CPT columns per thread) with every
-// jump-table and index read serving K vectors.  LDS: staged rows [col][G][K]
-// followed by Wb[nl][G][K].  Values per Hamiltonian (vs = K, vk = 1) or
+// design (one workgroup per proton row, CPT columns per thread, per-chunk
+// entry bounds, signed coefficient tables, buffer-load table walks) with every
+// jump-table read, index decode and barrier serving K vectors.  Dynamic LDS:
+//   [0, kWbRows*G*K*4)  Wb[2*op + s][g][k], the batch's signed coefficients
+//                       of each Hamiltonian k
+//   [kWbRows*G*K*4, ..) staged rows [col][K] (NN) or [col][G][K] (PN)
+// Neutron entries come as src | (2*op + sign) << 16 and are scaled to LDS
+// byte offsets with two shifts.  Values per Hamiltonian (vs = K, vk = 1) or
 // shared (vs = 1, vk = 0).
 template <typename T, int BLOCK, int CPT, int K, int G>
 __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals bv, const T* __restrict__ x,
                                                        T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int GK = G * K;
-  T* xs = reinterpret_cast<T*>(smem);
-  float* wb = reinterpret_cast<float*>(smem + (size_t)GK * tb.lmax * sizeof(T));
+  constexpr int XS = kWbRows * GK * (int)sizeof(float);
+  T* xs = reinterpret_cast<T*>(smem + XS);
+  float* wb = reinterpret_cast<float*>(smem);
   const int tid = threadIdx.x;
   const int tile = tb.tile_map[blockIdx.x];
   if (tile < 0) return;
@@ -790,66 +797,74 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
   const int nf0 = tb.nfirst[cn];
   const int vs = bv.vs, vk = bv.vk;
   const int64_t bf = tb.pstart[f];
+  using XV = VecG<T, K>;
+  uint32_t cl[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) cl[q] = (uint32_t)min(tid + q * BLOCK, L - 1);
   T acc[CPT][K];
 #pragma unroll
   for (int q = 0; q < CPT; q++)
 #pragma unroll
     for (int k = 0; k < K; k++) acc[q][k] = T(0);
-  using XV = VecG<T, K>;
   // ---- SPE
   if (tb.call_spe) {
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-      const int c = tid + q * BLOCK;
-      if (c < L) {
-        const XV xv = *reinterpret_cast<const XV*>(x + (bf + c) * K);
+      const XV xv = *reinterpret_cast<const XV*>(x + (bf + cl[q]) * K);
 #pragma unroll
-        for (int k = 0; k < K; k++)
-          acc[q][k] += ((T)bv.pspe[(int64_t)f * vs + k * vk] + (T)bv.nspe[(int64_t)(nf0 + c) * vs + k * vk]) * xv.v[k];
-      }
+      for (int k = 0; k < K; k++)
+        acc[q][k] = fma((T)bv.pspe[(int64_t)f * vs + k * vk] + (T)bv.nspe[(int64_t)(nf0 + cl[q]) * vs + k * vk],
+                        xv.v[k], acc[q][k]);
     }
   }
   // ---- PP: row AXPYs, K-wide coalesced reads
   for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
-    const int64_t bi = tb.pstart[tb.pp_src[e]];
+    const T* xi = x + tb.pstart[tb.pp_src[e]] * K;
     T v[K];
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = bval<T>(bv.pp32, bv.pp64, e * vs + k * vk);
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-      const int c = tid + q * BLOCK;
-      if (c < L) {
-        const XV xv = *reinterpret_cast<const XV*>(x + (bi + c) * K);
+      const XV xv = *reinterpret_cast<const XV*>(xi + (int64_t)cl[q] * K);
 #pragma unroll
-        for (int k = 0; k < K; k++) acc[q][k] += v[k] * xv.v[k];
-      }
+      for (int k = 0; k < K; k++) acc[q][k] = fma(v[k], xv.v[k], acc[q][k]);
     }
   }
   // ---- NN: the row staged [col][K]
   {
-    for (int i = tid; i < L * K; i += BLOCK) xs[i] = x[bf * K + i];
+    for (int c = tid; c < L; c += BLOCK)
+      *reinterpret_cast<XV*>(xs + (size_t)c * K) = *reinterpret_cast<const XV*>(x + (bf + c) * K);
     __syncthreads();
     const XV* xr = reinterpret_cast<const XV*>(xs);
-    const int64_t base = tb.nn_base[cn];
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(tb.nn_src + tb.nn_base[cn]);
+    const int64_t nb0 = tb.nn_base[cn];
     const int E = tb.nn_emax[cn];
     for (int e = 0; e < E; e++) {
+      const uint32_t so = (uint32_t)e * (uint32_t)L * 4u;
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
-        const int c = tid + q * BLOCK;
-        if (c < L) {
-          const int64_t o = base + (int64_t)e * L + c;
-          const XV xv = xr[tb.nn_src[o]];
+        const int sl = (int)buf_u32(rs, cl[q] * 4u, so);
+        const int64_t o = nb0 + (int64_t)e * L + cl[q];
+        const XV xv = xr[sl];
+        T v[K];
+        if (vk && sizeof(T) == 4) {  // K contiguous FP32 values of this slot
+          const VecG<float, K> vv = *reinterpret_cast<const VecG<float, K>*>(bv.nn32 + o * K);
 #pragma unroll
-          for (int k = 0; k < K; k++) acc[q][k] += bval<T>(bv.nn32, bv.nn64, o * vs + k * vk) * xv.v[k];
+          for (int k = 0; k < K; k++) v[k] = (T)vv.v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < K; k++) v[k] = bval<T>(bv.nn32, bv.nn64, o * vs + k * vk);
         }
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[q][k] = fma(v[k], xv.v[k], acc[q][k]);
       }
     }
   }
   // ---- PN: per Delta class, batches of G source rows staged [col][G][K]
   using GV = VecG<T, GK>;
   using WV = VecG<float, GK>;
-  const GV* xg = reinterpret_cast<const GV*>(xs);
-  const WV* wg = reinterpret_cast<const WV*>(wb);
+  constexpr int XSH = __builtin_ctz(GK * (int)sizeof(T));  // log2 bytes per staged column
+  constexpr int WSH = __builtin_ctz(GK * (int)sizeof(float));
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k0 = 0; k0 < nd; k0++) {
     const int p0 = tb.pcls[(int64_t)f * nd1 + k0], p1 = tb.pcls[(int64_t)f * nd1 + k0 + 1];
@@ -859,49 +874,65 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
     const int Li = tb.nlen[tb.pconj[si]];
     const int wsd = tb.wstride[k0];
     const int64_t woff = tb.woff[k0];
-    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
-    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
+    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(tb.n1_pkr + tb.n1_base[(int64_t)cn * nd + kn]);
+    const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
+    int Eq[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) Eq[q] = ch[q * (BLOCK / 64)];
     for (int b0 = p0; b0 < p1; b0 += G) {
       const int nb = min(G, p1 - b0);
       __syncthreads();
       for (int g = 0; g < G; g++) {
         const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
-        for (int i = tid; i < Li * K; i += BLOCK) {
-          const int c = i / K, k = i - c * K;
-          xs[(c * G + g) * K + k] = g < nb ? x[bi * K + i] : T(0);
+        for (int c = tid; c < Li; c += BLOCK) {
+          XV v;
+          if (g < nb) v = *reinterpret_cast<const XV*>(x + (bi + c) * K);
+          else
+#pragma unroll
+            for (int k = 0; k < K; k++) v.v[k] = T(0);
+          *reinterpret_cast<XV*>(xs + ((size_t)c * G + g) * K) = v;
         }
       }
-      for (int i = tid; i < (wsd + 1) * GK; i += BLOCK) {
-        const int nl = i / GK, r = i - nl * GK, g = r / K, k = r - g * K;
+      for (int i = tid; i < 2 * (wsd + 1) * GK; i += BLOCK) {
+        const int row = i / GK, r2 = i - row * GK, g = r2 / K, k = r2 - g * K, nl = row >> 1;
         float w = 0.0f;
         if (g < nb && nl < wsd) {
           const int meta = tb.p1_meta[b0 + g];
           w = bv.W[(woff + (int64_t)(meta & 0xff) * wsd + nl) * vs + k * vk];
-          if (meta >> 8) w = -w;
+          if ((meta >> 8) ^ (row & 1)) w = -w;
         }
         wb[i] = w;
       }
       __syncthreads();
-      const uint32_t pad = (uint32_t)wsd << 16;
-      for (int e = 0; e < E; e++) {
-        uint32_t pk[CPT];
+      const uint32_t step = (uint32_t)L * 4u;
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int c = tid + q * BLOCK;
-          pk[q] = c < L ? tb.n1_pk[ebase + (int64_t)e * L + c] : pad;
-        }
+      for (int q = 0; q < CPT; q++) {
+        const int Ee = Eq[q];
+        const uint32_t vo = cl[q] * 4u;
+        int e = 0;
+        // four entries per step, their loads in flight together
+        for (; e + 4 <= Ee; e += 4) {
+          uint32_t pk[4];
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const GV xv = xg[pk[q] & 0xffff];
-          const WV wv = wg[(pk[q] >> 16) & 0xff];
-          const bool neg = (pk[q] >> 24) & 1;
+          for (int u = 0; u < 4; u++) pk[u] = buf_u32(rpk, vo, (uint32_t)(e + u) * step);
 #pragma unroll
-          for (int k = 0; k < K; k++) {
-            T d = T(0);
+          for (int u = 0; u < 4; u++) {
+            const GV xv = *reinterpret_cast<const GV*>(smem + XS + ((pk[u] & 0xffffu) << XSH));
+            const WV wv = *reinterpret_cast<const WV*>(smem + ((pk[u] >> 16) << WSH));
 #pragma unroll
-            for (int g = 0; g < G; g++) d += (T)wv.v[g * K + k] * xv.v[g * K + k];
-            acc[q][k] += neg ? -d : d;
+            for (int k = 0; k < K; k++)
+#pragma unroll
+              for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * K + k], xv.v[g * K + k], acc[q][k]);
           }
+        }
+        for (; e < Ee; e++) {
+          const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
+          const GV xv = *reinterpret_cast<const GV*>(smem + XS + ((pk & 0xffffu) << XSH));
+          const WV wv = *reinterpret_cast<const WV*>(smem + ((pk >> 16) << WSH));
+#pragma unroll
+          for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * K + k], xv.v[g * K + k], acc[q][k]);
         }
       }
     }
@@ -1212,6 +1243,7 @@ struct bs_system {
   DevBuf<int16_t> p1_meta;
   DevBuf<uint32_t> n1_pk, n1_pkv[3][2];
   DevBuf<uint8_t> n1_chunk;
+  DevBuf<uint32_t> n1_pkr;
   DevBuf<float> pspe_sd, nspe_sd, pp_val32, nn_val32, W;
   DevBuf<double> pp_val64, nn_val64;
 };
@@ -1453,6 +1485,14 @@ void upload_tables(bs_system* s) {
   s->n1_emax.upload(n1_emax);
   s->n1_pk.upload(n1_pk);
   s->n1_chunk.upload(n1_chunk);
+  {
+    std::vector<uint32_t> v(n1_pk.size());
+    for (size_t i = 0; i < n1_pk.size(); i++) {
+      const uint32_t pk = n1_pk[i];
+      v[i] = (pk & 0xffffu) | ((2u * ((pk >> 16) & 0xffu) + ((pk >> 24) & 1u)) << 16);
+    }
+    s->n1_pkr.upload(v);
+  }
   // the same entries as LDS byte offsets for each staged-row count G and
   // element type: x offset src*G*sizeof(T) (< 2^17) | Wb offset
   // (2*nl + sign)*G*4 << 17; padding (nl = class size) lands on a zero row
@@ -1507,6 +1547,7 @@ void upload_tables(bs_system* s) {
   d.n1_emax = s->n1_emax.p;
   d.n1_pk = s->n1_pk.p;
   d.n1_chunk = s->n1_chunk.p;
+  d.n1_pkr = s->n1_pkr.p;
   for (int gi = 0; gi < 3; gi++)
     for (int ti = 0; ti < 2; ti++) d.n1_pkv[gi][ti] = s->n1_pkv[gi][ti].p;
   d.W = s->W.p;
@@ -2026,7 +2067,7 @@ void launch_multi_kg(const bs_system* s, int map, const BatchVals& bv, const T* 
   d.tile_f0 = s->tile_f0[0].p;
   d.tile_nr = s->tile_nr[0].p;
   d.tile_map = s->tile_map[0][map].p;
-  const size_t lds = (size_t)G * K * d.lmax * sizeof(T) + (size_t)d.wcols * G * K * sizeof(float);
+  const size_t lds = (size_t)G * K * d.lmax * sizeof(T) + (size_t)kWbRows * G * K * sizeof(float);
   if (lds > kLdsLimit) throw std::runtime_error("multi-vector variant does not fit in LDS");
   auto fn = k_apply_multi<T, kApplyBlock, CPT, K, G>;
   static bool attr_set = false;
@@ -2061,7 +2102,7 @@ bool multi_fits(const bs_system* s, int v, int K, size_t tsize) {
   if (v == 0) return true;
   if (K != 1 && K != 2 && K != 4 && K != 8) return false;
   const int G = (v - 1) / 2 == 0 ? 1 : 2;
-  return (size_t)G * K * s->dt.lmax * tsize + (size_t)s->dt.wcols * G * K * sizeof(float) <= kLdsLimit;
+  return (size_t)G * K * s->dt.lmax * tsize + (size_t)kWbRows * G * K * sizeof(float) <= kLdsLimit;
 }
 
 template <typename T>
