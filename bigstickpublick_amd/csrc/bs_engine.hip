@@ -187,7 +187,29 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   for (int r = 0; r < R; r++) {
     if (r < nr) {
       const int f = f0 + r;
-      for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++) {
+      // source rows through a scalar buffer descriptor per entry (no vector
+      // address math), two entries' loads in flight together
+      const int64_t e0 = tb.pp_ptr[f], e1 = tb.pp_ptr[f + 1];
+      int64_t e = e0;
+      for (; e + 2 <= e1; e += 2) {
+        const __amdgpu_buffer_rsrc_t ra = buf_rsrc(x + tb.pstart[tb.pp_src[e]]);
+        const __amdgpu_buffer_rsrc_t rb = buf_rsrc(x + tb.pstart[tb.pp_src[e + 1]]);
+        const T va = vload<T>(tb.pp_val32, tb.pp_val64, e), vb = vload<T>(tb.pp_val32, tb.pp_val64, e + 1);
+        T xa[CPT], xb[CPT];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          if constexpr (sizeof(T) == 4) {
+            xa[q] = __uint_as_float(buf_u32(ra, cl[q] * 4u, 0u));
+            xb[q] = __uint_as_float(buf_u32(rb, cl[q] * 4u, 0u));
+          } else {
+            xa[q] = buf_f64(ra, cl[q] * 8u, 0u);
+            xb[q] = buf_f64(rb, cl[q] * 8u, 0u);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) acc[r][q] = fma(vb, xb[q], fma(va, xa[q], acc[r][q]));
+      }
+      if (e < e1) {
         const T* xi = x + tb.pstart[tb.pp_src[e]];
         const T v = vload<T>(tb.pp_val32, tb.pp_val64, e);
 #pragma unroll
