@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -221,10 +222,17 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
   // per entry serves every row of the tile
   {
     VecG<T, R>* xr = reinterpret_cast<VecG<T, R>*>(xs);
+    {
+      // one column of all R rows per thread, one wide conflict-free LDS store
+      int64_t bfr[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int64_t bf = r < nr ? tb.pstart[f0 + r] : 0;
-      for (int c = tid; c < L; c += BLOCK) xs[c * R + r] = r < nr ? x[bf + c] : T(0);
+      for (int r = 0; r < R; r++) bfr[r] = r < nr ? tb.pstart[f0 + r] : -1;
+      for (int c = tid; c < L; c += BLOCK) {
+        VecG<T, R> v;
+#pragma unroll
+        for (int r = 0; r < R; r++) v.v[r] = bfr[r] >= 0 ? x[bfr[r] + c] : T(0);
+        xr[c] = v;
+      }
     }
     __syncthreads();
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(tb.nn_src + tb.nn_base[cn]);
@@ -301,29 +309,18 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           wreg[u] = w;
         }
         __syncthreads();
-        for (int g = 0; g < G; g++) {
-          if (g < nb) {
-            // 16-byte loads from the first 16-byte aligned column on
-            const int64_t bi = tb.pstart[tb.p1_src[b0 + g]];
-            constexpr int VW = 16 / (int)sizeof(T);
-            // columns before the first 16-byte boundary (from the address,
-            // so any element-aligned x works), at most the row length
-            const int head = min(Li, (int)(((16u - ((uint32_t)(uintptr_t)(x + bi) & 15u)) & 15u) / sizeof(T)));
-            const int nvec = (Li - head) / VW;
-            using V16 = typename Vec16<T>::type;
-            const V16* xv = reinterpret_cast<const V16*>(x + bi + head);
-            for (int v = tid; v < nvec; v += BLOCK) {
-              const V16 q = xv[v];
-              const T* pq = reinterpret_cast<const T*>(&q);
-              const int c = head + v * VW;
+        {
+          // each thread gathers one column of all G source rows (G coalesced
+          // loads) and writes it with one wide LDS store: consecutive lanes
+          // write consecutive LDS words, so the stores do not conflict
+          int64_t bi[G];
 #pragma unroll
-              for (int u = 0; u < VW; u++) xs[(c + u) * G + g] = pq[u];
-            }
-            const int tail0 = head + nvec * VW;
-            if (tid < head) xs[tid * G + g] = x[bi + tid];
-            for (int c = tail0 + tid; c < Li; c += BLOCK) xs[c * G + g] = x[bi + c];
-          } else {
-            for (int c = tid; c < Li; c += BLOCK) xs[c * G + g] = T(0);
+          for (int g = 0; g < G; g++) bi[g] = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : -1;
+          for (int c = tid; c < Li; c += BLOCK) {
+            VecG<T, G> v;
+#pragma unroll
+            for (int g = 0; g < G; g++) v.v[g] = bi[g] >= 0 ? x[bi[g] + c] : T(0);
+            *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = v;
           }
         }
 #pragma unroll
@@ -1522,9 +1519,10 @@ void upload_tables(bs_system* s) {
   for (int gi = 0; gi < 3; gi++)
     for (int ti = 0; ti < 2; ti++) {
       const uint32_t G = 1u << gi, ts = ti == 0 ? 4u : 8u;
-      std::vector<uint32_t> v(n1_pk.size());
-      for (size_t i = 0; i < n1_pk.size(); i++) {
-        const uint32_t pk = n1_pk[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
+      const std::vector<uint32_t>& ord = n1_pk;
+      std::vector<uint32_t> v(ord.size());
+      for (size_t i = 0; i < ord.size(); i++) {
+        const uint32_t pk = ord[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
         const uint32_t xo = sl * G * ts, wo = (2u * nl + sg) * G * 4u;
         if (xo >= (1u << 17) || wo >= (1u << 15)) throw std::runtime_error("neutron jump offsets do not pack");
         v[i] = xo | (wo << 17);
