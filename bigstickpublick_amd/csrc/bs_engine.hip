@@ -901,15 +901,24 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
     for (int b0 = p0; b0 < p1; b0 += G) {
       const int nb = min(G, p1 - b0);
       __syncthreads();
-      for (int g = 0; g < G; g++) {
-        const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
-        for (int c = tid; c < Li; c += BLOCK) {
-          XV v;
-          if (g < nb) v = *reinterpret_cast<const XV*>(x + (bi + c) * K);
-          else
+      {
+        // one column of all G source rows per thread, one wide LDS store
+        int64_t bi[G];
 #pragma unroll
-            for (int k = 0; k < K; k++) v.v[k] = T(0);
-          *reinterpret_cast<XV*>(xs + ((size_t)c * G + g) * K) = v;
+        for (int g = 0; g < G; g++) bi[g] = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : -1;
+        for (int c = tid; c < Li; c += BLOCK) {
+          GV v;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            XV u;
+            if (bi[g] >= 0) u = *reinterpret_cast<const XV*>(x + (bi[g] + c) * K);
+            else
+#pragma unroll
+              for (int k = 0; k < K; k++) u.v[k] = T(0);
+#pragma unroll
+            for (int k = 0; k < K; k++) v.v[g * K + k] = u.v[k];
+          }
+          *reinterpret_cast<GV*>(xs + (size_t)c * GK) = v;
         }
       }
       for (int i = tid; i < 2 * (wsd + 1) * GK; i += BLOCK) {
