@@ -102,3 +102,34 @@ def test_wfn_writer_reproduces_reference_file(name, Z, N, jz2, seed, casedir, tm
     s.write_wfn(out, [x["e"] for x in st], [x["xj"] for x in st], [x["xt2"] for x in st],
                 np.stack([x["vec"] for x in st]))
     assert open(out, "rb").read() == open(ref, "rb").read()
+
+
+def test_entry_points_refuse_bad_arguments(casedir, tmp_path):
+    """Argument checks that return before any device work: a nonzero code and
+    a message, never a crash."""
+    import ctypes
+    from bigstickpublick_amd.engine import BS_F32, load_library
+    L = load_library()
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    sps, intf = paths(case, d)
+    s = System(sps, intf, 2, 2, 0)  # host tables only
+    # device entry points on a system that was never uploaded
+    assert L.bs_apply_block(s.h, None, None, 4, BS_F32) != 0
+    assert L.bs_multi_variant(s.h, BS_F32, 4, 1, 1) != 0
+    # batches: K out of range, missing names
+    h = ctypes.c_void_p()
+    names = (ctypes.c_char_p * 1)(intf.encode())
+    assert L.bs_batch_init(sps.encode(), names, 0, 2, 2, 0, 1, 1, 1, 1, ctypes.byref(h)) != 0
+    assert L.bs_batch_init(sps.encode(), names, 65, 2, 2, 0, 1, 1, 1, 1, ctypes.byref(h)) != 0
+    nulls = (ctypes.c_char_p * 2)(intf.encode(), None)
+    assert L.bs_batch_init(sps.encode(), nulls, 2, 2, 2, 0, 1, 1, 1, 1, ctypes.byref(h)) != 0
+    assert b"null" in L.bs_last_error()
+    assert L.bs_batch_apply(None, None, None, BS_F32) != 0
+    # .wfn: bad dtype, unwritable path
+    v = np.zeros((1, s.dim), np.float32)
+    e = np.zeros(1, np.float32)
+    assert L.bs_write_wfn(s.h, str(tmp_path / "a.wfn").encode(), 1, e.ctypes.data, e.ctypes.data,
+                          e.ctypes.data, v.ctypes.data, 3, 0) != 0
+    assert L.bs_write_wfn(s.h, str(tmp_path / "no" / "such" / "dir.wfn").encode(), 1, e.ctypes.data,
+                          e.ctypes.data, e.ctypes.data, v.ctypes.data, BS_F32, 0) != 0
+    assert b"cannot open" in L.bs_last_error()
