@@ -2822,6 +2822,46 @@ int bs_observable(bs_system* s, int which, const void* psi, int dtype, double* v
 }
 
 
+int bs_operator_matrix(bs_system* s, const char* intf, float spscale, float a0, float a, float x, const void* vecs,
+                       int nvec, int dtype, double* out) {
+  try {
+    checked(s);
+    if (!intf || !vecs || !out || nvec < 1) return set_err("null argument or nvec < 1");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    // the operator as an interaction on the same orbits, nucleus and 2Jz
+    bs_system o;
+    o.t = bs::build_tables(s->sps_path, intf, s->t.in.Z, s->t.in.N, s->t.in.jz2, spscale, a0, a, x, true);
+    o.sps_path = s->sps_path;
+    if (o.t.dim != s->t.dim) return set_err("operator basis differs");
+    upload_tables(&o);
+    const int64_t D = s->t.dim;
+    const size_t es = dtype == BS_F32 ? 4 : 8;
+    const int nb = std::min(grid_for(D), 1024);
+    DevBuf<unsigned char> hv;
+    DevBuf<double> part, res;
+    hv.alloc(D * es);
+    part.alloc((size_t)nb * nvec);
+    res.alloc(nvec);
+    for (int i = 0; i < nvec; i++) {
+      const void* vi = static_cast<const char*>(vecs) + (size_t)i * D * es;
+      // (src/bexpect.f90:143-160) O v_i, then <v_f | O v_i> for every f
+      if (dtype == BS_F32) {
+        launch_apply<float>(&o, (const float*)vi, (float*)hv.p, 0, nullptr);
+        k_multidot<float><<<nb, kBlock>>>((const float*)vecs, D, nvec, (const float*)hv.p, D, part.p);
+      } else {
+        launch_apply<double>(&o, (const double*)vi, (double*)hv.p, 0, nullptr);
+        k_multidot<double><<<nb, kBlock>>>((const double*)vecs, D, nvec, (const double*)hv.p, D, part.p);
+      }
+      k_reduce<<<nvec, kBlock>>>(part.p, nb, nvec, res.p, 0);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpy(out + (size_t)i * nvec, res.p, sizeof(double) * nvec, hipMemcpyDeviceToHost));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
 int bs_num_params(const bs_system* s, int* nspe, int* ntbme) {
   if (!s) return set_err("null system");
   if (nspe) *nspe = (int)s->t.in.orbits.size();

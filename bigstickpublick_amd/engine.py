@@ -60,6 +60,7 @@ def load_library():
                              ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
     L.bs_write_wfn.argtypes = [vp, ctypes.c_char_p, i32, vp, vp, vp, vp, i32, i32]
+    L.bs_operator_matrix.argtypes = [vp, ctypes.c_char_p, f32, f32, f32, f32, vp, i32, i32, vp]
     L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
     L.bs_batch_set_variant.argtypes = [vp, i32, i32]
     L.bs_multi_info.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), vp]
@@ -281,6 +282,18 @@ class System:
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         return load_library().bs_multi_variant(self.h, dt, K, 1, v) == 0
+
+    def operator_matrix(self, intfile, vecs, scaling=(1.0, 1.0, 1.0, 1.0)):
+        """<v_f|O|v_i> for the two-body operator in `intfile` (iso .int) and an
+        (nvec, dim) CUDA tensor of vectors; returns an (nvec, nvec) array [i, f]."""
+        self.upload()
+        v = vecs.contiguous()
+        n = v.shape[0]
+        out = np.zeros((n, n))
+        dt = BS_F32 if v.dtype.itemsize == 4 else BS_F64
+        _check(load_library().bs_operator_matrix(self.h, str(intfile).encode(), *[float(s) for s in scaling],
+                                                 ctypes.c_void_p(v.data_ptr()), n, dt, out.ctypes.data))
+        return out
 
     def write_wfn(self, path, energy, xj, xt2, vecs):
         """Write the reference's binary .wfn: energies, J, T(T+1) (length

@@ -156,6 +156,14 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Matrix <v_f | O | v_i> of a scalar two-body operator O, given as an iso .int
+ * file with its scaling line, between nvec vectors (device, dim elements
+ * each, consecutive) on this system's basis: out[i*nvec + f].  Replaces
+ * expectator_p (src/bexpect.f90:143-160), which applies the operator with
+ * applyHbundled_g to each stored state and overlaps it with the others. */
+int bs_operator_matrix(bs_system *sys, const char *intf, float spscale, float a0, float a, float x,
+                       const void *vecs, int nvec, int dtype, double *out);
+
 /* Write the reference's binary .wfn file (src/bwfnlib.f90:77-134 vectors,
  * :373-541 header; stream access, little-endian): the basis header, nkeep,
  * then per state magic, index, energy, J, T(T+1) and the vector in single

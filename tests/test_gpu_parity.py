@@ -132,3 +132,29 @@ def test_observables_and_host_apply(casedir):
     yd = s.apply(torch.tensor(x, device="cuda")).cpu().numpy()
     yh = s.apply_host(x)
     assert np.array_equal(yd, yh)
+
+
+def test_operator_matrix(casedir):
+    """<v_f|O|v_i> of an operator given as an .int file (the reference's
+    expectation-value path): equal to the oracle's apply of the same operator;
+    with O = H on converged Ritz vectors, the diagonal matrix of the energies."""
+    import os
+    from bigstickpublick_amd.inputs import small_case
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    s = _sys(case, d)
+    op = small_case("sd", 2, 2, 0, seed=5).interaction
+    opf = os.path.join(d, "op5.int")
+    with open(opf, "w") as f:
+        f.write(op.int_text())
+    rng = np.random.default_rng(12)
+    V = rng.standard_normal((3, s.dim))
+    M = s.operator_matrix(opf, torch.tensor(V, device="cuda"))
+    o = Oracle(paths(case, d)[0], opf, case.Z, case.N, case.jz2)
+    want = np.array([[V[f] @ o.apply(V[i]) for f in range(3)] for i in range(3)])
+    assert _rel(M, want) < 1e-12
+    M32 = s.operator_matrix(opf, torch.tensor(V, device="cuda", dtype=torch.float32))
+    assert _rel(M32, want) < 1e-5
+    r = s.lanczos(nkeep=3, niter=min(s.dim, 300), dtype="f64", polish=True, want_vectors=True)
+    H = s.operator_matrix(paths(case, d)[1], r.evecs)
+    assert np.abs(np.diag(H) - r.evals).max() < 1e-8
+    assert np.abs(H - np.diag(np.diag(H))).max() < 1e-8
