@@ -1241,7 +1241,6 @@ int grid_for(int64_t D) {
 
 struct bs_system {
   bs::Tables t;
-  std::string sps_path;
   std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
   DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
   std::vector<int64_t> nn_slot;        // ELL slot of each merged NN entry (batched mode)
@@ -2614,8 +2613,10 @@ int bs_init_host(const char* sps, const char* intf, int Z, int N, int jz2, float
   try {
     if (!out) return set_err("null output pointer");
     std::unique_ptr<bs_system> s(new bs_system());
-    s->t = bs::build_tables(sps, intf, Z, N, jz2, spscale, a0, a, x, true);
-    s->sps_path = sps;
+    if (!sps || !intf) return set_err("null file name");
+    bs::ModelInput in;
+    bs::read_sps(sps, in);
+    s->t = bs::build_tables(in.orbits, intf, Z, N, jz2, spscale, a0, a, x, true);
     *out = s.release();
     return 0;
   } catch (const std::exception& e) {
@@ -2806,9 +2807,8 @@ int bs_observable(bs_system* s, int which, const void* psi, int dtype, double* v
     if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
     if (!s->obs[which]) {
       std::unique_ptr<bs_system> o(new bs_system());
-      o->t = bs::build_tables_text(s->sps_path, bs::observable_int_text(s->t.in.orbits, which), s->t.in.Z,
+      o->t = bs::build_tables_text(s->t.in.orbits, bs::observable_int_text(s->t.in.orbits, which), s->t.in.Z,
                                    s->t.in.N, s->t.in.jz2, true);
-      o->sps_path = s->sps_path;
       if (o->t.dim != s->t.dim) return set_err("observable basis differs");
       upload_tables(o.get());
       s->obs[which] = std::move(o);
@@ -2826,14 +2826,20 @@ int bs_operator_matrix(bs_system* s, const char* intf, float spscale, float a0, 
                        int nvec, int dtype, double* out) {
   try {
     checked(s);
-    if (!intf || !vecs || !out || nvec < 1) return set_err("null argument or nvec < 1");
+    if (!vecs || !out || nvec < 1) return set_err("null argument or nvec < 1");
     if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
-    // the operator as an interaction on the same orbits, nucleus and 2Jz
+    // the operator as an interaction on the same orbits, nucleus and 2Jz, or
+    // the system's own Hamiltonian when no file is given
     bs_system o;
-    o.t = bs::build_tables(s->sps_path, intf, s->t.in.Z, s->t.in.N, s->t.in.jz2, spscale, a0, a, x, true);
-    o.sps_path = s->sps_path;
-    if (o.t.dim != s->t.dim) return set_err("operator basis differs");
-    upload_tables(&o);
+    bs_system* op = s;
+    if (intf) {
+      o.t = bs::build_tables(s->t.in.orbits, intf, s->t.in.Z, s->t.in.N, s->t.in.jz2, spscale, a0, a, x, true);
+      if (o.t.dim != s->t.dim) return set_err("operator basis differs");
+      upload_tables(&o);
+      op = &o;
+    } else if (!s->uploaded) {
+      return set_err("tables not uploaded (call bs_upload)");
+    }
     const int64_t D = s->t.dim;
     const size_t es = dtype == BS_F32 ? 4 : 8;
     const int nb = std::min(grid_for(D), 1024);
@@ -2844,12 +2850,12 @@ int bs_operator_matrix(bs_system* s, const char* intf, float spscale, float a0, 
     res.alloc(nvec);
     for (int i = 0; i < nvec; i++) {
       const void* vi = static_cast<const char*>(vecs) + (size_t)i * D * es;
-      // (src/bexpect.f90:143-160) O v_i, then <v_f | O v_i> for every f
+      // (src/bexpect.f90:136-160) O v_i, then <v_f | O v_i> for every f
       if (dtype == BS_F32) {
-        launch_apply<float>(&o, (const float*)vi, (float*)hv.p, 0, nullptr);
+        launch_apply<float>(op, (const float*)vi, (float*)hv.p, 0, nullptr);
         k_multidot<float><<<nb, kBlock>>>((const float*)vecs, D, nvec, (const float*)hv.p, D, part.p);
       } else {
-        launch_apply<double>(&o, (const double*)vi, (double*)hv.p, 0, nullptr);
+        launch_apply<double>(op, (const double*)vi, (double*)hv.p, 0, nullptr);
         k_multidot<double><<<nb, kBlock>>>((const double*)vecs, D, nvec, (const double*)hv.p, D, part.p);
       }
       k_reduce<<<nvec, kBlock>>>(part.p, nb, nvec, res.p, 0);
@@ -2896,7 +2902,7 @@ int bs_gradient(bs_system* s, const void* psi, int dtype, double* grad) {
     if (rc) return rc;
     if (!(n2 > 0.0)) return set_err("zero vector");
     if (!s->jac_ready) {
-      s->jac = bs::param_jacobian(t, s->sps_path);
+      s->jac = bs::param_jacobian(t);
       s->jac_ready = true;
     }
     const int norb = (int)t.in.orbits.size();
@@ -3001,8 +3007,9 @@ int bs_batch_init(const char* sps, const char* const* intfiles, int K, int Z, in
     b->K = K;
     for (int k = 0; k < K; k++) b->intfiles.push_back(intfiles[k]);
     b->base.reset(new bs_system());
-    b->base->t = bs::build_tables_structural(sps, intfiles[0], Z, N, jz2, spscale, a0, a, x);
-    b->base->sps_path = sps;
+    bs::ModelInput in;
+    bs::read_sps(sps, in);
+    b->base->t = bs::build_tables_structural(in.orbits, intfiles[0], Z, N, jz2, spscale, a0, a, x);
     upload_tables(b->base.get());
     batch_upload_values(b.get());
     HIP_OK(hipDeviceSynchronize());
@@ -3113,6 +3120,104 @@ int bs_write_wfn(const bs_system* s, const char* path, int nkeep, const float* e
       ok = ok && std::fwrite(v32.data(), 4, (size_t)D, f) == (size_t)D;
     }
     if (!ok) return set_err(std::string("write failed: ") + path);
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_init_wfn_host(const char* wfn, const char* intf, float spscale, float a0, float a, float x,
+                     bs_system** out) {
+  try {
+    if (!wfn || !intf || !out) return set_err("null argument");
+    *out = nullptr;
+    const bs::WfnInfo w = bs::read_wfn_info(wfn);
+    std::unique_ptr<bs_system> s(new bs_system());
+    s->t = bs::build_tables(w.orbits, intf, w.Z, w.N, w.jz2, spscale, a0, a, x, true);
+    // the rebuilt basis must be the one the vectors are stored in
+    if (bs::wfn_header(s->t) != w.header)
+      return set_err(std::string("basis built from ") + wfn +
+                     " differs from the one it records (parity, W truncation or orbit order)");
+    *out = s.release();
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_init_wfn(const char* wfn, const char* intf, float spscale, float a0, float a, float x, bs_system** out) {
+  int r = bs_init_wfn_host(wfn, intf, spscale, a0, a, x, out);
+  if (r) return r;
+  r = bs_upload(*out);
+  if (r) {
+    delete *out;
+    *out = nullptr;
+  }
+  return r;
+}
+
+int bs_read_wfn(const bs_system* s, const char* path, int first, int count, float* energy, float* xj, float* xt2,
+                void* vecs, int dtype, int on_device, int* nkeep) {
+  try {
+    if (!s || !path) return set_err("null argument");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    const bs::WfnInfo w = bs::read_wfn_info(path);
+    if (w.header != bs::wfn_header(s->t))
+      return set_err(std::string("the basis of ") + path + " is not this system's basis");
+    if (nkeep) *nkeep = w.nkeep;
+    if (count == 0) return 0;
+    if (first < 0 || count < 0 || (int64_t)first + count > w.nkeep)
+      return set_err("states " + std::to_string(first + 1) + ".." + std::to_string(first + count) +
+                     " requested, file holds " + std::to_string(w.nkeep));
+    if (!vecs || !energy || !xj || !xt2) return set_err("null output");
+    const int64_t D = w.dim;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return set_err(std::string("cannot open ") + path);
+    std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    std::vector<unsigned char> raw((size_t)D * w.elem);
+    std::vector<float> v32(dtype == BS_F32 ? D : 0);
+    std::vector<double> v64(dtype == BS_F64 ? D : 0);
+    const int64_t reclen = 20 + (int64_t)w.elem * D;
+    for (int q = 0; q < count; q++) {
+      const int k = first + q;
+      // (src/bwfnlib.f90:926-993 position, :159-273 record and its checks)
+      if (fseeko(f, (off_t)(w.offset + (int64_t)k * reclen), SEEK_SET) != 0) return set_err("seek failed");
+      int32_t head[2];
+      float ejt[3];
+      if (std::fread(head, 4, 2, f) != 2 || std::fread(ejt, 4, 3, f) != 3 ||
+          std::fread(raw.data(), 1, raw.size(), f) != raw.size())
+        return set_err(std::string("read failed: ") + path);
+      if (head[0] != 27182818) return set_err("vector " + std::to_string(k + 1) + " is missing magic number");
+      if (head[1] != k + 1) return set_err("read wrong vector: wanted " + std::to_string(k + 1) + ", found " +
+                                            std::to_string(head[1]));
+      energy[q] = ejt[0];
+      xj[q] = ejt[1];
+      xt2[q] = ejt[2];
+      const size_t es = dtype == BS_F32 ? 4 : 8;
+      void* host;
+      if (dtype == BS_F32) {
+        if (w.elem == 4) std::memcpy(v32.data(), raw.data(), raw.size());
+        else
+          for (int64_t i = 0; i < D; i++) {
+            double d;
+            std::memcpy(&d, &raw[8 * i], 8);
+            v32[i] = (float)d;
+          }
+        host = v32.data();
+      } else {
+        if (w.elem == 8) std::memcpy(v64.data(), raw.data(), raw.size());
+        else
+          for (int64_t i = 0; i < D; i++) {
+            float g;
+            std::memcpy(&g, &raw[4 * i], 4);
+            v64[i] = g;
+          }
+        host = v64.data();
+      }
+      char* dst = static_cast<char*>(vecs) + (size_t)q * D * es;
+      if (on_device) HIP_OK(hipMemcpy(dst, host, es * (size_t)D, hipMemcpyHostToDevice));
+      else std::memcpy(dst, host, es * (size_t)D);
+    }
     return 0;
   } catch (const std::exception& e) {
     return set_err(e.what());

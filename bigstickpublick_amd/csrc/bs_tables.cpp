@@ -800,23 +800,23 @@ struct Builder {
 }  // namespace
 
 namespace {
-Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
-                 int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural = false);
+Tables build_any(const std::vector<Orbit>& orbits, const std::string* intfile, const std::string* inttext, int Z,
+                 int N, int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural = false);
 }
 
-Tables build_tables_structural(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
-                               float spscale, float a0, float a, float x) {
-  return build_any(sps, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, true, true);
+Tables build_tables_structural(const std::vector<Orbit>& orbits, const std::string& intfile, int Z, int N,
+                               int jz2, float spscale, float a0, float a, float x) {
+  return build_any(orbits, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, true, true);
 }
 
-Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
+Tables build_tables(const std::vector<Orbit>& orbits, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps) {
-  return build_any(sps, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, with_jumps);
+  return build_any(orbits, &intfile, nullptr, Z, N, jz2, spscale, a0, a, x, with_jumps);
 }
 
-Tables build_tables_text(const std::string& sps, const std::string& inttext, int Z, int N, int jz2,
+Tables build_tables_text(const std::vector<Orbit>& orbits, const std::string& inttext, int Z, int N, int jz2,
                          bool with_jumps) {
-  return build_any(sps, nullptr, &inttext, Z, N, jz2, 1.0f, 1.0f, 1.0f, 1.0f, with_jumps);
+  return build_any(orbits, nullptr, &inttext, Z, N, jz2, 1.0f, 1.0f, 1.0f, 1.0f, with_jumps);
 }
 
 // J^2 = sum_i j_i^2 + sum_{i<j} (J_ij^2 - j_i^2 - j_j^2) and the same for T^2,
@@ -851,11 +851,14 @@ std::string observable_int_text(const std::vector<Orbit>& orbits, int which) {
 }
 
 namespace {
-Tables build_any(const std::string& sps, const std::string* intfile, const std::string* inttext, int Z, int N,
-                 int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural) {
+Tables build_any(const std::vector<Orbit>& orbits, const std::string* intfile, const std::string* inttext, int Z,
+                 int N, int jz2, float spscale, float a0, float a, float x, bool with_jumps, bool structural) {
   Tables t;
   t.keep_structural = structural;
-  read_sps(sps, t.in);
+  if (orbits.empty() || (int)orbits.size() > kMaxOrb) fail("bad number of orbits");
+  for (const Orbit& o : orbits)
+    if (o.j2 < 1 || o.j2 % 2 == 0 || o.l < 0 || o.n < 0) fail("bad orbit quantum numbers");
+  t.in.orbits = orbits;
   for (auto& o : t.in.orbits) {
     if (o.w != t.in.orbits[0].w) fail("W truncation / mixed-W spaces are not supported");
     if (o.l % 2 != t.in.orbits[0].l % 2) fail("mixed-parity spaces are not supported");
@@ -981,7 +984,7 @@ std::vector<double> merged_values(const Tables& t, int it, const std::vector<flo
   return out;
 }
 
-std::vector<ParamColumn> param_jacobian(const Tables& t0, const std::string& sps) {
+std::vector<ParamColumn> param_jacobian(const Tables& t0) {
   const int norb = (int)t0.in.orbits.size();
   const int np = norb + (int)t0.lines.size();
   std::vector<ParamColumn> cols(np);
@@ -997,7 +1000,6 @@ std::vector<ParamColumn> param_jacobian(const Tables& t0, const std::string& sps
     }
     Tables t;
     t.in = t0.in;
-    (void)sps;
     Builder b(t);
     std::istringstream is(os.str());
     b.read_int_stream(is);
@@ -1122,6 +1124,67 @@ std::string wfn_header(const Tables& t) {
   const int32_t off = (int32_t)h.size() + 4;
   std::memcpy(&h[12], &off, 4);
   return h;
+}
+
+WfnInfo read_wfn_info(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail("cannot open .wfn file " + path);
+  // the header of the largest space this engine builds (16 orbits, 64
+  // states per species) is under 5 KB
+  std::string b(1 << 16, '\0');
+  f.read(&b[0], (std::streamsize)b.size());
+  b.resize((size_t)f.gcount());
+  size_t pos = 0;
+  auto i4 = [&]() {
+    if (pos + 4 > b.size()) fail(".wfn: truncated header: " + path);
+    int32_t v;
+    std::memcpy(&v, &b[pos], 4);
+    pos += 4;
+    return v;
+  };
+  WfnInfo w;
+  if (i4() != 31415926) fail(".wfn: bad magic number (not a BIGSTICK .wfn file): " + path);
+  if (i4() != 1) fail(".wfn: unsupported file version: " + path);
+  i4();
+  w.offset = i4();
+  for (int i = 0; i < 4; i++) i4();
+  w.Z = i4();
+  w.N = i4();
+  if (i4() != 1) fail(".wfn: only isospin-format (isoflag 1) files are supported: " + path);
+  const int no1 = i4(), no2 = i4();
+  if (no1 != no2 || no1 < 1 || no1 > kMaxOrb) fail(".wfn: bad orbit counts: " + path);
+  for (int it = 0; it < 2; it++)
+    for (int i = 0; i < no1; i++) {
+      Orbit o;
+      o.n = i4();
+      o.j2 = i4();
+      o.l = i4();
+      i4();  // parity flag
+      o.w = i4();
+      if (it == 0) w.orbits.push_back(o);
+    }
+  const int ns1 = i4(), ns2 = i4();
+  if (ns1 < 1 || ns1 > kMaxSps || ns2 < 1 || ns2 > kMaxSps) fail(".wfn: bad s.p. state counts: " + path);
+  for (int i = 0; i < 8 * (ns1 + ns2); i++) i4();
+  w.jz2 = i4();
+  for (int i = 0; i < 5; i++) i4();  // parity character, maxWtot, same-parity, same-W, spinless
+  if (pos + 8 > b.size()) fail(".wfn: truncated header: " + path);
+  std::memcpy(&w.dim, &b[pos], 8);
+  pos += 8;
+  w.header = b.substr(0, pos);
+  if (w.offset != (int64_t)pos + 4) fail(".wfn: vector offset does not follow the header: " + path);
+  w.nkeep = i4();
+  if (w.nkeep < 0 || w.dim < 1) fail(".wfn: bad dimension or vector count: " + path);
+  f.clear();
+  f.seekg(0, std::ios::end);
+  const int64_t body = (int64_t)f.tellg() - w.offset;
+  if (w.nkeep > 0) {
+    if (body == (int64_t)w.nkeep * (20 + 4 * w.dim)) w.elem = 4;
+    else if (body == (int64_t)w.nkeep * (20 + 8 * w.dim)) w.elem = 8;
+    else fail(".wfn: file size does not match " + std::to_string(w.nkeep) + " vectors of dimension " +
+              std::to_string(w.dim) + ": " + path);
+  }
+  return w;
 }
 
 }  // namespace bs

@@ -124,7 +124,9 @@ struct Tables {
 
 // Errors are reported by throwing std::runtime_error with a message.
 void read_sps(const std::string& path, ModelInput& in);
-Tables build_tables(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
+// The builders take the orbit list of an .sps file (read_sps) or of a .wfn
+// header (read_wfn_header).
+Tables build_tables(const std::vector<Orbit>& orbits, const std::string& intfile, int Z, int N, int jz2,
                     float spscale, float a0, float a, float x, bool with_jumps = true);
 // Element tables of another interaction on the same orbits, particle numbers
 // and scaling (no basis, no jumps), with the pn elements in the Delta-class
@@ -138,17 +140,17 @@ MeTables build_me_tables(const Tables& base, const std::string& intfile);
 // Merged two-body values of element table v (dense pair-pair) in the entry
 // order of base.tb[it]; base must have been built with keep_structural.
 std::vector<double> merged_values(const Tables& base, int it, const std::vector<float>& v);
-Tables build_tables_structural(const std::string& sps, const std::string& intfile, int Z, int N, int jz2,
-                               float spscale, float a0, float a, float x);
+Tables build_tables_structural(const std::vector<Orbit>& orbits, const std::string& intfile, int Z, int N,
+                               int jz2, float spscale, float a0, float a, float x);
 
 // Jacobian of the FP32 uncoupled elements with respect to the .int parameters
 // (SPEs first, then TBME lines in file order): built from unit interactions
 // through the same reading, subsumption and uncoupling steps.
-std::vector<ParamColumn> param_jacobian(const Tables& t, const std::string& sps);
+std::vector<ParamColumn> param_jacobian(const Tables& t);
 
 // Same as build_tables with the interaction given as the text of an iso .int
 // file (scaling 1 1 1 1).
-Tables build_tables_text(const std::string& sps, const std::string& inttext, int Z, int N, int jz2,
+Tables build_tables_text(const std::vector<Orbit>& orbits, const std::string& inttext, int Z, int N, int jz2,
                          bool with_jumps = true);
 // Iso interaction text of J^2 (which = 0) or T^2 (which = 1) on these orbits.
 std::string observable_int_text(const std::vector<Orbit>& orbits, int which);
@@ -157,6 +159,18 @@ void build_twobody_ops(const Tables& t, int it, TwoBodyOps& out);
 // including the basis dimension (src/bwfnlib.f90:373-541); the vectors start
 // 4 bytes after it (the count of vectors sits in between).
 std::string wfn_header(const Tables& t);
+// Basis recorded in a .wfn header (the fields read by read_wfn_header,
+// src/bwfnlib.f90:678-904): orbits of species 1, Z, N, 2Jz, dimension, the
+// header bytes up to the dimension, the vector offset and the vector count.
+// Vectors are stored in the writer's Lanczos precision (4 or 8 bytes per
+// amplitude, src/bwfnlib.f90:963-968), found from the file size.
+struct WfnInfo {
+  int Z = 0, N = 0, jz2 = 0, nkeep = 0, elem = 4;
+  std::vector<Orbit> orbits;
+  int64_t dim = 0, offset = 0;
+  std::string header;
+};
+WfnInfo read_wfn_info(const std::string& path);
 float cleb(int j1, int m1, int j2, int m2, int j3, int m3);
 
 }  // namespace bs

@@ -61,6 +61,9 @@ def load_library():
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
     L.bs_write_wfn.argtypes = [vp, ctypes.c_char_p, i32, vp, vp, vp, vp, i32, i32]
     L.bs_operator_matrix.argtypes = [vp, ctypes.c_char_p, f32, f32, f32, f32, vp, i32, i32, vp]
+    for name in ("bs_init_wfn_host", "bs_init_wfn"):
+        getattr(L, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, f32, f32, f32, f32, ctypes.POINTER(vp)]
+    L.bs_read_wfn.argtypes = [vp, ctypes.c_char_p, i32, i32, vp, vp, vp, vp, i32, i32, ctypes.POINTER(i32)]
     L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
     L.bs_batch_set_variant.argtypes = [vp, i32, i32]
     L.bs_multi_info.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), vp]
@@ -101,6 +104,23 @@ class System:
         h = ctypes.c_void_p()
         _check(L.bs_init_host(str(sps).encode(), str(intfile).encode(), Z, N, jz2,
                               *[float(v) for v in scaling], ctypes.byref(h)))
+        self._adopt(h, upload)
+
+    @classmethod
+    def from_wfn(cls, wfn, intfile, scaling=(1.0, 1.0, 1.0, 1.0), upload=False):
+        """The basis recorded in a .wfn file (orbits, Z, N, 2Jz) with the
+        interaction `intfile`; fails unless the rebuilt basis reproduces the
+        file's header."""
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.bs_init_wfn_host(str(wfn).encode(), str(intfile).encode(), *[float(v) for v in scaling],
+                                  ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._adopt(h, upload)
+        return self
+
+    def _adopt(self, h, upload):
+        L = load_library()
         self.h = h
         d = ctypes.c_int64()
         _check(L.bs_dim(h, ctypes.byref(d)))
@@ -284,16 +304,49 @@ class System:
         return load_library().bs_multi_variant(self.h, dt, K, 1, v) == 0
 
     def operator_matrix(self, intfile, vecs, scaling=(1.0, 1.0, 1.0, 1.0)):
-        """<v_f|O|v_i> for the two-body operator in `intfile` (iso .int) and an
-        (nvec, dim) CUDA tensor of vectors; returns an (nvec, nvec) array [i, f]."""
+        """<v_f|O|v_i> for the two-body operator in `intfile` (iso .int; None:
+        this system's Hamiltonian) and an (nvec, dim) CUDA tensor of vectors;
+        returns an (nvec, nvec) array [i, f]."""
         self.upload()
         v = vecs.contiguous()
         n = v.shape[0]
         out = np.zeros((n, n))
         dt = BS_F32 if v.dtype.itemsize == 4 else BS_F64
-        _check(load_library().bs_operator_matrix(self.h, str(intfile).encode(), *[float(s) for s in scaling],
+        name = None if intfile is None else str(intfile).encode()
+        _check(load_library().bs_operator_matrix(self.h, name, *[float(s) for s in scaling],
                                                  ctypes.c_void_p(v.data_ptr()), n, dt, out.ctypes.data))
         return out
+
+    def wfn_count(self, path):
+        """Number of states in a .wfn file on this basis."""
+        n = ctypes.c_int()
+        _check(load_library().bs_read_wfn(self.h, str(path).encode(), 0, 0, None, None, None, None, BS_F32, 0,
+                                          ctypes.byref(n)))
+        return n.value
+
+    def read_wfn(self, path, first=0, count=None, dtype="f32", device=False):
+        """States first..first+count-1 of a .wfn file on this basis: energy,
+        J, T(T+1) (float32 arrays) and the vectors, an (count, dim) numpy
+        array, or a CUDA tensor when device=True."""
+        n = self.wfn_count(path)
+        if count is None:
+            count = n - first
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        e, j, t = (np.zeros(count, np.float32) for _ in range(3))
+        if device:
+            import torch
+            self.upload()
+            v = torch.empty((count, self.dim), dtype=torch.float32 if dt == BS_F32 else torch.float64,
+                            device="cuda")
+            ptr = v.data_ptr()
+        else:
+            v = np.zeros((count, self.dim), np.float32 if dt == BS_F32 else np.float64)
+            ptr = v.ctypes.data
+        nk = ctypes.c_int()
+        _check(load_library().bs_read_wfn(self.h, str(path).encode(), first, count, e.ctypes.data, j.ctypes.data,
+                                          t.ctypes.data, ctypes.c_void_p(ptr), dt, 1 if device else 0,
+                                          ctypes.byref(nk)))
+        return e, j, t, v
 
     def write_wfn(self, path, energy, xj, xt2, vecs):
         """Write the reference's binary .wfn: energies, J, T(T+1) (length

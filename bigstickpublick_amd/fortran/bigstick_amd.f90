@@ -120,6 +120,34 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_write_wfn
 
+    function bs_init_wfn(wfn, intf, spscale, a0, a, x, sys) bind(C, name='bs_init_wfn') result(rc)
+      import :: c_char, c_int, c_float, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: wfn, intf
+      real(c_float), value :: spscale, a0, a, x
+      type(c_ptr), intent(out) :: sys
+      integer(c_int) :: rc
+    end function bs_init_wfn
+
+    function bs_read_wfn(sys, path, first, count, energy, xj, xt2, vecs, dtype, on_device, nkeep) &
+        bind(C, name='bs_read_wfn') result(rc)
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: sys, energy, xj, xt2, vecs
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: first, count, dtype, on_device
+      integer(c_int), intent(out) :: nkeep
+      integer(c_int) :: rc
+    end function bs_read_wfn
+
+    ! intf = c_null_ptr applies the system's own Hamiltonian
+    function bs_operator_matrix(sys, intf, spscale, a0, a, x, vecs, nvec, dtype, out) &
+        bind(C, name='bs_operator_matrix') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: sys, intf, vecs, out
+      real(c_float), value :: spscale, a0, a, x
+      integer(c_int), value :: nvec, dtype
+      integer(c_int) :: rc
+    end function bs_operator_matrix
+
     ! HIP runtime, for device buffers owned by the Fortran host
     function hipMalloc(ptr, bytes) bind(C, name='hipMalloc') result(rc)
       import :: c_ptr, c_size_t, c_int
@@ -127,6 +155,15 @@ module bigstick_amd
       integer(c_size_t), value :: bytes
       integer(c_int) :: rc
     end function hipMalloc
+
+    ! kind 1 = host to device, 2 = device to host
+    function hipMemcpy(dst, src, bytes, kind) bind(C, name='hipMemcpy') result(rc)
+      import :: c_ptr, c_size_t, c_int
+      type(c_ptr), value :: dst, src
+      integer(c_size_t), value :: bytes
+      integer(c_int), value :: kind
+      integer(c_int) :: rc
+    end function hipMemcpy
 
     function hipFree(ptr) bind(C, name='hipFree') result(rc)
       import :: c_ptr, c_int

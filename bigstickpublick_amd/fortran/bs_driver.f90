@@ -1,5 +1,6 @@
 ! bs_driver: a Fortran host for the MI355X engine that reads BIGSTICK's own
 ! stdin menu answers for a normal run (menu option n, src/bmenu.f90:11;
+! menu x, expectation values from a previous .wfn, below;
 ! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td,
 ! src/blanczosmain.f90:380-745) and writes <name>.res (energy table in the
 ! format of src/bparticleocc.f90:256) and <name>.lcoef (alpha, beta per
@@ -29,8 +30,12 @@ program bs_driver
 
   read(5, '(a)') opt
   opt = adjustl(opt)
+  if (opt(1:1) == 'x' .or. opt(1:1) == 'X') then
+    call expectation_run()
+    stop
+  end if
   if (opt(1:1) /= 'n' .and. opt(1:1) /= 'N') then
-    write(6, *) ' bs_driver: only the normal diagonalization run (menu n / ns) is supported'
+    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization) and x (expectation values)'
     stop 1
   end if
   ! n writes <name>.wfn, ns suppresses it (src/bmenu.f90:680-687)
@@ -141,6 +146,77 @@ program bs_driver
   rc = bs_finalize(sys)
 
 contains
+  ! Menu x: expectation value of a scalar two-body operator (an .int file)
+  ! in each state of a previous .wfn (src/bmenu.f90:520-526,
+  ! src/bexpect.f90:16-74).  stdin: x, .wfn name, output name, .int name,
+  ! scaling line, end.  Each state is normalized first (dnormvec_p,
+  ! src/bvectorlib.f90:200-264); <name>.res gets the reference's table.
+  subroutine expectation_run()
+    character(len=256) :: wfnname, oname, iname, l
+    character(len=:), allocatable :: wpath
+    real(c_float) :: s1, s2, s3, s4
+    type(c_ptr) :: sy, dv
+    integer(c_int64_t) :: d
+    integer(c_int) :: r, nk, nk2
+    real(c_float), allocatable, target :: e(:), ej(:), et(:)
+    real(c_double), allocatable, target :: hv(:, :), hme(:, :)
+    real(c_double), allocatable :: dn(:)
+    integer :: i, u
+
+    read(5, '(a)') wfnname
+    read(5, '(a)') oname
+    read(5, '(a)') iname
+    read(5, *) s1, s2, s3, s4
+    read(5, '(a)') l
+    if (adjustl(l) /= 'end') then
+      write(6, *) ' bs_driver: one interaction file is supported'
+      stop 1
+    end if
+    wpath = trim(first_word(wfnname)) // '.wfn'
+    r = bs_init_wfn(cstr(wpath), cstr(trim(first_word(iname)) // '.int'), s1, s2, s3, s4, sy)
+    call bs_check(r, 'bs_init_wfn')
+    r = bs_dim(sy, d)
+    r = bs_read_wfn(sy, cstr(wpath), 0, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, BS_F64, 0, nk)
+    call bs_check(r, 'bs_read_wfn')
+    if (nk < 1) then
+      write(6, *) ' bs_driver: no states in ', trim(wpath)
+      stop 1
+    end if
+    allocate(e(nk), ej(nk), et(nk), hv(d, nk), hme(nk, nk), dn(nk))
+    r = bs_read_wfn(sy, cstr(wpath), 0, nk, c_loc(e), c_loc(ej), c_loc(et), c_loc(hv), BS_F64, 0, nk2)
+    call bs_check(r, 'bs_read_wfn')
+    do i = 1, nk
+      dn(i) = sqrt(sum(hv(:, i)**2))
+      if (dn(i) > 0.0d0) hv(:, i) = hv(:, i) / dn(i)
+    end do
+    r = hipMalloc(dv, int(8 * d * nk, c_size_t))
+    if (r /= 0) then
+      write(6, *) ' hipMalloc failed ', r
+      stop 1
+    end if
+    r = hipMemcpy(dv, c_loc(hv), int(8 * d * nk, c_size_t), 1_c_int)
+    if (r /= 0) then
+      write(6, *) ' hipMemcpy failed ', r
+      stop 1
+    end if
+    ! hme(f, i) = <v_f | O | v_i>; the diagonal is the expectation value
+    r = bs_operator_matrix(sy, c_null_ptr, 1.0, 1.0, 1.0, 1.0, dv, nk, BS_F64, c_loc(hme))
+    call bs_check(r, 'bs_operator_matrix')
+    open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
+    write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(u, '(a,a)') '  states read from ', wpath
+    write(u, '(''nkeep ='',1x,i10)') nk
+    write(u, '(a)') '   STATE      E            J           T^2           <H >      (norm)'
+    do i = 1, nk
+      write(u, 101) i, e(i), ej(i), et(i), hme(i, i), dn(i)
+    end do
+101 format(i5,2x,2(1x,f11.4),2(1x,f13.6),1x,f10.5)
+    close(u)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', nk, ' states'
+    r = hipFree(dv)
+    r = bs_finalize(sy)
+  end subroutine expectation_run
+
   ! the reference's menu lines may carry a trailing '! comment'
   function first_word(s) result(w)
     character(len=*), intent(in) :: s

@@ -157,10 +157,13 @@ int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double
                double *evals, void *evecs, double *resid, int *iters);
 
 /* Matrix <v_f | O | v_i> of a scalar two-body operator O, given as an iso .int
- * file with its scaling line, between nvec vectors (device, dim elements
- * each, consecutive) on this system's basis: out[i*nvec + f].  Replaces
- * expectator_p (src/bexpect.f90:143-160), which applies the operator with
- * applyHbundled_g to each stored state and overlaps it with the others. */
+ * file with its scaling line (intf = NULL: the system's own Hamiltonian),
+ * between nvec vectors (device, dim elements each, consecutive) on this
+ * system's basis: out[i*nvec + f].  Replaces the hme table of
+ * hmatrixelements_boss (menu h, src/bexpect.f90:84-180), which applies the
+ * operator with applyHbundled_g to each stored state and overlaps it with
+ * the others; its diagonal on normalized states is the <H> column of
+ * expectator_p (menu x, src/bexpect.f90:16-74). */
 int bs_operator_matrix(bs_system *sys, const char *intf, float spscale, float a0, float a, float x,
                        const void *vecs, int nvec, int dtype, double *out);
 
@@ -172,6 +175,31 @@ int bs_operator_matrix(bs_system *sys, const char *intf, float spscale, float a0
  * write_wfn_header for the single-file, one-fragment case. */
 int bs_write_wfn(const bs_system *sys, const char *path, int nkeep, const float *energy, const float *xj,
                  const float *xt2, const void *vecs, int dtype, int on_device);
+
+/* Build and upload a system on the basis recorded in a .wfn file (orbits, Z,
+ * N, 2Jz from its header) with the interaction intf and its scaling line.
+ * Fails when the basis rebuilt from those fields does not reproduce the
+ * file's header byte for byte (a parity, W or ordering this engine does not
+ * build).  Replaces wfn_ropen_file + read_wfn_header (src/bwfnlib.f90:1619-
+ * 1669, :678-904), the basis set-up of menu options x and h
+ * (src/bmenu.f90:407-421, :520-526). */
+int bs_init_wfn(const char *wfn, const char *intf, float spscale, float a0, float a, float x,
+                bs_system **sys);
+/* bs_init_wfn without the upload (host tables only; bs_upload completes it). */
+int bs_init_wfn_host(const char *wfn, const char *intf, float spscale, float a0, float a, float x,
+                     bs_system **sys);
+
+/* Read states first .. first+count-1 (0-based) of a .wfn file on this
+ * system's basis: energy, J, T(T+1) (count entries each) and the vectors
+ * (count*dim elements in dtype, on the device when on_device = 1, else on
+ * the host).  nkeep (may be NULL) receives the number of states in the file;
+ * count = 0 only reads it.  Vectors stored in single or double precision are
+ * both accepted.  Errors: a header that is not this basis, a state range
+ * beyond nkeep, a record without its magic number or with the wrong index.
+ * Replaces wfn_read_nkeep + wfn_readeigenvec (src/bwfnlib.f90:1052-1065,
+ * :159-273). */
+int bs_read_wfn(const bs_system *sys, const char *path, int first, int count, float *energy, float *xj,
+                float *xt2, void *vecs, int dtype, int on_device, int *nkeep);
 
 /* ---------------------------------------------------------------- block and batched
  * Interleaved multi-vector layout: element k of basis state i at [i*K + k],

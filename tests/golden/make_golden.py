@@ -232,6 +232,28 @@ def wfn(name, space, Z, N, jz2, seed):
         shutil.rmtree(wd)
 
 
+# expectation values (menu x) of a second seeded interaction in the states of
+# each committed .wfn: (wfn name, operator seed)
+EXPECT = [("sd2p2n", 5), ("pf2p1n_m3", 6)]
+
+
+def expect(name, op_seed):
+    """The reference's menu-x table (src/bexpect.f90:16-74) for the states of
+    the committed <name>.wfn and the seeded operator op<seed>.int, kept as
+    <name>_x.res."""
+    case = small_case(*[c for c in SMALL if c[0] == name][0][1:5], seed=op_seed)
+    case.name = "op%d" % op_seed
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        shutil.copy(os.path.join(HERE, name + ".wfn"), wd)
+        run_ref(wd, "x\n%s\n%s_x\n%s\n1 1 1 1\nend\n" % (name, name, case.name))
+        shutil.copy(os.path.join(wd, name + "_x.res"), os.path.join(HERE, name + "_x.res"))
+        print(name, "menu x with", case.name)
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -265,6 +287,9 @@ if __name__ == "__main__":
     for c in WFN:
         if "wfn_" + c[0] in want:
             wfn(*c)
+    for c in EXPECT:
+        if "x_" + c[0] in want:
+            expect(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -101,3 +101,61 @@ def test_fortran_host_wfn_matches_reference(name, space, Z, N, jz2, seed, casedi
     r = _run(menu_script(case, "ns", "lf"), wd)
     assert r.returncode == 0, r.stdout + r.stderr
     assert not os.path.exists(os.path.join(wd, name + ".wfn"))
+
+
+def read_x_table(path):
+    """Rows of the menu-x table (src/bexpect.f90:54-69): state, E, J, T(T+1),
+    <O>, norm, as printed."""
+    rows = []
+    lines = open(path, errors="replace").read().splitlines()
+    for i, l in enumerate(lines):
+        if l.strip().startswith("STATE"):
+            for l2 in lines[i + 1:]:
+                p = l2.split()
+                if len(p) != 6 or not p[0].isdigit():
+                    break
+                rows.append(p)
+            break
+    return rows
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,space,Z,N,jz2,op_seed", [("sd2p2n", "sd", 2, 2, 0, 5), ("pf2p1n_m3", "pf", 2, 1, 3, 6)])
+def test_fortran_host_menu_x_matches_reference(name, space, Z, N, jz2, op_seed, casedir):
+    """Menu x on the reference's own .wfn and a second seeded interaction:
+    the table equals the reference's <name>_x.res; E, J, T(T+1) and the norm
+    as printed, <O> within 2e-6 (printed to 1e-6; the reference evaluates it
+    with FP32 vectors)."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    op = small_case(space, Z, N, jz2, seed=op_seed)
+    op.name = "op%d" % op_seed
+    wd = os.path.join(casedir, "fx_" + name)
+    write_case(op, wd)
+    shutil.copy(os.path.join(gold, name + ".wfn"), wd)
+    r = _run("x\n%s\n%s_x\n%s\n1 1 1 1\nend\n" % (name, name, op.name), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_x_table(os.path.join(wd, name + "_x.res"))
+    ref = read_x_table(os.path.join(gold, name + "_x.res"))
+    assert len(got) == len(ref) == 5
+    for g, f in zip(got, ref):
+        assert g[:4] == f[:4] and g[5] == f[5], (g, f)
+        assert abs(float(g[4]) - float(f[4])) <= 2e-6, (g, f)
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_menu_x_refuses_foreign_basis(casedir):
+    """A .wfn whose basis the interaction's orbits cannot rebuild stops with a
+    message."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    op = small_case("pf", 2, 2, 0, seed=6)
+    op.name = "oppf"
+    wd = os.path.join(casedir, "fx_bad")
+    write_case(op, wd)
+    shutil.copy(os.path.join(gold, "sd2p2n.wfn"), wd)
+    r = _run("x\nsd2p2n\nbad_x\noppf\n1 1 1 1\nend\n", wd)
+    assert r.returncode != 0
+    assert "bs_init_wfn" in r.stdout + r.stderr

@@ -158,3 +158,31 @@ def test_operator_matrix(casedir):
     H = s.operator_matrix(paths(case, d)[1], r.evecs)
     assert np.abs(np.diag(H) - r.evals).max() < 1e-8
     assert np.abs(H - np.diag(np.diag(H))).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_states_from_reference_wfn(casedir):
+    """The reference's sd2p2n.wfn read onto the device through a system built
+    from the file: its own Hamiltonian (intf None) on those states is
+    diagonal with the stored energies (FP32 vectors, converged to 1e-4), and
+    a second operator's matrix equals the oracle's on the same vectors."""
+    import os
+    from bigstickpublick_amd.engine import System
+    from bigstickpublick_amd.inputs import small_case
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    wfn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sd2p2n.wfn")
+    w = System.from_wfn(wfn, paths(case, d)[1], upload=True)
+    e, j, t, v = w.read_wfn(wfn, dtype="f64", device=True)
+    H = w.operator_matrix(None, v)
+    vh = v.cpu().numpy()
+    nrm = np.einsum("ij,ij->i", vh, vh)
+    assert np.abs(np.diag(H) / nrm - e).max() < 1e-4
+    assert np.abs(H - np.diag(np.diag(H))).max() < 1e-4
+    op = small_case("sd", 2, 2, 0, seed=5).interaction
+    opf = os.path.join(d, "op5.int")
+    with open(opf, "w") as f:
+        f.write(op.int_text())
+    M = w.operator_matrix(opf, v)
+    o = Oracle(paths(case, d)[0], opf, case.Z, case.N, case.jz2)
+    want = np.array([[vh[f] @ o.apply(vh[i]) for f in range(len(e))] for i in range(len(e))])
+    assert _rel(M, want) < 1e-12
