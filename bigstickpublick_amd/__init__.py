@@ -2,4 +2,4 @@
 driver.  The compute lives in libbigstick_amd.so (HIP, gfx950) behind the C
 ABI declared in include/bigstick_amd.h; this module is a thin ctypes binding
 plus torch plumbing for device memory and streams."""
-from .engine import Batch, BatchResult, System, LanczosResult, load_library  # noqa: F401
+from .engine import Batch, BatchResult, LanczosResult, StrengthResult, System, load_library  # noqa: F401

@@ -1693,9 +1693,20 @@ void launch_apply(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st
 }
 
 // Lanczos workspace and loop
+// Strength-function extras: a pivot to start from (device, D elements of T;
+// NULL = the uniform pivot 1/sqrt(D)), and for every Ritz value of the final
+// projected matrix its value and |Y(0, k)|^2 ||pivot||^2.
+template <typename T>
+struct PivotRun {
+  const T* pivot = nullptr;
+  double* pivot_norm = nullptr;
+  double* ritz = nullptr;
+  double* strength = nullptr;
+};
+
 template <typename T>
 int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, double* beta_out,
-                double* evals, void* evecs, double* resid, int* iters) {
+                double* evals, void* evecs, double* resid, int* iters, const PivotRun<T>& pr = PivotRun<T>()) {
   const int64_t D = s->t.dim;
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
@@ -1719,8 +1730,19 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   double* d_alpha = ab.p;
   double* d_beta = ab.p + mtot;
   const int g = grid_for(D);
-  k_fill<T><<<g, kBlock, 0, st>>>(V.p, (T)(1.0 / std::sqrt((double)D)), D);
+  double pnorm = 1.0;
+  if (pr.pivot) {
+    // (src/blanczosmain.f90:1560-1563) the pivot normalized, its norm kept
+    k_multidot<T><<<nb, kBlock, 0, st>>>(pr.pivot, D, 1, pr.pivot, D, part.p);
+    k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, dots.p, 1);
+    HIP_OK(hipMemcpy(&pnorm, dots.p, sizeof(double), hipMemcpyDeviceToHost));
+    if (!(pnorm > 0.0) || !std::isfinite(pnorm)) return set_err("pivot vector is zero or not finite");
+    k_normalize<T><<<g, kBlock, 0, st>>>(V.p, pr.pivot, dots.p, D);
+  } else {
+    k_fill<T><<<g, kBlock, 0, st>>>(V.p, (T)(1.0 / std::sqrt((double)D)), D);
+  }
   HIP_OK(hipGetLastError());
+  if (pr.pivot_norm) *pr.pivot_norm = pnorm;
 
   std::vector<double> alpha_h, beta_h;  // current arrowhead/tridiagonal data
   std::vector<double> e_prev;
@@ -1809,6 +1831,11 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     restarts++;
   }
   for (int k = 0; k < nkeep; k++) evals[k] = e[k];
+  // (src/blanczoslib.f90:1122-1136) E_k and eiglvec(1,k)^2 dnorm0^2
+  for (int k = 0; k < niter; k++) {
+    if (pr.ritz) pr.ritz[k] = e[k];
+    if (pr.strength) pr.strength[k] = Y[k] * Y[k] * pnorm * pnorm;
+  }
   if (alpha_out) for (int i = 0; i < niter; i++) alpha_out[i] = alpha_h[i];
   if (beta_out) for (int i = 0; i < niter; i++) beta_out[i] = beta_h[i];
   if (evecs || resid || p->polish) {
@@ -3228,6 +3255,29 @@ int bs_autotune_times(const bs_system* s, int dtype, float* ms) {
   if (!s || !ms || (dtype != BS_F32 && dtype != BS_F64)) return set_err("bad argument");
   for (int v = 0; v < kNumVariants; v++) ms[v] = s->tune_ms[dtype == BS_F32 ? 0 : 1][v];
   return 0;
+}
+
+int bs_strength(bs_system* s, const bs_lanczos_params* p, const void* pivot, double* pivot_norm, double* ritz,
+                double* strength, double* alpha, double* beta, double* evals, void* evecs, double* resid,
+                int* iters) {
+  try {
+    checked(s);
+    if (!p || !evals || !pivot) return set_err("null argument");
+    if (p->thick != 0) return set_err("strength functions need a fixed run (thick = 0)");
+    if (p->dtype != BS_F32 && p->dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (p->dtype == BS_F32) {
+      PivotRun<float> pr{(const float*)pivot, pivot_norm, ritz, strength};
+      return run_lanczos<float>(s, p, alpha, beta, evals, evecs, resid, iters, pr);
+    }
+    if (p->dtype == BS_F64) {
+      PivotRun<double> pr{(const double*)pivot, pivot_norm, ritz, strength};
+      return run_lanczos<double>(s, p, alpha, beta, evals, evecs, resid, iters, pr);
+    }
+    return set_err("dtype must be BS_F32 or BS_F64");
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
 }
 
 int bs_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,

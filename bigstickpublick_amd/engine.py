@@ -58,6 +58,8 @@ def load_library():
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
+    L.bs_strength.argtypes = [vp, ctypes.POINTER(_Params), vp, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp,
+                              vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
     L.bs_write_wfn.argtypes = [vp, ctypes.c_char_p, i32, vp, vp, vp, vp, i32, i32]
     L.bs_operator_matrix.argtypes = [vp, ctypes.c_char_p, f32, f32, f32, f32, vp, i32, i32, vp]
@@ -88,6 +90,19 @@ def _check(rc):
 @dataclass
 class LanczosResult:
     evals: np.ndarray
+    alpha: np.ndarray
+    beta: np.ndarray
+    resid: np.ndarray
+    matvecs: int
+    evecs: object = None
+
+
+@dataclass
+class StrengthResult:
+    ritz: np.ndarray       # every Ritz value of the fixed run
+    strength: np.ndarray   # |Y(1,k)|^2 ||pivot||^2 for each
+    pivot_norm: float
+    evals: np.ndarray      # the nkeep lowest, polished as in lanczos()
     alpha: np.ndarray
     beta: np.ndarray
     resid: np.ndarray
@@ -295,6 +310,32 @@ class System:
                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
+
+    def strength(self, pivot, nkeep, niter, polish=True, want_vectors=False):
+        """Strength function: a fixed Lanczos run of niter steps started from
+        `pivot` (a CUDA float32/float64 tensor of dim elements; its dtype is
+        the run's vector precision)."""
+        import torch
+        self.upload()
+        pv = pivot.contiguous().reshape(-1)
+        if pv.numel() != self.dim or pv.dtype not in (torch.float32, torch.float64):
+            raise ValueError("pivot must be a float32/float64 CUDA tensor of dim elements")
+        dt = BS_F32 if pv.dtype == torch.float32 else BS_F64
+        p = _Params(nkeep, niter, 0, 0, dt, 0.0, int(polish))
+        m = min(niter, self.dim)
+        ritz, stren, alpha, beta = (np.zeros(m) for _ in range(4))
+        ev = np.zeros(nkeep)
+        res = np.zeros(nkeep)
+        pn = ctypes.c_double()
+        it = ctypes.c_int()
+        vec = torch.empty((nkeep, self.dim), device="cuda", dtype=pv.dtype) if want_vectors else None
+        torch.cuda.synchronize()
+        _check(load_library().bs_strength(self.h, ctypes.byref(p), ctypes.c_void_p(pv.data_ptr()), ctypes.byref(pn),
+                                          ritz.ctypes.data, stren.ctypes.data, alpha.ctypes.data, beta.ctypes.data,
+                                          ev.ctypes.data,
+                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
+                                          res.ctypes.data, ctypes.byref(it)))
+        return StrengthResult(ritz, stren, pn.value, ev, alpha, beta, res, it.value, vec)
 
     def set_block_variant(self, dtype, K, v):
         """Force multi-vector kernel variant v (0..4) for apply_block with K

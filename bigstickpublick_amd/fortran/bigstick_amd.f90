@@ -103,6 +103,16 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_lanczos
 
+    function bs_strength(sys, p, pivot, pivot_norm, ritz, strength, alpha, beta, evals, evecs, resid, iters) &
+        bind(C, name='bs_strength') result(rc)
+      import :: c_ptr, c_int, c_double, bs_lanczos_params
+      type(c_ptr), value :: sys, pivot, ritz, strength, alpha, beta, evals, evecs, resid
+      type(bs_lanczos_params), intent(in) :: p
+      real(c_double), intent(out) :: pivot_norm
+      integer(c_int), intent(out) :: iters
+      integer(c_int) :: rc
+    end function bs_strength
+
     function bs_observable(sys, which, psi, dtype, val) bind(C, name='bs_observable') result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: sys, psi

@@ -1,6 +1,7 @@
 ! bs_driver: a Fortran host for the MI355X engine that reads BIGSTICK's own
 ! stdin menu answers for a normal run (menu option n, src/bmenu.f90:11;
-! menu x, expectation values from a previous .wfn, below;
+! menu s, strength function from a pivot in a previous .wfn, and menu x,
+! expectation values in previous states, below;
 ! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td,
 ! src/blanczosmain.f90:380-745) and writes <name>.res (energy table in the
 ! format of src/bparticleocc.f90:256) and <name>.lcoef (alpha, beta per
@@ -14,138 +15,273 @@ program bs_driver
   use iso_c_binding
   use bigstick_amd
   implicit none
-  character(len=256) :: opt, outname, spsname, intname, line, lopt
-  integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu
-  real(c_float) :: spscale, a0, a, x
-  type(c_ptr) :: sys, dvec
-  integer(c_int64_t) :: dim, nbytes, esize
-  integer(c_int) :: rc, iters, dtype, variant
-  type(bs_lanczos_params) :: p
-  real(c_double), allocatable, target :: alpha(:), beta(:), evals(:), resid(:)
-  real(c_double) :: j2, t2
-  real :: xj, xt
-  real(c_float), allocatable, target :: we(:), wj(:), wt2(:)
-  logical :: write_wfn
-  character(len=16) :: env
+  character(len=256) :: opt
 
   read(5, '(a)') opt
   opt = adjustl(opt)
-  if (opt(1:1) == 'x' .or. opt(1:1) == 'X') then
+  ! a second letter s suppresses the .wfn (menus ns, ss: src/bmenu.f90:572-580, :680-687)
+  select case (opt(1:1))
+  case ('n', 'N')
+    call diagonalize_run(.not. (opt(2:2) == 's' .or. opt(2:2) == 'S'))
+  case ('s', 'S')
+    call strength_run(opt(2:2) == 's' .or. opt(2:2) == 'S')
+  case ('x', 'X')
     call expectation_run()
-    stop
-  end if
-  if (opt(1:1) /= 'n' .and. opt(1:1) /= 'N') then
-    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization) and x (expectation values)'
+  case default
+    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function)', &
+                ' and x (expectation values)'
     stop 1
-  end if
-  ! n writes <name>.wfn, ns suppresses it (src/bmenu.f90:680-687)
-  write_wfn = .not. (opt(2:2) == 's' .or. opt(2:2) == 'S')
-  read(5, '(a)') outname
-  read(5, '(a)') spsname
-  read(5, *) z, n
-  read(5, *) jz2
-  read(5, '(a)') intname
-  read(5, *) spscale, a0, a, x
-  read(5, '(a)') line
-  if (adjustl(line) /= 'end') then
-    write(6, *) ' bs_driver: one interaction file is supported'
-    stop 1
-  end if
-  call get_environment_variable('BS_HOST_ONLY', env)
-  if (trim(env) == '1') then
-    rc = bs_init_host(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
-                      z, n, jz2, spscale, a0, a, x, sys)
-    call bs_check(rc, 'bs_init_host')
-    rc = bs_dim(sys, dim)
-    write(6, '(a,i0)') ' dimension = ', dim
-    rc = bs_finalize(sys)
-    stop
-  end if
-  read(5, '(a)') lopt
-  lopt = adjustl(lopt)
-  read(5, *) nkeep, niter
-  nthick = 0
-  maxrest = 0
-  if (lopt(1:1) == 't') then
-    read(5, *) nthick
-    read(5, *, iostat=ios) maxrest
-    if (ios /= 0) maxrest = 20
-  end if
-
-  rc = bs_init(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
-               z, n, jz2, spscale, a0, a, x, sys)
-  call bs_check(rc, 'bs_init')
-  rc = bs_dim(sys, dim)
-  call get_environment_variable('BS_DTYPE', env)
-  dtype = BS_F32
-  if (trim(env) == 'f64') dtype = BS_F64
-  esize = 4
-  if (dtype == BS_F64) esize = 8
-  rc = bs_autotune(sys, dtype, variant)
-  call bs_check(rc, 'bs_autotune')
-
-  niter = int(min(int(niter, c_int64_t), dim))
-  nkeep = min(nkeep, niter)
-  allocate(alpha(niter), beta(niter), evals(nkeep), resid(nkeep))
-  p%nkeep = nkeep
-  p%niter = niter
-  p%thick = nthick
-  p%max_restarts = maxrest
-  p%dtype = dtype
-  p%tol = 0.0d0
-  if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c') p%tol = 1.0d-6
-  p%polish = 1
-  nbytes = int(nkeep, c_int64_t) * dim * esize
-  rc = hipMalloc(dvec, int(nbytes, c_size_t))
-  if (rc /= 0) then
-    write(6, *) ' hipMalloc failed ', rc
-    stop 1
-  end if
-  rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
-  call bs_check(rc, 'bs_lanczos')
-
-  open(newunit=iu, file=trim(first_word(outname)) // '.res', status='replace')
-  write(iu, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
-  write(iu, '(a,a)') '  single-particle file = ', trim(first_word(spsname))
-  write(iu, '(1x,i0,1x,i0)') z, n
-  write(iu, '(1x,i0,a)') jz2, ' +'
-  write(iu, '(1x,i0,a)') iters, '  iterations '
-  write(iu, '(a)') '  '
-  write(iu, '(a)') '  State      E        Ex         J       T      par'
-  allocate(we(nkeep), wj(nkeep), wt2(nkeep))
-  do k = 1, nkeep
-    rc = bs_observable(sys, 0, ptr_offset(dvec, int(k - 1, c_int64_t) * dim * esize), dtype, j2)
-    call bs_check(rc, 'bs_observable J^2')
-    rc = bs_observable(sys, 1, ptr_offset(dvec, int(k - 1, c_int64_t) * dim * esize), dtype, t2)
-    call bs_check(rc, 'bs_observable T^2')
-    xj = real(-0.5d0 + sqrt(0.25d0 + max(j2, 0.0d0)))
-    xt = real(-0.5d0 + sqrt(0.25d0 + max(t2, 0.0d0)))
-    write(iu, 100) k, evals(k), evals(k) - evals(1), xj, xt, 1
-    ! .wfn carries E, J and T(T+1) in single precision (src/blanczoslib.f90:1330-1350)
-    we(k) = real(evals(k), c_float)
-    wj(k) = xj
-    wt2(k) = real(t2, c_float)
-  end do
-100 format(i5,3x,2f10.5,2x,2f8.3,3x,i2)
-  close(iu)
-  if (write_wfn) then
-    rc = bs_write_wfn(sys, cstr(trim(first_word(outname)) // '.wfn'), nkeep, c_loc(we), c_loc(wj), c_loc(wt2), &
-                      dvec, dtype, 1_c_int)
-    call bs_check(rc, 'bs_write_wfn')
-  end if
-  open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
-  do k = 1, niter
-    write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
-  end do
-  close(iu)
-  write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', dim, ', ', iters, ' matvecs'
-  do k = 1, nkeep
-    write(6, '(i5,3x,f12.6,2x,es10.2)') k, evals(k), resid(k)
-  end do
-  rc = hipFree(dvec)
-  rc = bs_finalize(sys)
+  end select
 
 contains
+  ! Menu n / ns: Lanczos from the uniform pivot.
+  subroutine diagonalize_run(write_wfn)
+    logical, intent(in) :: write_wfn
+    character(len=256) :: outname, spsname, intname, line, lopt
+    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu
+    real(c_float) :: spscale, a0, a, x
+    type(c_ptr) :: sys, dvec
+    integer(c_int64_t) :: dim, nbytes, esize
+    integer(c_int) :: rc, iters, dtype, variant
+    type(bs_lanczos_params) :: p
+    real(c_double), allocatable, target :: alpha(:), beta(:), evals(:), resid(:)
+    character(len=16) :: env
+
+    read(5, '(a)') outname
+    read(5, '(a)') spsname
+    read(5, *) z, n
+    read(5, *) jz2
+    read(5, '(a)') intname
+    read(5, *) spscale, a0, a, x
+    read(5, '(a)') line
+    if (adjustl(line) /= 'end') then
+      write(6, *) ' bs_driver: one interaction file is supported'
+      stop 1
+    end if
+    call get_environment_variable('BS_HOST_ONLY', env)
+    if (trim(env) == '1') then
+      rc = bs_init_host(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
+                        z, n, jz2, spscale, a0, a, x, sys)
+      call bs_check(rc, 'bs_init_host')
+      rc = bs_dim(sys, dim)
+      write(6, '(a,i0)') ' dimension = ', dim
+      rc = bs_finalize(sys)
+      return
+    end if
+    read(5, '(a)') lopt
+    lopt = adjustl(lopt)
+    read(5, *) nkeep, niter
+    nthick = 0
+    maxrest = 0
+    if (lopt(1:1) == 't') then
+      read(5, *) nthick
+      read(5, *, iostat=ios) maxrest
+      if (ios /= 0) maxrest = 20
+    end if
+
+    rc = bs_init(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
+                 z, n, jz2, spscale, a0, a, x, sys)
+    call bs_check(rc, 'bs_init')
+    rc = bs_dim(sys, dim)
+    dtype = dtype_from_env()
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    rc = bs_autotune(sys, dtype, variant)
+    call bs_check(rc, 'bs_autotune')
+
+    niter = int(min(int(niter, c_int64_t), dim))
+    nkeep = min(nkeep, niter)
+    allocate(alpha(niter), beta(niter), evals(nkeep), resid(nkeep))
+    p%nkeep = nkeep
+    p%niter = niter
+    p%thick = nthick
+    p%max_restarts = maxrest
+    p%dtype = dtype
+    p%tol = 0.0d0
+    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c') p%tol = 1.0d-6
+    p%polish = 1
+    nbytes = int(nkeep, c_int64_t) * dim * esize
+    dvec = device_alloc(nbytes)
+    rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
+    call bs_check(rc, 'bs_lanczos')
+
+    open(newunit=iu, file=trim(first_word(outname)) // '.res', status='replace')
+    write(iu, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(iu, '(a,a)') '  single-particle file = ', trim(first_word(spsname))
+    write(iu, '(1x,i0,1x,i0)') z, n
+    write(iu, '(1x,i0,a)') jz2, ' +'
+    write(iu, '(1x,i0,a)') iters, '  iterations '
+    write(iu, '(a)') '  '
+    call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
+    close(iu)
+    open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
+    do k = 1, niter
+      write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
+    end do
+    close(iu)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', dim, ', ', iters, ' matvecs'
+    do k = 1, nkeep
+      write(6, '(i5,3x,f12.6,2x,es10.2)') k, evals(k), resid(k)
+    end do
+    rc = hipFree(dvec)
+    rc = bs_finalize(sys)
+  end subroutine diagonalize_run
+
+  ! Menu s / ss: strength function.  stdin: s, .wfn holding the pivot, output
+  ! name, .int name, scaling line, end, nkeep and the number of (fixed)
+  ! iterations (src/blanczosmain.f90:230-250), the pivot's index in the
+  ! .wfn (readpivot, src/bwfnlib.f90:298-366).  <name>.res gets the total
+  ! strength, the Energy/Strength table of every Ritz value
+  ! (src/blanczoslib.f90:1122-1142) and, for s, the state table and
+  ! <name>.wfn of the normalized Ritz vectors; ss writes neither
+  ! (src/bmenu.f90:572-580).
+  subroutine strength_run(suppress)
+    logical, intent(in) :: suppress
+    character(len=256) :: wfnname, oname, iname, l
+    character(len=:), allocatable :: wpath
+    real(c_float) :: s1, s2, s3, s4
+    type(c_ptr) :: sy, dpiv, dvec
+    integer(c_int64_t) :: d, esize
+    integer(c_int) :: r, nk, nfile, ipiv, niter, iters, dtype, variant
+    type(bs_lanczos_params) :: p
+    real(c_float), target :: pe(1), pj(1), pt(1)
+    real(c_double), allocatable, target :: ritz(:), stren(:), alpha(:), beta(:), evals(:), resid(:)
+    real(c_double), target :: pnorm
+    integer :: j, u
+
+    read(5, '(a)') wfnname
+    read(5, '(a)') oname
+    read(5, '(a)') iname
+    read(5, *) s1, s2, s3, s4
+    read(5, '(a)') l
+    if (adjustl(l) /= 'end') then
+      write(6, *) ' bs_driver: one interaction file is supported'
+      stop 1
+    end if
+    read(5, *) nk, niter
+    read(5, *) ipiv
+    wpath = trim(first_word(wfnname)) // '.wfn'
+    r = bs_init_wfn(cstr(wpath), cstr(trim(first_word(iname)) // '.int'), s1, s2, s3, s4, sy)
+    call bs_check(r, 'bs_init_wfn')
+    r = bs_dim(sy, d)
+    r = bs_read_wfn(sy, cstr(wpath), 0, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, BS_F32, 0, nfile)
+    call bs_check(r, 'bs_read_wfn')
+    if (ipiv < 1 .or. ipiv > nfile) then
+      write(6, '(a,i0)') ' Selected vector is not in range 1:', nfile
+      stop 1
+    end if
+    dtype = dtype_from_env()
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    dpiv = device_alloc(d * esize)
+    r = bs_read_wfn(sy, cstr(wpath), ipiv - 1, 1, c_loc(pe), c_loc(pj), c_loc(pt), dpiv, dtype, 1, nfile)
+    call bs_check(r, 'bs_read_wfn')
+    r = bs_autotune(sy, dtype, variant)
+    call bs_check(r, 'bs_autotune')
+
+    niter = int(min(int(niter, c_int64_t), d))
+    nk = min(nk, niter)
+    allocate(ritz(niter), stren(niter), alpha(niter), beta(niter), evals(nk), resid(nk))
+    p%nkeep = nk
+    p%niter = niter
+    p%thick = 0
+    p%max_restarts = 0
+    p%dtype = dtype
+    p%tol = 0.0d0
+    p%polish = 0     ! the table prints Ritz values, as the reference does
+    dvec = device_alloc(int(nk, c_int64_t) * d * esize)
+    r = bs_strength(sy, p, dpiv, pnorm, c_loc(ritz), c_loc(stren), c_loc(alpha), c_loc(beta), c_loc(evals), &
+                    dvec, c_loc(resid), iters)
+    call bs_check(r, 'bs_strength')
+
+    open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
+    write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(u, '(a,a,a,i0)') '  pivot read from ', wpath, ', state ', ipiv
+    write(u, *) pnorm * pnorm, ' = total strength '
+    write(u, '(1x,i0,a)') niter, '  iterations '
+    write(u, '(a)') '  '
+    write(u, *) ' Energy   Strength    '
+    write(u, *) ' ______   ________  '
+    do j = 1, niter
+      write(u, 88) ritz(j), stren(j)
+    end do
+88  format(3f12.6)
+    write(u, *) '_______   ________   '
+    write(u, *) pnorm**2, '= total strength, included above '
+    write(u, *) ' '
+    if (suppress) then
+      write(u, *) ' Note: not writing wfns to file, nor computing J, T '
+    else
+      call state_table(u, sy, dvec, nk, evals, dtype, d, .true., trim(first_word(oname)) // '.wfn')
+    end if
+    close(u)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', iters, ' matvecs'
+    r = hipFree(dvec)
+    r = hipFree(dpiv)
+    r = bs_finalize(sy)
+  end subroutine strength_run
+
+  ! The State/E/Ex/J/T/par table (src/bparticleocc.f90:256) of nk device
+  ! vectors with J and T from <J^2>, <T^2>, and optionally their .wfn.
+  subroutine state_table(iu, sy, dv, nk, ev, dtype, d, write_wfn, wfnpath)
+    integer, intent(in) :: iu, nk
+    type(c_ptr), intent(in) :: sy, dv
+    real(c_double), intent(in) :: ev(nk)
+    integer(c_int), intent(in) :: dtype
+    integer(c_int64_t), intent(in) :: d
+    logical, intent(in) :: write_wfn
+    character(len=*), intent(in) :: wfnpath
+    real(c_float), allocatable, target :: we(:), wj(:), wt2(:)
+    real(c_double) :: j2, t2
+    real :: xj, xt
+    integer(c_int64_t) :: esize
+    integer(c_int) :: rc
+    integer :: k
+
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    write(iu, '(a)') '  State      E        Ex         J       T      par'
+    allocate(we(nk), wj(nk), wt2(nk))
+    do k = 1, nk
+      rc = bs_observable(sy, 0, ptr_offset(dv, int(k - 1, c_int64_t) * d * esize), dtype, j2)
+      call bs_check(rc, 'bs_observable J^2')
+      rc = bs_observable(sy, 1, ptr_offset(dv, int(k - 1, c_int64_t) * d * esize), dtype, t2)
+      call bs_check(rc, 'bs_observable T^2')
+      xj = real(-0.5d0 + sqrt(0.25d0 + max(j2, 0.0d0)))
+      xt = real(-0.5d0 + sqrt(0.25d0 + max(t2, 0.0d0)))
+      write(iu, 100) k, ev(k), ev(k) - ev(1), xj, xt, 1
+      ! .wfn carries E, J and T(T+1) in single precision (src/blanczoslib.f90:1330-1350)
+      we(k) = real(ev(k), c_float)
+      wj(k) = xj
+      wt2(k) = real(t2, c_float)
+    end do
+100 format(i5,3x,2f10.5,2x,2f8.3,3x,i2)
+    if (write_wfn) then
+      rc = bs_write_wfn(sy, cstr(wfnpath), nk, c_loc(we), c_loc(wj), c_loc(wt2), dv, dtype, 1_c_int)
+      call bs_check(rc, 'bs_write_wfn')
+    end if
+  end subroutine state_table
+
+  ! vector precision: FP32 (default) or FP64 with environment BS_DTYPE=f64
+  function dtype_from_env() result(dt)
+    integer(c_int) :: dt
+    character(len=16) :: env
+    call get_environment_variable('BS_DTYPE', env)
+    dt = BS_F32
+    if (trim(env) == 'f64') dt = BS_F64
+  end function dtype_from_env
+
+  function device_alloc(nbytes) result(ptr)
+    integer(c_int64_t), intent(in) :: nbytes
+    type(c_ptr) :: ptr
+    integer(c_int) :: rc
+    rc = hipMalloc(ptr, int(nbytes, c_size_t))
+    if (rc /= 0) then
+      write(6, *) ' hipMalloc failed ', rc
+      stop 1
+    end if
+  end function device_alloc
+
   ! Menu x: expectation value of a scalar two-body operator (an .int file)
   ! in each state of a previous .wfn (src/bmenu.f90:520-526,
   ! src/bexpect.f90:16-74).  stdin: x, .wfn name, output name, .int name,
@@ -189,11 +325,7 @@ contains
       dn(i) = sqrt(sum(hv(:, i)**2))
       if (dn(i) > 0.0d0) hv(:, i) = hv(:, i) / dn(i)
     end do
-    r = hipMalloc(dv, int(8 * d * nk, c_size_t))
-    if (r /= 0) then
-      write(6, *) ' hipMalloc failed ', r
-      stop 1
-    end if
+    dv = device_alloc(8 * d * nk)
     r = hipMemcpy(dv, c_loc(hv), int(8 * d * nk, c_size_t), 1_c_int)
     if (r /= 0) then
       write(6, *) ' hipMemcpy failed ', r

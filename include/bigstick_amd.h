@@ -156,6 +156,19 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Strength function: bs_lanczos started from `pivot` (device, dim elements
+ * in p->dtype; not modified) instead of the uniform vector, with a fixed
+ * number of iterations (p->thick must be 0).  pivot_norm receives ||pivot||;
+ * ritz and strength (niter entries each, may be NULL) receive every Ritz
+ * value E_k of the final tridiagonal matrix and its strength
+ * |Y(1,k)|^2 ||pivot||^2, which sum to ||pivot||^2.  The other outputs are
+ * those of bs_lanczos.  Replaces menu s: readpivot as the starting vector
+ * (src/bwfnlib.f90:298-366, src/blanczosmain.f90:1560-1568) and the
+ * Energy/Strength table of lanczos_output (src/blanczoslib.f90:1122-1142). */
+int bs_strength(bs_system *sys, const bs_lanczos_params *p, const void *pivot, double *pivot_norm,
+                double *ritz, double *strength, double *alpha, double *beta, double *evals, void *evecs,
+                double *resid, int *iters);
+
 /* Matrix <v_f | O | v_i> of a scalar two-body operator O, given as an iso .int
  * file with its scaling line (intf = NULL: the system's own Hamiltonian),
  * between nvec vectors (device, dim elements each, consecutive) on this

@@ -254,6 +254,50 @@ def expect(name, op_seed):
         shutil.rmtree(wd)
 
 
+def pivot_wfn(name, seed):
+    """<name>_pivot.wfn: the header of the reference's <name>.wfn with two
+    pivots that are not eigenstates: a seeded Gaussian vector of norm 1.3,
+    and 0.6 (state 1) + 0.8 (state 3) + 0.05 (the Gaussian, normalized)."""
+    import struct
+    sys.path.insert(0, os.path.dirname(HERE))
+    from wfn_io import read_wfn
+    ref = read_wfn(os.path.join(HERE, name + ".wfn"))
+    g = np.random.default_rng(seed).standard_normal(ref["dim"])
+    g /= np.linalg.norm(g)
+    st = [s["vec"].astype(np.float64) for s in ref["states"]]
+    vecs = [1.3 * g, 0.6 * st[0] + 0.8 * st[2] + 0.05 * g]
+    with open(os.path.join(HERE, name + "_pivot.wfn"), "wb") as f:
+        f.write(ref["header"])
+        f.write(struct.pack("<i", len(vecs)))
+        for k, v in enumerate(vecs):
+            f.write(struct.pack("<2i3f", 27182818, k + 1, 0.0, 0.0, 0.0))
+            f.write(v.astype("<f4").tobytes())
+
+
+# strength functions (menu s / ss) from the pivots of <name>_pivot.wfn with
+# the Hamiltonian that made <name>.wfn: (name, menu, pivot, nkeep, niter)
+STRENGTH = [("sd2p2n", "s", 1, 5, 20), ("sd2p2n", "ss", 2, 3, 30), ("pf2p1n_m3", "s", 2, 4, 25)]
+
+
+def strength(name, menu, pivot, nkeep, niter):
+    """The reference's strength-function .res, kept as
+    <name>_<menu><pivot>.res."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    out = "%s_%s%d" % (name, menu, pivot)
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        shutil.copy(os.path.join(HERE, name + "_pivot.wfn"), wd)
+        run_ref(wd, "%s\n%s_pivot\n%s\n%s\n1 1 1 1\nend\n%d %d\n%d\n" %
+                (menu, name, out, name, nkeep, niter, pivot))
+        shutil.copy(os.path.join(wd, out + ".res"), os.path.join(HERE, out + ".res"))
+        print(out)
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -290,6 +334,12 @@ if __name__ == "__main__":
     for c in EXPECT:
         if "x_" + c[0] in want:
             expect(*c)
+    for c, seed in (("sd2p2n", 90), ("pf2p1n_m3", 91)):
+        if "pivot_" + c in want:
+            pivot_wfn(c, seed)
+    for c in STRENGTH:
+        if "strength" in want:
+            strength(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -159,3 +159,76 @@ def test_fortran_host_menu_x_refuses_foreign_basis(casedir):
     r = _run("x\nsd2p2n\nbad_x\noppf\n1 1 1 1\nend\n", wd)
     assert r.returncode != 0
     assert "bs_init_wfn" in r.stdout + r.stderr
+
+
+def read_strength(path):
+    """Total strength, the Energy/Strength rows and the state table of a
+    strength-function .res (src/blanczoslib.f90:1122-1142)."""
+    lines = open(path, errors="replace").read().splitlines()
+    total = [float(l.split()[0]) for l in lines if l.strip().endswith("= total strength")][0]
+    i0 = [i for i, l in enumerate(lines) if l.strip().startswith("______")][0]
+    rows = []
+    for l in lines[i0 + 1:]:
+        if l.strip().startswith("_______"):
+            break
+        rows.append((float(l[:12]), float(l[12:24]) if "*" not in l[12:24] else np.inf))
+    table = []
+    for i, l in enumerate(lines):
+        if l.strip().startswith("State") and "Ex" in l:
+            for l2 in lines[i + 1:]:
+                p = l2.split()
+                if len(p) != 6 or not p[0].isdigit():
+                    break
+                table.append([float(p[1]), float(p[3]), float(p[4])])
+            break
+    return total, np.array(rows), np.array(table)
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,menu,pivot,nkeep,niter", [("sd2p2n", "s", 1, 5, 20), ("sd2p2n", "ss", 2, 3, 30),
+                                                         ("pf2p1n_m3", "s", 2, 4, 25)])
+def test_fortran_host_strength_matches_reference(name, menu, pivot, nkeep, niter, casedir):
+    """Menu s / ss from a pivot in a .wfn: the total strength, the first nkeep
+    Energy/Strength rows and (s) the state table agree with the reference's
+    .res; every row's strength is a valid weight and they sum to the total.
+    Rows beyond nkeep are not compared: the reference fills them from
+    eigenvectors its DSYEVR call does not compute (range 1..nkeep,
+    src/blanczoslib.f90:230-236)."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    from wfn_io import read_wfn
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    meta = {"sd2p2n": ("sd", 2, 2, 0, 77), "pf2p1n_m3": ("pf", 2, 1, 3, 83)}[name]
+    case = small_case(*meta[:4], seed=meta[4])
+    case.name = name
+    wd = os.path.join(casedir, "fs_%s_%s%d" % (name, menu, pivot))
+    write_case(case, wd)
+    shutil.copy(os.path.join(gold, name + "_pivot.wfn"), wd)
+    out = "%s_%s%d" % (name, menu, pivot)
+    r = _run("%s\n%s_pivot\n%s\n%s\n1 1 1 1\nend\n%d %d\n%d\n" % (menu, name, out, name, nkeep, niter, pivot), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tot, rows, table = read_strength(os.path.join(wd, out + ".res"))
+    rtot, rrows, rtable = read_strength(os.path.join(gold, out + ".res"))
+    assert abs(tot - rtot) <= 1e-12 * rtot
+    assert rows.shape == (niter, 2) and rrows.shape[0] == niter
+    # strengths to the printed digits; energies of the rows that carry
+    # strength (a Ritz value with weight below 1e-3 in a 20-30 step FP32 run is
+    # not converged and moves with rounding, in either code)
+    assert np.abs(rows[:nkeep, 1] - rrows[:nkeep, 1]).max() <= 5e-6, (rows[:nkeep], rrows[:nkeep])
+    big = rrows[:nkeep, 1] > 1e-3
+    assert np.abs(rows[:nkeep, 0] - rrows[:nkeep, 0])[big].max() <= 5e-6, (rows[:nkeep], rrows[:nkeep])
+    assert (rows[:, 1] >= 0).all() and abs(rows[:, 1].sum() - tot) <= 1e-5 * niter
+    assert np.all(np.diff(rows[:, 0]) > 0)
+    wfn = os.path.join(wd, out + ".wfn")
+    if menu == "ss":
+        assert len(table) == 0 and not os.path.exists(wfn)
+    else:
+        assert table.shape == rtable.shape == (nkeep, 3)
+        assert np.abs(table[:, 0] - rtable[:, 0])[big].max() < 2e-5
+        assert np.abs(table[:, 1:] - rtable[:, 1:])[big].max() < 2e-3
+        w = read_wfn(wfn)
+        assert w["nkeep"] == nkeep and w["header"] == read_wfn(os.path.join(gold, name + ".wfn"))["header"]
+        for st in w["states"]:
+            assert abs(float(np.dot(st["vec"], st["vec"])) - 1.0) < 1e-5
+    r = _run("%s\n%s_pivot\n%s\n%s\n1 1 1 1\nend\n%d %d\n%d\n" % (menu, name, out, name, nkeep, niter, 3), wd)
+    assert r.returncode != 0 and "not in range 1:2" in r.stdout

@@ -186,3 +186,31 @@ def test_states_from_reference_wfn(casedir):
     o = Oracle(paths(case, d)[0], opf, case.Z, case.N, case.jz2)
     want = np.array([[vh[f] @ o.apply(vh[i]) for f in range(len(e))] for i in range(len(e))])
     assert _rel(M, want) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 2e-5)])
+def test_strength_function_moments(dtype, tol, casedir):
+    """Lanczos from a pivot p is a Gauss quadrature of p's spectral measure:
+    sum_k S_k E_k^n = <p|H^n|p> for n < 2 niter.  Checked for n = 0..8
+    against the oracle's H (tolerance relative to |<p|H^n|p>| + ||p||^2 ||H||^n),
+    and the pivot norm is ||p||."""
+    import torch
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    s = _sys(case, d)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    p = np.random.default_rng(21).standard_normal(s.dim) * 0.05
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    r = s.strength(torch.tensor(p, dtype=tdt, device="cuda"), nkeep=4, niter=24)
+    pq = p.astype(np.float32).astype(np.float64) if dtype == "f32" else p
+    assert abs(r.pivot_norm - np.linalg.norm(pq)) <= 1e-12 * np.linalg.norm(pq)
+    assert (r.strength >= 0).all() and np.all(np.diff(r.ritz) > 0)
+    hn = max(abs(r.ritz[0]), abs(r.ritz[-1]))
+    v = pq.copy()
+    for n in range(9):
+        want = float(pq @ v)
+        got = float(np.sum(r.strength * r.ritz ** n))
+        assert abs(got - want) <= tol * (abs(want) + pq @ pq * hn ** n), (n, got, want)
+        v = o.apply(v)
+    # the lowest Ritz values equal bs_lanczos's from the same pivot direction
+    assert np.abs(r.evals[:2] - r.ritz[:2]).max() < 1e-6

@@ -133,3 +133,15 @@ def test_entry_points_refuse_bad_arguments(casedir, tmp_path):
     assert L.bs_write_wfn(s.h, str(tmp_path / "no" / "such" / "dir.wfn").encode(), 1, e.ctypes.data,
                           e.ctypes.data, e.ctypes.data, v.ctypes.data, BS_F32, 0) != 0
     assert b"cannot open" in L.bs_last_error()
+    # strength function: thick restart, bad dtype, null pivot, no upload
+    from bigstickpublick_amd.engine import _Params
+    ev = np.zeros(4)
+    pn = ctypes.c_double()
+    it = ctypes.c_int()
+    for prm, piv, msg in ((_Params(2, 10, 4, 1, BS_F32, 0.0, 1), v.ctypes.data, b"thick = 0"),
+                          (_Params(2, 10, 0, 0, 3, 0.0, 1), v.ctypes.data, b"dtype"),
+                          (_Params(2, 10, 0, 0, BS_F32, 0.0, 1), None, b"null"),
+                          (_Params(2, 10, 0, 0, BS_F32, 0.0, 1), v.ctypes.data, b"not uploaded")):
+        assert L.bs_strength(s.h, ctypes.byref(prm), piv, ctypes.byref(pn), None, None, None, None,
+                             ev.ctypes.data, None, None, ctypes.byref(it)) != 0
+        assert msg in L.bs_last_error(), L.bs_last_error()
