@@ -2895,6 +2895,34 @@ int bs_operator_matrix(bs_system* s, const char* intf, float spscale, float a0, 
   }
 }
 
+int bs_apply_onebody(bs_system* s, const char* opme, const void* x, void* y, int nvec, int dtype, int* jop,
+                     int* top) {
+  try {
+    checked(s);
+    if (!opme || !x || !y || nvec < 1) return set_err("null argument or nvec < 1");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    const bs::OneBodyOp op = bs::read_opme(opme, s->t);
+    if (jop) *jop = op.jop;
+    if (top) *top = op.top;
+    bs_system o;
+    o.t = bs::onebody_tables(s->t, op);
+    upload_tables(&o);
+    const int64_t D = s->t.dim;
+    const size_t es = dtype == BS_F32 ? 4 : 8;
+    // (src/bdenslib1.f90:1666-1689) each state: y = O x
+    for (int k = 0; k < nvec; k++) {
+      const char* xk = static_cast<const char*>(x) + (size_t)k * D * es;
+      char* yk = static_cast<char*>(y) + (size_t)k * D * es;
+      if (dtype == BS_F32) launch_apply<float>(&o, (const float*)xk, (float*)yk, 0, nullptr);
+      else launch_apply<double>(&o, (const double*)xk, (double*)yk, 0, nullptr);
+    }
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
 int bs_num_params(const bs_system* s, int* nspe, int* ntbme) {
   if (!s) return set_err("null system");
   if (nspe) *nspe = (int)s->t.in.orbits.size();
@@ -3156,11 +3184,18 @@ int bs_write_wfn(const bs_system* s, const char* path, int nkeep, const float* e
 int bs_init_wfn_host(const char* wfn, const char* intf, float spscale, float a0, float a, float x,
                      bs_system** out) {
   try {
-    if (!wfn || !intf || !out) return set_err("null argument");
+    if (!wfn || !out) return set_err("null argument");
     *out = nullptr;
     const bs::WfnInfo w = bs::read_wfn_info(wfn);
     std::unique_ptr<bs_system> s(new bs_system());
-    s->t = bs::build_tables(w.orbits, intf, w.Z, w.N, w.jz2, spscale, a0, a, x, true);
+    if (intf) {
+      s->t = bs::build_tables(w.orbits, intf, w.Z, w.N, w.jz2, spscale, a0, a, x, true);
+    } else {
+      // no interaction: the basis and its jumps only (H = 0)
+      std::string zero = "0";
+      for (size_t i = 0; i < w.orbits.size(); i++) zero += " 0";
+      s->t = bs::build_tables_text(w.orbits, zero + "\n", w.Z, w.N, w.jz2, true);
+    }
     // the rebuilt basis must be the one the vectors are stored in
     if (bs::wfn_header(s->t) != w.header)
       return set_err(std::string("basis built from ") + wfn +

@@ -15,6 +15,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
+#include <utility>
 
 namespace bs {
 
@@ -1124,6 +1125,162 @@ std::string wfn_header(const Tables& t) {
   const int32_t off = (int32_t)h.size() + 4;
   std::memcpy(&h[12], &off, 4);
   return h;
+}
+
+OneBodyOp read_opme(const std::string& path, const Tables& t) {
+  std::ifstream f(path);
+  if (!f) fail("cannot open .opme file " + path);
+  std::string title;
+  std::getline(f, title);
+  OneBodyOp op;
+  bool xpn = false;
+  const std::string kind = title.substr(0, 3);
+  if (kind == "iso") op.pn = false;
+  else if (kind == "pns") op.pn = true;
+  else if (kind == "xpn") op.pn = xpn = true;
+  else fail(".opme: something wrong with the first line (iso, pns or xpn expected): " + path);
+  const std::vector<Orbit>& orb = t.in.orbits;
+  const int no = (int)orb.size();
+  int n = -1;
+  if (!(f >> n) || n != no) fail(".opme: mismatch in single particle orbits: " + path);
+  for (int i = 1; i <= no; i++) {
+    int i1 = 0, i2 = 0, nr = 0, l = 0;
+    double xj = 0.0;
+    if (xpn) {
+      if (!(f >> i1 >> i2 >> nr >> l >> xj) || i2 != i + no) fail(".opme: bad orbit line: " + path);
+    } else if (!(f >> i1 >> nr >> l >> xj)) {
+      fail(".opme: bad orbit line: " + path);
+    }
+    if (i1 != i || nr != orb[i - 1].n || l != orb[i - 1].l || (int)std::lround(2.0 * xj) != orb[i - 1].j2)
+      fail(".opme: mismatch in orbits at orbit " + std::to_string(i) + ": " + path);
+  }
+  op.top = 2;
+  if (!(f >> op.jop)) fail(".opme: missing J of the operator: " + path);
+  if (!op.pn && !(f >> op.top)) fail(".opme: missing T of the operator: " + path);
+  if (op.jop < 0 || op.top < 0 || (!op.pn && op.top > 1)) fail(".opme: bad J or T of the operator: " + path);
+  std::vector<float> iso(no * no, 0.0f), pv(no * no, 0.0f), nv(no * no, 0.0f);
+  for (int k = 0; k < 10000; k++) {
+    int a = 0, b = 0;
+    float x = 0.0f, y = 0.0f;
+    if (op.pn && !xpn) {
+      if (!(f >> a >> b >> x >> y)) break;
+    } else if (!(f >> a >> b >> x)) {
+      break;
+    }
+    bool hasp = true, hasn = true;
+    if (xpn) {
+      if ((a > no) != (b > no)) continue;  // charge-changing: skipped, as the reference does
+      if (a > no) {
+        a -= no;
+        b -= no;
+        y = x;
+        x = 0.0f;
+        hasp = false;
+      } else {
+        y = 0.0f;
+        hasn = false;
+      }
+    }
+    if (a < 1 || a > no || b < 1 || b > no) fail(".opme: orbit index out of range: " + path);
+    const int ja = orb[a - 1].j2, jb = orb[b - 1].j2;
+    if (ja + jb < 2 * op.jop || std::abs(ja - jb) > 2 * op.jop)
+      fail(".opme: mismatch in angular momentum of operator for orbits " + std::to_string(a) + " " +
+           std::to_string(b) + ": " + path);
+    if (op.pn) {
+      if (t.in.Z > 0 && hasp) pv[(a - 1) * no + b - 1] = x;
+      if (t.in.N > 0 && hasn) nv[(a - 1) * no + b - 1] = y;
+    } else {
+      iso[(a - 1) * no + b - 1] = x;
+    }
+  }
+  // decouple, in float arithmetic in the reference's order of operations
+  const int ns = t.nsps;
+  op.p.assign((size_t)ns * ns, 0.0f);
+  op.n.assign((size_t)ns * ns, 0.0f);
+  for (int it = 0; it < 2; it++) {
+    std::vector<float>& out = it == 0 ? op.p : op.n;
+    if (op.pn && (it == 0 ? t.in.Z : t.in.N) == 0) continue;
+    for (int sa = 0; sa < ns; sa++)
+      for (int sb = 0; sb < ns; sb++) {
+        const int a = t.sorb[sa], b = t.sorb[sb];
+        const int ja = t.sj2[sa], ma = t.sm2[sa], jb = t.sj2[sb], mb = t.sm2[sb];
+        const float v = op.pn ? (it == 0 ? pv : nv)[a * no + b] : iso[a * no + b];
+        if (v == 0.0f || ma != mb) continue;
+        if (op.jop > (ja + jb) / 2 || op.jop < std::abs(ja - jb) / 2) continue;
+        int iphase = ((jb - mb) / 2) % 2 == 0 ? 1 : -1;
+        float r;
+        if (op.pn) {
+          r = (float)iphase * cleb(ja, ma, jb, -mb, 2 * op.jop, ma - mb) * v / std::sqrt(2.0f * op.jop + 1.0f);
+        } else {
+          if (it == 1) iphase = -iphase;  // neutrons: T_z = -1/2
+          const float ct = it == 0 ? cleb(1, 1, 1, -1, 2 * op.top, 0) : cleb(1, -1, 1, 1, 2 * op.top, 0);
+          r = (float)iphase * cleb(ja, ma, jb, -mb, 2 * op.jop, ma - mb) * v / std::sqrt(2.0f * op.top + 1.0f) /
+              std::sqrt(2.0f * op.jop + 1.0f) * ct;
+        }
+        out[(size_t)sa * ns + sb] = r;
+      }
+  }
+  return op;
+}
+
+Tables onebody_tables(const Tables& base, const OneBodyOp& op) {
+  Tables t;
+  t.in = base.in;
+  t.nsps = base.nsps;
+  t.nL = base.nL;
+  t.nR = base.nR;
+  t.sorb = base.sorb;
+  t.sj2 = base.sj2;
+  t.sm2 = base.sm2;
+  t.ssorb = base.ssorb;
+  t.ssm2 = base.ssm2;
+  t.sp[0] = base.sp[0];
+  t.sp[1] = base.sp[1];
+  t.dim = base.dim;
+  t.call_spe = false;
+  t.pspe.assign(base.in.orbits.size(), 0.0f);
+  t.nspe.assign(base.in.orbits.size(), 0.0f);
+  t.ndelta = base.ndelta;
+  t.delta_of_class = base.delta_of_class;
+  t.class_of_op = base.class_of_op;
+  t.local_of_op = base.local_of_op;
+  t.class_size = base.class_size;
+  t.class_offset = base.class_offset;
+  t.wpn = base.wpn;
+  t.max_class = base.max_class;
+  const int ns = base.nsps;
+  for (int it = 0; it < 2; it++) {
+    const OneBody& O = base.ob[it];
+    const std::vector<float>& pot = it == 0 ? op.p : op.n;
+    const size_t nsd = base.sp[it].mask.size();
+    if (O.rowptr.size() != nsd + 1) fail("one-body jumps missing");
+    TwoBody& T = t.tb[it];
+    T.rowptr.assign(1, 0);
+    std::vector<std::pair<int32_t, double>> row;
+    for (size_t f = 0; f < nsd; f++) {
+      row.clear();
+      for (int64_t q = O.rowptr[f]; q < O.rowptr[f + 1]; q++) {
+        const int a = O.op[q] / ns, c = O.op[q] % ns;
+        if (t.sm2[a] != t.sm2[c]) continue;
+        const float v = pot[(size_t)a * ns + c];
+        if (v == 0.0f) continue;
+        row.emplace_back(O.src[q], (double)v * O.phase[q]);
+      }
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) { return x.first < y.first; });
+      for (size_t e = 0; e < row.size(); e++) {
+        if (e > 0 && row[e].first == T.src.back() && T.src.size() > (size_t)T.rowptr.back()) {
+          T.val.back() += row[e].second;
+        } else {
+          T.src.push_back(row[e].first);
+          T.val.push_back(row[e].second);
+        }
+      }
+      T.rowptr.push_back((int64_t)T.src.size());
+    }
+    t.ob[it].rowptr.assign(nsd + 1, 0);
+  }
+  return t;
 }
 
 WfnInfo read_wfn_info(const std::string& path) {

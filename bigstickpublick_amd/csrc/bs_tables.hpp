@@ -173,4 +173,19 @@ struct WfnInfo {
 WfnInfo read_wfn_info(const std::string& path);
 float cleb(int j1, int m1, int j2, int m2, int j3, int m3);
 
+// A one-body operator read from an .opme file (readin1bodyop,
+// src/bdenslib1.f90:1823-2071) and decoupled to its M = 0 m-scheme
+// coefficients (decouple1bodyop, :2444-2597): p / n hold, over combined
+// labels, [a * nsps + b] = the FP32 coefficient of a+_a a_b.
+struct OneBodyOp {
+  int jop = 0, top = 0;
+  bool pn = false;
+  std::vector<float> p, n;
+};
+OneBodyOp read_opme(const std::string& path, const Tables& t);
+// Tables that apply the operator with the engine's PP and NN sections: the
+// one-body jumps of `base` with Delta m = 0, merged per (target, source) SD
+// pair; no PN part, no SPEs.
+Tables onebody_tables(const Tables& base, const OneBodyOp& op);
+
 }  // namespace bs

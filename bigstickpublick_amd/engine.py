@@ -66,6 +66,7 @@ def load_library():
     for name in ("bs_init_wfn_host", "bs_init_wfn"):
         getattr(L, name).argtypes = [ctypes.c_char_p, ctypes.c_char_p, f32, f32, f32, f32, ctypes.POINTER(vp)]
     L.bs_read_wfn.argtypes = [vp, ctypes.c_char_p, i32, i32, vp, vp, vp, vp, i32, i32, ctypes.POINTER(i32)]
+    L.bs_apply_onebody.argtypes = [vp, ctypes.c_char_p, vp, vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.bs_multi_variant.argtypes = [vp, i32, i32, i32, i32]
     L.bs_batch_set_variant.argtypes = [vp, i32, i32]
     L.bs_multi_info.argtypes = [vp, i32, i32, i32, ctypes.POINTER(i32), vp]
@@ -122,14 +123,14 @@ class System:
         self._adopt(h, upload)
 
     @classmethod
-    def from_wfn(cls, wfn, intfile, scaling=(1.0, 1.0, 1.0, 1.0), upload=False):
+    def from_wfn(cls, wfn, intfile=None, scaling=(1.0, 1.0, 1.0, 1.0), upload=False):
         """The basis recorded in a .wfn file (orbits, Z, N, 2Jz) with the
-        interaction `intfile`; fails unless the rebuilt basis reproduces the
-        file's header."""
+        interaction `intfile` (None: no interaction); fails unless the rebuilt
+        basis reproduces the file's header."""
         L = load_library()
         h = ctypes.c_void_p()
-        _check(L.bs_init_wfn_host(str(wfn).encode(), str(intfile).encode(), *[float(v) for v in scaling],
-                                  ctypes.byref(h)))
+        name = None if intfile is None else str(intfile).encode()
+        _check(L.bs_init_wfn_host(str(wfn).encode(), name, *[float(v) for v in scaling], ctypes.byref(h)))
         self = cls.__new__(cls)
         self._adopt(h, upload)
         return self
@@ -357,6 +358,22 @@ class System:
         _check(load_library().bs_operator_matrix(self.h, name, *[float(s) for s in scaling],
                                                  ctypes.c_void_p(v.data_ptr()), n, dt, out.ctypes.data))
         return out
+
+    def apply_onebody(self, opme, x):
+        """O x for the one-body operator of an .opme file and a (dim,) or
+        (nvec, dim) CUDA float32/float64 tensor; returns (y, J, T)."""
+        import torch
+        xv = x.contiguous()
+        if xv.shape[-1] != self.dim or xv.dtype not in (torch.float32, torch.float64):
+            raise ValueError("x must be a float32/float64 CUDA tensor with dim columns")
+        y = torch.empty_like(xv)
+        nvec = xv.numel() // self.dim
+        j, t = ctypes.c_int(), ctypes.c_int()
+        _check(load_library().bs_apply_onebody(self.h, str(opme).encode(), ctypes.c_void_p(xv.data_ptr()),
+                                               ctypes.c_void_p(y.data_ptr()), nvec,
+                                               BS_F32 if xv.dtype == torch.float32 else BS_F64,
+                                               ctypes.byref(j), ctypes.byref(t)))
+        return y, j.value, t.value
 
     def wfn_count(self, path):
         """Number of states in a .wfn file on this basis."""

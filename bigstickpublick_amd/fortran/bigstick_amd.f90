@@ -138,6 +138,25 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_init_wfn
 
+    ! intf = c_null_ptr: the basis only (H = 0)
+    function bs_init_wfn_host(wfn, intf, spscale, a0, a, x, sys) bind(C, name='bs_init_wfn_host') result(rc)
+      import :: c_char, c_int, c_float, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: wfn
+      type(c_ptr), value :: intf
+      real(c_float), value :: spscale, a0, a, x
+      type(c_ptr), intent(out) :: sys
+      integer(c_int) :: rc
+    end function bs_init_wfn_host
+
+    function bs_apply_onebody(sys, opme, x, y, nvec, dtype, jop, top) bind(C, name='bs_apply_onebody') result(rc)
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: sys, x, y
+      character(kind=c_char), dimension(*), intent(in) :: opme
+      integer(c_int), value :: nvec, dtype
+      integer(c_int), intent(out) :: jop, top
+      integer(c_int) :: rc
+    end function bs_apply_onebody
+
     function bs_read_wfn(sys, path, first, count, energy, xj, xt2, vecs, dtype, on_device, nkeep) &
         bind(C, name='bs_read_wfn') result(rc)
       import :: c_ptr, c_int, c_char

@@ -1,7 +1,8 @@
 ! bs_driver: a Fortran host for the MI355X engine that reads BIGSTICK's own
 ! stdin menu answers for a normal run (menu option n, src/bmenu.f90:11;
-! menu s, strength function from a pivot in a previous .wfn, and menu x,
-! expectation values in previous states, below;
+! menu s, strength function from a pivot in a previous .wfn; menu x,
+! expectation values in previous states; menu o, a one-body operator applied
+! to previous states, below;
 ! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td,
 ! src/blanczosmain.f90:380-745) and writes <name>.res (energy table in the
 ! format of src/bparticleocc.f90:256) and <name>.lcoef (alpha, beta per
@@ -27,9 +28,15 @@ program bs_driver
     call strength_run(opt(2:2) == 's' .or. opt(2:2) == 'S')
   case ('x', 'X')
     call expectation_run()
+  case ('o', 'O')
+    if (len_trim(opt) > 1) then
+      write(6, *) ' bs_driver: of the o options only o itself is supported'
+      stop 1
+    end if
+    call onebody_run()
   case default
-    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function)', &
-                ' and x (expectation values)'
+    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function),', &
+                ' x (expectation values) and o (one-body operator applied to states)'
     stop 1
   end select
 
@@ -220,6 +227,57 @@ contains
     r = hipFree(dpiv)
     r = bs_finalize(sy)
   end subroutine strength_run
+
+  ! Menu o: the one-body operator of an .opme file applied to every state of
+  ! a previous .wfn; <name>.wfn gets O|state>, not normalized, with each
+  ! state's E, J, T(T+1) (src/bmenu.f90:423-434, src/bdenslib1.f90:1645-1693).
+  ! stdin: o, .wfn name, output name, .opme name.
+  subroutine onebody_run()
+    character(len=256) :: wfnname, oname, opname
+    character(len=:), allocatable :: wpath
+    type(c_ptr) :: sy, dx, dy
+    integer(c_int64_t) :: d, esize
+    integer(c_int) :: r, nk, nk2, dtype, jop, top
+    real(c_float), allocatable, target :: e(:), ej(:), et(:)
+    integer :: u
+
+    read(5, '(a)') wfnname
+    read(5, '(a)') oname
+    read(5, '(a)') opname
+    wpath = trim(first_word(wfnname)) // '.wfn'
+    r = bs_init_wfn_host(cstr(wpath), c_null_ptr, 1.0, 1.0, 1.0, 1.0, sy)
+    call bs_check(r, 'bs_init_wfn_host')
+    r = bs_dim(sy, d)
+    r = bs_read_wfn(sy, cstr(wpath), 0, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, BS_F32, 0, nk)
+    call bs_check(r, 'bs_read_wfn')
+    if (nk < 1) then
+      write(6, *) ' bs_driver: no states in ', trim(wpath)
+      stop 1
+    end if
+    dtype = dtype_from_env()
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    allocate(e(nk), ej(nk), et(nk))
+    dx = device_alloc(int(nk, c_int64_t) * d * esize)
+    dy = device_alloc(int(nk, c_int64_t) * d * esize)
+    r = bs_read_wfn(sy, cstr(wpath), 0, nk, c_loc(e), c_loc(ej), c_loc(et), dx, dtype, 1, nk2)
+    call bs_check(r, 'bs_read_wfn')
+    r = bs_apply_onebody(sy, cstr(trim(first_word(opname)) // '.opme'), dx, dy, nk, dtype, jop, top)
+    call bs_check(r, 'bs_apply_onebody')
+    r = bs_write_wfn(sy, cstr(trim(first_word(oname)) // '.wfn'), nk, c_loc(e), c_loc(ej), c_loc(et), dy, dtype, 1)
+    call bs_check(r, 'bs_write_wfn')
+    open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
+    write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(u, '(a,a)') '  states read from ', wpath
+    write(u, '(a,a,a,i0,a,i0)') '  one-body operator ', trim(first_word(opname)) // '.opme', ', J = ', jop, &
+                                ', T = ', top
+    write(u, '(1x,i0,a)') nk, '  states '
+    close(u)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', nk, ' states'
+    r = hipFree(dy)
+    r = hipFree(dx)
+    r = bs_finalize(sy)
+  end subroutine onebody_run
 
   ! The State/E/Ex/J/T/par table (src/bparticleocc.f90:256) of nk device
   ! vectors with J and T from <J^2>, <T^2>, and optionally their .wfn.

@@ -169,6 +169,16 @@ int bs_strength(bs_system *sys, const bs_lanczos_params *p, const void *pivot, d
                 double *ritz, double *strength, double *alpha, double *beta, double *evals, void *evecs,
                 double *resid, int *iters);
 
+/* y_k = O x_k (overwritten) for nvec device vectors of dim elements, O the
+ * one-body operator of an .opme file (first line iso, pns or xpn; orbits
+ * checked against the system's; charge-changing xpn lines skipped) in its
+ * M = 0 m-scheme form.  jop, top (may be NULL) receive the operator's J and
+ * T (2 for pns/xpn).  Replaces menu o: readin1bodyop + decouple1bodyop
+ * (src/bdenslib1.f90:1823-2071, :2444-2597) and applicator1b's
+ * master_apply1bop per state (:1645-1693, :2612-2707). */
+int bs_apply_onebody(bs_system *sys, const char *opme, const void *x, void *y, int nvec, int dtype, int *jop,
+                     int *top);
+
 /* Matrix <v_f | O | v_i> of a scalar two-body operator O, given as an iso .int
  * file with its scaling line (intf = NULL: the system's own Hamiltonian),
  * between nvec vectors (device, dim elements each, consecutive) on this
@@ -190,7 +200,8 @@ int bs_write_wfn(const bs_system *sys, const char *path, int nkeep, const float 
                  const float *xt2, const void *vecs, int dtype, int on_device);
 
 /* Build and upload a system on the basis recorded in a .wfn file (orbits, Z,
- * N, 2Jz from its header) with the interaction intf and its scaling line.
+ * N, 2Jz from its header) with the interaction intf and its scaling line
+ * (intf = NULL: no interaction, H = 0, for operators applied to the states).
  * Fails when the basis rebuilt from those fields does not reproduce the
  * file's header byte for byte (a parity, W or ordering this engine does not
  * build).  Replaces wfn_ropen_file + read_wfn_header (src/bwfnlib.f90:1619-

@@ -19,7 +19,10 @@
  *   - pair lists and their sort      src/btbme.f90:208-308,324-423,548-703
  *   - FP32 m-scheme MEs              src/btbme.f90:1209-1558 (uncoupleXXtbme),
  *                                    src/btbmepn.f90:177-594 (count_uncouplepn)
- * and then applies H to a vector by brute-force second quantization over the
+ *   - one-body operators (.opme)     src/bdenslib1.f90:1823-2071 (readin1bodyop),
+ *                                    :2444-2597 (decouple1bodyop)
+ * and then applies H (or a one-body operator) to a vector by brute-force
+ * second quantization over the
  * basis (a different algorithm from the reference's factorized jumps,
  * src/bapplyhlibOMP.f90:979-1450), accumulating in FP64 with the FP32 matrix
  * elements promoted, which is what the reference computes with lanc_prec=8.
@@ -1221,4 +1224,141 @@ int64_t bso_port_apply(const bso_port_t *p, const float *x, float *y, int64_t lo
     total += upd;
   }
   return total;
+}
+
+/* ------------------------------------------------------- one-body operators */
+/* readin1bodyop (src/bdenslib1.f90:1823-2071): .opme file, first line iso /
+ * pns / xpn, number of orbits, the orbit list (checked against the space),
+ * "J T" (iso) or "J" (pns, xpn), then "a b value" (iso, xpn) or "a b vp vn"
+ * (pns) lines to the end of the file (at most 10000); xpn lines that connect
+ * a proton and a neutron orbit are skipped.  decouple1bodyop
+ * (src/bdenslib1.f90:2444-2597): the M = 0 component in the m-scheme, in
+ * float arithmetic in the order written there.
+ * op_p, op_n: nsps x nsps, [a * nsps + b] is the coefficient of a+_a a_b.
+ * Returns 0, or a negative code: -1 file, -2 format, -3 orbit mismatch,
+ * -4 angular momentum out of range. */
+int bso_opme(const bso_t *o, const char *path, double *op_p, double *op_n, int *jop_out, int *top_out) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char line[512];
+  if (!fgets(line, sizeof line, f)) { fclose(f); return -2; }
+  int pn = 0, xpn = 0;
+  if (!strncmp(line, "iso", 3)) pn = 0;
+  else if (!strncmp(line, "pns", 3)) pn = 1;
+  else if (!strncmp(line, "xpn", 3)) { pn = 1; xpn = 1; }
+  else { fclose(f); return -2; }
+  int n;
+  if (fscanf(f, "%d", &n) != 1 || n != o->norb) { fclose(f); return -3; }
+  for (int i = 1; i <= n; i++) {
+    int i1, i2, nr, l;
+    float xj;
+    if (xpn) {
+      if (fscanf(f, "%d %d %d %d %f", &i1, &i2, &nr, &l, &xj) != 5 || i2 != i + n) { fclose(f); return -3; }
+    } else if (fscanf(f, "%d %d %d %f", &i1, &nr, &l, &xj) != 4) { fclose(f); return -3; }
+    if (i1 != i || nr != o->on[i - 1] || l != o->ol[i - 1] || (int)lrintf(2.0f * xj) != o->oj2[i - 1]) {
+      fclose(f);
+      return -3;
+    }
+  }
+  int jop, top = 2;
+  if (fscanf(f, "%d", &jop) != 1) { fclose(f); return -2; }
+  if (!pn && fscanf(f, "%d", &top) != 1) { fclose(f); return -2; }
+  float op[MAXORB][MAXORB], pop[MAXORB][MAXORB], nop[MAXORB][MAXORB];
+  memset(op, 0, sizeof op);
+  memset(pop, 0, sizeof pop);
+  memset(nop, 0, sizeof nop);
+  for (int k = 0; k < 10000; k++) {
+    int a, b;
+    float x = 0.0f, y = 0.0f;
+    if (pn && !xpn) {
+      if (fscanf(f, "%d %d %f %f", &a, &b, &x, &y) != 4) break;
+    } else if (fscanf(f, "%d %d %f", &a, &b, &x) != 3) {
+      break;
+    }
+    int pv = 1, nv = 1;
+    if (xpn) {
+      if ((a > n) != (b > n)) continue; /* charge changing: skipped */
+      if (a > n) { a -= n; b -= n; y = x; x = 0.0f; pv = 0; }
+      else { y = 0.0f; nv = 0; }
+    }
+    if (a < 1 || a > n || b < 1 || b > n) { fclose(f); return -2; }
+    int ja = o->oj2[a - 1], jb = o->oj2[b - 1];
+    if (ja + jb < 2 * jop || abs(ja - jb) > 2 * jop) { fclose(f); return -4; }
+    if (pn) {
+      if (o->Z > 0 && pv) pop[a - 1][b - 1] = x;
+      if (o->N > 0 && nv) nop[a - 1][b - 1] = y;
+    } else {
+      op[a - 1][b - 1] = x;
+    }
+  }
+  fclose(f);
+  const int ns = o->nsps;
+  for (int i = 0; i < ns * ns; i++) op_p[i] = op_n[i] = 0.0;
+  for (int it = 0; it < 2; it++) {
+    double *out = it == 0 ? op_p : op_n;
+    for (int sa = 0; sa < ns; sa++)
+      for (int sb = 0; sb < ns; sb++) {
+        const int a = o->sorb[sa], b = o->sorb[sb];
+        const int ja = o->sj2[sa], ma = o->sm2[sa], jb = o->sj2[sb], mb = o->sm2[sb];
+        const float v = pn ? (it == 0 ? pop[a][b] : nop[a][b]) : op[a][b];
+        if (pn && (it == 0 ? o->Z : o->N) == 0) continue;
+        if (v == 0.0f || ma != mb) continue;
+        if (jop > (ja + jb) / 2 || jop < abs(ja - jb) / 2) continue;
+        int iphase = ipow_m1((jb - mb) / 2);
+        float r;
+        if (pn) {
+          r = (float)iphase * bso_cleb(ja, ma, jb, -mb, 2 * jop, ma - mb) * v / sqrtf(2.0f * jop + 1.0f);
+        } else {
+          if (it == 1) iphase = -iphase;
+          const float ct = it == 0 ? bso_cleb(1, 1, 1, -1, 2 * top, 0) : bso_cleb(1, -1, 1, 1, 2 * top, 0);
+          r = (float)iphase * bso_cleb(ja, ma, jb, -mb, 2 * jop, ma - mb) * v / sqrtf(2.0f * top + 1.0f) /
+              sqrtf(2.0f * jop + 1.0f) * ct;
+        }
+        out[sa * ns + sb] = r;
+      }
+  }
+  *jop_out = jop;
+  *top_out = top;
+  return 0;
+}
+
+/* y += O x for the one-body operator sum_ab op_p[a,b] a+_a a_b (protons) +
+ * op_n (neutrons), by brute force over the basis: for every source state and
+ * every a+_a a_b that does not annihilate it (src/bdenslib3.f90:319-427 and
+ * its neutron twin apply the same operator through jumps).  FP64. */
+void bso_apply_onebody(const bso_t *o, const double *op_p, const double *op_n, const double *x, double *y) {
+  const int ns = o->nsps;
+  sdmap_t mp[2];
+  for (int it = 0; it < 2; it++) map_build(&mp[it], o->mask[it], o->nsd[it]);
+  for (int64_t ip = 0; ip < o->nsd[0]; ip++) {
+    const int sn = o->conj[0][o->sdsec[0][ip]];
+    for (int64_t in = o->secfirst[1][sn]; in < o->secfirst[1][sn] + o->seccount[1][sn]; in++) {
+      const int64_t s = o->start[0][ip] + o->start[1][in];
+      if (x[s] == 0.0) continue;
+      for (int it = 0; it < 2; it++) {
+        const uint64_t m0 = it == 0 ? o->mask[0][ip] : o->mask[1][in];
+        const double *op = it == 0 ? op_p : op_n;
+        for (uint64_t tb = m0; tb; tb &= tb - 1) {
+          const int b = __builtin_ctzll(tb);
+          const int s1 = fsign(m0, b);
+          const uint64_t m1 = m0 & ~(1ULL << b);
+          for (int a = 0; a < ns; a++) {
+            if (m1 >> a & 1) continue;
+            const double v = op[a * ns + b];
+            if (v == 0.0) continue;
+            const uint64_t m2 = m1 | (1ULL << a);
+            const int sg = s1 * fsign(m1, a);
+            int64_t t;
+            if (it == 0) t = o->start[0][map_get(&mp[0], m2)] + o->start[1][in];
+            else t = o->start[0][ip] + o->start[1][map_get(&mp[1], m2)];
+            y[t] += v * sg * x[s];
+          }
+        }
+      }
+    }
+  }
+  for (int it = 0; it < 2; it++) {
+    free(mp[it].keys);
+    free(mp[it].vals);
+  }
 }

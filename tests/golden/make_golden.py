@@ -298,6 +298,63 @@ def strength(name, menu, pivot, nkeep, niter):
         shutil.rmtree(wd)
 
 
+# one-body operators for menu o: (name, space, format, J, T, seed)
+OPME = [("sd_e2", "sd", "iso", 2, 0, 31), ("sd_gt", "sd", "iso", 1, 1, 32), ("sd_pns2", "sd", "pns", 2, 0, 33),
+        ("pf_xpn1", "pf", "xpn", 1, 0, 34)]
+# (state file, operator) pairs run through the reference's menu o
+APPLY1B = [("sd2p2n", "sd_e2"), ("sd2p2n", "sd_gt"), ("sd2p2n", "sd_pns2"), ("pf2p1n_m3", "pf_xpn1")]
+
+
+def opme(name, space, fmt, J, T, seed):
+    """<name>.opme: seeded reduced matrix elements of a rank-J one-body
+    operator in the reference's .opme layout (src/bdenslib1.f90:1823-2071);
+    the xpn file also carries a charge-changing line, which the reader skips."""
+    from bigstickpublick_amd.inputs import PF_ORBITS, SD_ORBITS
+    orbits = SD_ORBITS if space == "sd" else PF_ORBITS
+    n = len(orbits)
+    rng = np.random.default_rng(seed)
+    out = [fmt, str(n)]
+    for i, o in enumerate(orbits, 1):
+        if fmt == "xpn":
+            out.append("%d %d %d %d %.1f" % (i, i + n, o.n, o.l, o.j2 / 2))
+        else:
+            out.append("%d %d %d %.1f" % (i, o.n, o.l, o.j2 / 2))
+    out.append("%d %d" % (J, T) if fmt == "iso" else "%d" % J)
+    for a in range(1, n + 1):
+        for b in range(1, n + 1):
+            ja, jb = orbits[a - 1].j2, orbits[b - 1].j2
+            if abs(ja - jb) > 2 * J or ja + jb < 2 * J:
+                continue
+            v = [float(np.float32(x)) for x in rng.standard_normal(2)]
+            if fmt == "pns":
+                out.append("%d %d %.7g %.7g" % (a, b, v[0], v[1]))
+            elif fmt == "xpn":
+                out.append("%d %d %.7g" % (a, b, v[0]))
+                out.append("%d %d %.7g" % (a + n, b + n, v[1]))
+            else:
+                out.append("%d %d %.7g" % (a, b, v[0]))
+    if fmt == "xpn":
+        out.append("%d %d %.7g" % (1, 1 + n, 0.5))
+    with open(os.path.join(HERE, name + ".opme"), "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def apply1b(wfn_name, op):
+    """The reference's menu o (src/bdenslib1.f90:1645-1693): the one-body
+    operator <op>.opme applied to every state of <wfn_name>.wfn, kept as
+    <wfn_name>_<op>.wfn."""
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        shutil.copy(os.path.join(HERE, wfn_name + ".wfn"), wd)
+        shutil.copy(os.path.join(HERE, op + ".opme"), wd)
+        out = "%s_%s" % (wfn_name, op)
+        run_ref(wd, "o\n%s\n%s\n%s\n" % (wfn_name, out, op))
+        shutil.copy(os.path.join(wd, out + ".wfn"), os.path.join(HERE, out + ".wfn"))
+        print(out)
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -340,6 +397,12 @@ if __name__ == "__main__":
     for c in STRENGTH:
         if "strength" in want:
             strength(*c)
+    for c in OPME:
+        if "opme" in want:
+            opme(*c)
+    for c in APPLY1B:
+        if "apply1b" in want:
+            apply1b(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

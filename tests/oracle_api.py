@@ -46,6 +46,9 @@ def lib():
         L.bso_port_count.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]
         L.bso_cleb.restype = ctypes.c_float
         L.bso_cleb.argtypes = [ctypes.c_int] * 6
+        L.bso_opme.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.bso_apply_onebody.argtypes = [ctypes.c_void_p] * 5
         _lib = L
     return _lib
 
@@ -96,6 +99,25 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         lib().bso_apply(self.h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def opme(self, path):
+        """m-scheme coefficients (op_p, op_n: nsps x nsps, [a, b] multiplies
+        a+_a a_b), J and T of the one-body operator in an .opme file."""
+        n = self.nsps
+        op_p, op_n = np.zeros((n, n)), np.zeros((n, n))
+        j, t = ctypes.c_int(), ctypes.c_int()
+        rc = lib().bso_opme(self.h, str(path).encode(), op_p.ctypes.data, op_n.ctypes.data, ctypes.byref(j),
+                            ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError("oracle .opme read failed (code %d)" % rc)
+        return op_p, op_n, j.value, t.value
+
+    def apply_onebody(self, op_p, op_n, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        lib().bso_apply_onebody(self.h, np.ascontiguousarray(op_p).ctypes.data,
+                                np.ascontiguousarray(op_n).ctypes.data, x.ctypes.data, y.ctypes.data)
         return y
 
     def port(self):

@@ -232,3 +232,27 @@ def test_fortran_host_strength_matches_reference(name, menu, pivot, nkeep, niter
             assert abs(float(np.dot(st["vec"], st["vec"])) - 1.0) < 1e-5
     r = _run("%s\n%s_pivot\n%s\n%s\n1 1 1 1\nend\n%d %d\n%d\n" % (menu, name, out, name, nkeep, niter, 3), wd)
     assert r.returncode != 0 and "not in range 1:2" in r.stdout
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,op", [("sd2p2n", "sd_e2"), ("pf2p1n_m3", "pf_xpn1")])
+def test_fortran_host_menu_o_matches_reference(name, op, casedir):
+    """Menu o writes O|state> for every state of the .wfn: header, E, J,
+    T(T+1) equal to the reference's menu-o file, vectors within FP32
+    rounding."""
+    from wfn_io import read_wfn
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    wd = os.path.join(casedir, "fo_%s_%s" % (name, op))
+    os.makedirs(wd, exist_ok=True)
+    shutil.copy(os.path.join(gold, name + ".wfn"), wd)
+    shutil.copy(os.path.join(gold, op + ".opme"), wd)
+    out = "%s_%s" % (name, op)
+    r = _run("o\n%s\n%s\n%s\n" % (name, out, op), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_wfn(os.path.join(wd, out + ".wfn"))
+    ref = read_wfn(os.path.join(gold, out + ".wfn"))
+    assert got["header"] == ref["header"] and got["nkeep"] == ref["nkeep"] and got["end"] == got["size"]
+    for g, f in zip(got["states"], ref["states"]):
+        assert (g["index"], g["e"], g["xj"], g["xt2"]) == (f["index"], f["e"], f["xj"], f["xt2"])
+        assert np.abs(g["vec"] - f["vec"]).max() <= 1e-6 * np.abs(f["vec"]).max()

@@ -1,0 +1,125 @@
+"""One-body operators (.opme, menu o): the oracle pinned to the reference's
+own menu-o output and to closed forms; the engine's reader refusals on the
+CPU; the GPU apply against the oracle and the reference."""
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from bigstickpublick_amd.inputs import small_case, write_case
+from oracle_api import Oracle
+from wfn_io import read_wfn
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = {"sd2p2n": ("sd", 2, 2, 0, 77), "pf2p1n_m3": ("pf", 2, 1, 3, 83)}
+APPLY = [("sd2p2n", "sd_e2", 2, 0), ("sd2p2n", "sd_gt", 1, 1), ("sd2p2n", "sd_pns2", 2, 2),
+         ("pf2p1n_m3", "pf_xpn1", 1, 2)]
+
+
+def _system_files(name, d):
+    sp, Z, N, jz2, seed = CASES[name]
+    case = small_case(sp, Z, N, jz2, seed=seed)
+    case.name = name
+    write_case(case, d)
+    return os.path.join(d, sp + ".sps"), os.path.join(d, name + ".int"), Z, N, jz2
+
+
+def _oracle(name, d):
+    sps, intf, Z, N, jz2 = _system_files(name, d)
+    return Oracle(sps, intf, Z, N, jz2)
+
+
+@pytest.mark.parametrize("name,op,J,T", APPLY)
+def test_oracle_onebody_matches_reference_menu_o(name, op, J, T, tmp_path):
+    """The oracle's .opme reading, decoupling and brute-force apply reproduce
+    the reference's menu-o output (FP32 vectors) on every state."""
+    o = _oracle(name, str(tmp_path))
+    op_p, op_n, j, t = o.opme(os.path.join(GOLD, op + ".opme"))
+    assert (j, t) == (J, T)
+    win = read_wfn(os.path.join(GOLD, name + ".wfn"))
+    wout = read_wfn(os.path.join(GOLD, "%s_%s.wfn" % (name, op)))
+    assert wout["header"] == win["header"] and wout["nkeep"] == win["nkeep"]
+    for si, so in zip(win["states"], wout["states"]):
+        assert (so["e"], so["xj"], so["xt2"]) == (si["e"], si["xj"], si["xt2"])
+        y = o.apply_onebody(op_p, op_n, si["vec"].astype(np.float64))
+        assert np.abs(y - so["vec"]).max() <= 5e-7 * np.abs(so["vec"]).max()
+
+
+def _number_opme(path, orbits, T):
+    """J = 0 operator with reduced elements sqrt(2j+1) sqrt(2T+1) / <1/2 1/2 1/2 -1/2 | T 0>
+    on the diagonal: in the m-scheme every a+_a a_a gets 1 for protons and
+    (-1)^T for neutrons, so O = Z + N (T = 0) or Z - N (T = 1)."""
+    cg = 1.0 / np.sqrt(2.0)
+    lines = ["iso", str(len(orbits))]
+    lines += ["%d %d %d %.1f" % (i, o.n, o.l, o.j2 / 2) for i, o in enumerate(orbits, 1)]
+    lines.append("0 %d" % T)
+    for i, o in enumerate(orbits, 1):
+        lines.append("%d %d %.9g" % (i, i, np.sqrt(o.j2 + 1.0) * np.sqrt(2 * T + 1.0) / cg))
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("name", ["sd2p2n", "pf2p1n_m3"])
+@pytest.mark.parametrize("T", [0, 1])
+def test_oracle_number_operators(name, T, tmp_path):
+    from bigstickpublick_amd.inputs import PF_ORBITS, SD_ORBITS
+    sp, Z, N, jz2, _ = CASES[name]
+    o = _oracle(name, str(tmp_path))
+    p = str(tmp_path / "num.opme")
+    _number_opme(p, SD_ORBITS if sp == "sd" else PF_ORBITS, T)
+    op_p, op_n, j, t = o.opme(p)
+    x = np.random.default_rng(4).standard_normal(o.dim)
+    want = (Z + N if T == 0 else Z - N) * x
+    # FP32 coefficients: 1 to within an FP32 ulp
+    assert np.abs(o.apply_onebody(op_p, op_n, x) - want).max() < 1e-6 * (Z + N) * np.abs(x).max()
+
+
+def test_engine_opme_refusals(tmp_path):
+    """The engine's .opme reader refuses what the reference stops on, before
+    any device work."""
+    import ctypes
+    from bigstickpublick_amd.engine import BS_F32, System, load_library
+    L = load_library()
+    sps, intf, Z, N, jz2 = _system_files("sd2p2n", str(tmp_path))
+    s = System(sps, intf, Z, N, jz2)
+    good = open(os.path.join(GOLD, "sd_e2.opme")).read().splitlines()
+    x = np.zeros(s.dim, np.float32)
+
+    def refused(lines, msg):
+        p = str(tmp_path / "bad.opme")
+        open(p, "w").write("\n".join(lines) + "\n")
+        rc = L.bs_apply_onebody(s.h, p.encode(), x.ctypes.data, x.ctypes.data, 1, BS_F32, None, None)
+        assert rc != 0 and msg in L.bs_last_error().decode(), L.bs_last_error()
+
+    refused(["abc"] + good[1:], "first line")
+    refused(good[:1] + ["4"] + good[2:], "single particle orbits")
+    refused(good[:3] + ["2 1 0 1.5"] + good[4:], "mismatch in orbits")
+    refused(good[:6] + ["2 2 0.5"], "angular momentum")
+    rc = L.bs_apply_onebody(s.h, str(tmp_path / "none.opme").encode(), x.ctypes.data, x.ctypes.data, 1,
+                            BS_F32, None, None)
+    assert rc != 0 and b"cannot open" in L.bs_last_error()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,op,J,T", APPLY)
+def test_onebody_apply_matches_oracle_and_reference(name, op, J, T, tmp_path):
+    """GPU apply through a system built from the .wfn alone (no interaction):
+    FP64 equal to the oracle to rounding, FP32 within FP32 rounding; FP32 also
+    against the reference's menu-o vectors."""
+    import torch
+    from bigstickpublick_amd.engine import System
+    o = _oracle(name, str(tmp_path))
+    op_p, op_n, _, _ = o.opme(os.path.join(GOLD, op + ".opme"))
+    wfn = os.path.join(GOLD, name + ".wfn")
+    s = System.from_wfn(wfn)
+    e, j, t, v = s.read_wfn(wfn, dtype="f64", device=True)
+    y, jj, tt = s.apply_onebody(os.path.join(GOLD, op + ".opme"), v)
+    assert (jj, tt) == (J, T)
+    vh = v.cpu().numpy()
+    want = np.stack([o.apply_onebody(op_p, op_n, vh[k]) for k in range(len(vh))])
+    assert np.abs(y.cpu().numpy() - want).max() <= 1e-12 * np.abs(want).max()
+    y32, _, _ = s.apply_onebody(os.path.join(GOLD, op + ".opme"), v.float())
+    assert np.abs(y32.cpu().numpy() - want).max() <= 1e-6 * np.abs(want).max()
+    ref = np.stack([st["vec"] for st in read_wfn(os.path.join(GOLD, "%s_%s.wfn" % (name, op)))["states"]])
+    assert np.abs(y32.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
