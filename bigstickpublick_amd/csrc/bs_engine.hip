@@ -145,8 +145,10 @@ constexpr int kWbRows = 128;                    // 2 * (max class size + 1) <= 1
 constexpr int kWbBytes = kWbRows * (int)sizeof(float);
 constexpr int kMaxCols = 4096;                  // longest neutron sector (staged row length)
 
-template <typename T, int BLOCK, int CPT, int R, int G>
-__global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __restrict__ x,
+// PF: the PN source rows of the next batch are loaded into registers while
+// the current batch is computed from LDS (register double buffering).
+template <typename T, int BLOCK, int CPT, int R, int G, bool PF>
+__global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb, const T* __restrict__ x,
                                                        T* __restrict__ y, int accumulate) {
   constexpr int S = R > G ? R : G;
   constexpr int XS = kWbBytes * G;  // byte offset of the staged rows
@@ -288,6 +290,27 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
     int Eq[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; q++) Eq[q] = ch[q * (BLOCK / 64)];
+    // PF: columns tid + q*BLOCK (< Li <= CPT*BLOCK) of the G source rows of a
+    // batch, loaded one batch ahead
+    VecG<T, G> pre[PF ? CPT : 1];
+    if constexpr (PF) {
+      int fb = 0, fe = 0;
+#pragma unroll
+      for (int r = R - 1; r >= 0; r--)
+        if (p1[r] > p0[r]) {
+          fb = p0[r];
+          fe = p1[r];
+        }
+      int64_t bi[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) bi[g] = g < fe - fb ? tb.pstart[tb.p1_src[fb + g]] : -1;
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * BLOCK;
+#pragma unroll
+        for (int g = 0; g < G; g++) pre[q].v[g] = (c < Li && bi[g] >= 0) ? x[bi[g] + c] : T(0);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
       for (int b0 = p0[r]; b0 < p1[r]; b0 += G) {
@@ -309,7 +332,13 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           wreg[u] = w;
         }
         __syncthreads();
-        {
+        if constexpr (PF) {
+#pragma unroll
+          for (int q = 0; q < CPT; q++) {
+            const int c = tid + q * BLOCK;
+            if (c < Li) *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = pre[q];
+          }
+        } else {
           // each thread gathers one column of all G source rows (G coalesced
           // loads) and writes it with one wide LDS store: consecutive lanes
           // write consecutive LDS words, so the stores do not conflict
@@ -329,6 +358,33 @@ __global__ __launch_bounds__(BLOCK) void k_apply_tiles(DevTables tb, const T* __
           if (i < 2 * (wsd + 1) * G) wb[i] = wreg[u];
         }
         __syncthreads();
+        if constexpr (PF) {
+          // the next batch: the rest of this row, else the first later row
+          // with entries in this class; its loads overlap this batch's work
+          int nb0 = 0, ne = 0;
+          if (b0 + G < p1[r]) {
+            nb0 = b0 + G;
+            ne = p1[r];
+          } else {
+#pragma unroll
+            for (int rr = R - 1; rr > r; rr--)
+              if (p1[rr] > p0[rr]) {
+                nb0 = p0[rr];
+                ne = p1[rr];
+              }
+          }
+          if (ne > nb0) {
+            int64_t bi[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) bi[g] = g < ne - nb0 ? tb.pstart[tb.p1_src[nb0 + g]] : -1;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+              const int c = tid + q * BLOCK;
+#pragma unroll
+              for (int g = 0; g < G; g++) pre[q].v[g] = (c < Li && bi[g] >= 0) ? x[bi[g] + c] : T(0);
+            }
+          }
+        }
         // column-major over the thread's columns: each (wave, q) runs to its
         // own chunk's entry count (wave-uniform), four entries per step with
         // their loads in flight together
@@ -1588,7 +1644,9 @@ void upload_tables(bs_system* s) {
 
 // Kernel variants: (R rows per tile) x (G staged source rows).
 constexpr int kNumShapes = 7;
-constexpr int kNumVariants = 2 * kNumShapes;  // (R, G) shape x {identity, XCD} map
+// (R, G) shape x {identity, XCD} map x {staged after, loaded ahead (PF)}:
+// variant v = pf * 2 * kNumShapes + map * kNumShapes + shape
+constexpr int kNumVariants = 4 * kNumShapes;
 constexpr int kVarR[kNumShapes] = {1, 1, 2, 2, 2, 4, 4};
 constexpr int kVarG[kNumShapes] = {1, 2, 1, 2, 4, 2, 4};
 
@@ -1603,13 +1661,14 @@ size_t lds_bytes(const bs_system* s, int v);
 constexpr size_t kLdsLimit = 160 * 1024;
 
 template <typename T, int CPT, int R, int G>
-void launch_variant(bs_system* s, int map, const T* x, T* y, int accumulate, hipStream_t st) {
+void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumulate, hipStream_t st) {
   const int ri = R == 1 ? 0 : (R == 2 ? 1 : 2);
   DevTables d = s->dt;
   d.tile_f0 = s->tile_f0[ri].p;
   d.tile_nr = s->tile_nr[ri].p;
   d.tile_map = s->tile_map[ri][map].p;
-  auto fn = k_apply_tiles<T, kApplyBlock, CPT, R, G>;  // static LDS (lds_bytes_rg)
+  // static LDS (lds_bytes_rg)
+  auto fn = pf ? k_apply_tiles<T, kApplyBlock, CPT, R, G, true> : k_apply_tiles<T, kApplyBlock, CPT, R, G, false>;
   hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), 0, st, d, x, y, accumulate);
   HIP_OK(hipGetLastError());
 }
@@ -1621,15 +1680,16 @@ size_t lds_bytes(const bs_system* s, int v) {
 
 template <typename T, int CPT>
 void launch_cpt(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
-  const int m = v / kNumShapes;
+  const int m = (v / kNumShapes) % 2;
+  const bool pf = v >= 2 * kNumShapes;
   switch (v % kNumShapes) {
-    case 0: launch_variant<T, CPT, 1, 1>(s, m, x, y, accumulate, st); break;
-    case 1: launch_variant<T, CPT, 1, 2>(s, m, x, y, accumulate, st); break;
-    case 2: launch_variant<T, CPT, 2, 1>(s, m, x, y, accumulate, st); break;
-    case 3: launch_variant<T, CPT, 2, 2>(s, m, x, y, accumulate, st); break;
-    case 4: launch_variant<T, CPT, 2, 4>(s, m, x, y, accumulate, st); break;
-    case 5: launch_variant<T, CPT, 4, 2>(s, m, x, y, accumulate, st); break;
-    default: launch_variant<T, CPT, 4, 4>(s, m, x, y, accumulate, st); break;
+    case 0: launch_variant<T, CPT, 1, 1>(s, m, pf, x, y, accumulate, st); break;
+    case 1: launch_variant<T, CPT, 1, 2>(s, m, pf, x, y, accumulate, st); break;
+    case 2: launch_variant<T, CPT, 2, 1>(s, m, pf, x, y, accumulate, st); break;
+    case 3: launch_variant<T, CPT, 2, 2>(s, m, pf, x, y, accumulate, st); break;
+    case 4: launch_variant<T, CPT, 2, 4>(s, m, pf, x, y, accumulate, st); break;
+    case 5: launch_variant<T, CPT, 4, 2>(s, m, pf, x, y, accumulate, st); break;
+    default: launch_variant<T, CPT, 4, 4>(s, m, pf, x, y, accumulate, st); break;
   }
 }
 
@@ -1685,7 +1745,7 @@ void launch_apply(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st
   const int di = sizeof(T) == 4 ? 0 : 1;
   if (s->variant[di] < 0) {
     // default: the largest tile that fits, until autotuned
-    int v = kNumVariants - 1;
+    int v = 2 * kNumShapes - 1;
     while (v > 0 && lds_bytes<T>(s, v) > kLdsLimit) v--;
     s->variant[di] = v;
   }
@@ -2757,11 +2817,12 @@ int bs_num_variants(int* n) {
   return 0;
 }
 
-int bs_variant_info(int v, int* rows, int* staged, int* xcd_map) {
+int bs_variant_info(int v, int* rows, int* staged, int* xcd_map, int* prefetch) {
   if (v < 0 || v >= kNumVariants) return set_err("bad variant");
   if (rows) *rows = kVarR[v % kNumShapes];
   if (staged) *staged = kVarG[v % kNumShapes];
-  if (xcd_map) *xcd_map = v / kNumShapes;
+  if (xcd_map) *xcd_map = (v / kNumShapes) % 2;
+  if (prefetch) *prefetch = v >= 2 * kNumShapes;
   return 0;
 }
 

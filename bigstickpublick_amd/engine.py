@@ -46,7 +46,7 @@ def load_library():
     L.bs_op_counts.argtypes = [vp, vp]
     L.bs_apply_h.argtypes = [vp, vp, vp, i32, i32, vp]
     L.bs_num_variants.argtypes = [ctypes.POINTER(i32)]
-    L.bs_variant_info.argtypes = [i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.bs_variant_info.argtypes = [i32] + [ctypes.POINTER(i32)] * 4
     L.bs_set_variant.argtypes = [vp, i32, i32]
     L.bs_get_variant.argtypes = [vp, i32, ctypes.POINTER(i32)]
     L.bs_autotune.argtypes = [vp, i32, ctypes.POINTER(i32)]
@@ -207,9 +207,9 @@ class System:
         _check(L.bs_num_variants(ctypes.byref(n)))
         out = []
         for v in range(n.value):
-            r, g, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-            _check(L.bs_variant_info(v, ctypes.byref(r), ctypes.byref(g), ctypes.byref(m)))
-            out.append((r.value, g.value, m.value))
+            r, g, m, pf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _check(L.bs_variant_info(v, ctypes.byref(r), ctypes.byref(g), ctypes.byref(m), ctypes.byref(pf)))
+            out.append((r.value, g.value, m.value, pf.value))
         return out
 
     def set_variant(self, dtype, v):

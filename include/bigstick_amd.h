@@ -86,15 +86,16 @@ int bs_op_counts(const bs_system *sys, int64_t out[8]);
 int bs_apply_h(bs_system *sys, const void *x, void *y, int dtype, int accumulate, void *stream);
 
 /* Kernel variants of the H x kernel: a tile of `rows` proton rows per
- * workgroup, `staged` source rows held in LDS at a time, and the
- * workgroup-to-tile order (xcd_map 0: sector order; 1: the tile list cut into
- * 8 contiguous chunks of equal work, chunk j on the workgroups that the
- * dispatcher places on XCD j, so each XCD's L2 holds its sectors' tables and
- * rows).  bs_autotune times every variant that fits in LDS on this system and
- * keeps the fastest for that dtype; bs_set_variant forces one (-1 = default
+ * workgroup, `staged` source rows held in LDS at a time, the
+ * workgroup-to-tile order (xcd_map 0: sector order; 1: runs of consecutive
+ * tiles on the workgroups that the dispatcher places on one XCD, so each
+ * XCD's L2 holds their shared source rows), and prefetch (1: the next batch
+ * of source rows is loaded into registers while the current one is applied).
+ * bs_autotune times every variant that fits in LDS on this system and keeps
+ * the fastest for that dtype; bs_set_variant forces one (-1 = default
  * choice). */
 int bs_num_variants(int *n);
-int bs_variant_info(int variant, int *rows, int *staged, int *xcd_map);
+int bs_variant_info(int variant, int *rows, int *staged, int *xcd_map, int *prefetch);
 int bs_set_variant(bs_system *sys, int dtype, int variant);
 int bs_get_variant(const bs_system *sys, int dtype, int *variant);
 int bs_autotune(bs_system *sys, int dtype, int *variant);
