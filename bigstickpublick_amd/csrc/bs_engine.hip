@@ -568,6 +568,18 @@ __global__ void k_normalize(T* __restrict__ v, const T* __restrict__ w, const do
     v[i] = (T)((double)w[i] * m);
 }
 
+// seeded values in [-1, 1): SplitMix64 (Steele, Lea, Flood 2014) of the index
+template <typename T>
+__global__ void k_pivot_random(T* __restrict__ v, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)i * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    v[i] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+  }
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ v, T val, int64_t D) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x)
@@ -635,6 +647,115 @@ template <typename T, typename U>
 __global__ void k_convert(U* __restrict__ out, const T* __restrict__ in, int64_t D) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = (U)in[i];
+}
+
+// ------------------------------------------------------------ block Lanczos
+// Block vectors are interleaved like the multi-vector apply: element k of
+// basis state s of block i at V[i*D*K + s*K + k].
+
+// part[b * (m*K*K) + (i*K + k1)*K + k2] = workgroup b's share of
+// sum_s V_i[s, k1] * W[s, k2]  (blockIdx.y = i), FP64.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_block_dots(const T* __restrict__ V, int64_t D, int m,
+                                                        const T* __restrict__ W, double* __restrict__ part) {
+  const int i = blockIdx.y;
+  const T* Vi = V + (int64_t)i * D * K;
+  double acc[K][K];
+#pragma unroll
+  for (int a = 0; a < K; a++)
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[a][c] = 0.0;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < D; s += (int64_t)gridDim.x * blockDim.x) {
+    double v[K], w[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+      v[a] = (double)Vi[s * K + a];
+      w[a] = (double)W[s * K + a];
+    }
+#pragma unroll
+    for (int a = 0; a < K; a++)
+#pragma unroll
+      for (int c = 0; c < K; c++) acc[a][c] += v[a] * w[c];
+  }
+  __shared__ double red[kBlock / 64];
+  double* out = part + (int64_t)blockIdx.x * m * K * K + (int64_t)i * K * K;
+#pragma unroll
+  for (int a = 0; a < K; a++)
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      double x = acc[a][c];
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kBlock / 64; q++) t += red[q];
+        out[a * K + c] = t;
+      }
+      __syncthreads();
+    }
+}
+
+// W[s, k2] -= T(sum_{i, k1} V_i[s, k1] * C[(i*K + k1)*K + k2])  (block CGS)
+template <typename T, int K>
+__global__ void k_block_update(T* __restrict__ W, const T* __restrict__ V, int64_t D, int m,
+                               const double* __restrict__ C) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < D; s += (int64_t)gridDim.x * blockDim.x) {
+    double acc[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[c] = 0.0;
+    for (int i = 0; i < m; i++) {
+      const T* Vi = V + (int64_t)i * D * K + s * K;
+#pragma unroll
+      for (int a = 0; a < K; a++) {
+        const double v = (double)Vi[a];
+        const double* Cr = C + ((int64_t)i * K + a) * K;
+#pragma unroll
+        for (int c = 0; c < K; c++) acc[c] += v * Cr[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < K; c++) W[s * K + c] = W[s * K + c] - (T)acc[c];
+  }
+}
+
+// dst[s, c] = T(sum_a src[s, a] * M[a*K + c])  (block orthonormalization)
+template <typename T, int K>
+__global__ void k_block_transform(T* __restrict__ dst, const T* __restrict__ src, int64_t D,
+                                  const double* __restrict__ M) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < D; s += (int64_t)gridDim.x * blockDim.x) {
+    double v[K];
+#pragma unroll
+    for (int a = 0; a < K; a++) v[a] = (double)src[s * K + a];
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+      double t = 0.0;
+#pragma unroll
+      for (int a = 0; a < K; a++) t += v[a] * M[a * K + c];
+      dst[s * K + c] = (T)t;
+    }
+  }
+}
+
+// out_q[s] = sum_{c < n} col_c[s] * Y[c*nq + q], col_{iK+k} = V_i[., k]; FP64
+// output vectors at stride D (Ritz vectors of the block basis)
+template <typename T, int K>
+__global__ void k_block_combine(double* __restrict__ out, const T* __restrict__ V, int64_t D, int n,
+                                const double* __restrict__ Y, int nq) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < D; s += (int64_t)gridDim.x * blockDim.x) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.0;
+    for (int c = 0; c < n; c++) {
+      const double v = (double)V[(int64_t)(c / K) * D * K + s * K + c % K];
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (q < nq) acc[q] += v * Y[(int64_t)c * nq + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (q < nq) out[(int64_t)q * D + s] = acc[q];
+  }
 }
 
 // ------------------------------------------------------- adjoint (densities)
@@ -2363,6 +2484,196 @@ BatchVals shared_vals(const bs_system* s, int K) {
   return v;
 }
 
+// Block Lanczos (lanczos_block, src/blanczosmain.f90:1804-2153): blocks of K
+// vectors, H applied to a block with the multi-vector kernel, full block
+// classical Gram-Schmidt against every stored block (FP64 sums, one summed
+// subtraction), each new block orthonormalized by the spectral decomposition
+// of its Gram matrix (spectral_decomp in the reference), Ritz pairs from the
+// block tridiagonal matrix, then the same FP64 normalization / Rayleigh
+// quotient / residual as run_lanczos.  The pivot block is seeded
+// pseudo-random (the reference draws it with random_number).
+template <typename T, int K>
+int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, void* evecs, double* resid,
+                      int* iters) {
+  const int64_t D = s->t.dim;
+  const hipStream_t st = nullptr;
+  const int nkeep = p->nkeep;
+  if (nkeep < 1 || nkeep > 16) return set_err("nkeep must be in 1..16");
+  if (p->thick != 0) return set_err("block Lanczos runs fixed block iterations (thick must be 0)");
+  if (D < K) return set_err("block larger than the basis");
+  int nblk = (int)(std::min<int64_t>(p->niter, D) / K);
+  if (nblk * K < nkeep) return set_err("niter must give at least nkeep basis vectors (niter >= nkeep, multiple of block)");
+  const int64_t bsz = D * K;
+  const int g = grid_for(D);
+  const int nbx = std::min(g, 512);
+  DevBuf<T> V, W;
+  DevBuf<double> part, C, M;
+  V.alloc((size_t)nblk * bsz);
+  W.alloc(bsz);
+  part.alloc((size_t)nbx * nblk * K * K);
+  C.alloc((size_t)nblk * K * K);
+  M.alloc(K * K);
+  const BatchVals bv = shared_vals(s, K);
+  auto gram = [&](const T* X, std::vector<double>& S) {
+    k_block_dots<T, K><<<dim3(nbx, 1), kBlock, 0, st>>>(X, D, 1, X, part.p);
+    k_reduce<<<K * K, kBlock, 0, st>>>(part.p, nbx, K * K, C.p, 0);
+    HIP_OK(hipGetLastError());
+    S.assign(K * K, 0.0);
+    HIP_OK(hipMemcpy(S.data(), C.p, sizeof(double) * K * K, hipMemcpyDeviceToHost));
+  };
+  // X := X U diag(lambda)^(-1/2); returns B = diag(lambda)^(1/2) U^T (X_old = X_new B).
+  // False when a direction of X is at the rounding level of H (relative to
+  // floor): the stored span is then invariant to working precision.
+  double hnorm = 0.0;
+  const double eps = sizeof(T) == 4 ? 1.2e-7 : 2.3e-16;
+  auto orthonormalize = [&](T* dst, const T* src, std::vector<double>& B, double floor) -> bool {
+    std::vector<double> S, lam, U;
+    gram(src, S);
+    for (int a = 0; a < K; a++)
+      for (int c = 0; c < a; c++) S[a * K + c] = S[c * K + a] = 0.5 * (S[a * K + c] + S[c * K + a]);
+    jacobi_eigen(K, S, lam, U);
+    if (!(lam[0] > 1e-28 * std::max(lam[K - 1], 1e-300))) return false;  // the block lost rank
+    if (!(std::sqrt(lam[0]) > floor)) return false;
+    std::vector<double> m(K * K);
+    B.assign(K * K, 0.0);
+    for (int a = 0; a < K; a++)
+      for (int c = 0; c < K; c++) {
+        m[a * K + c] = U[a * K + c] / std::sqrt(lam[c]);
+        B[c * K + a] = std::sqrt(lam[c]) * U[a * K + c];
+      }
+    HIP_OK(hipMemcpy(M.p, m.data(), sizeof(double) * K * K, hipMemcpyHostToDevice));
+    k_block_transform<T, K><<<g, kBlock, 0, st>>>(dst, src, D, M.p);
+    HIP_OK(hipGetLastError());
+    return true;
+  };
+  k_pivot_random<T><<<g, kBlock, 0, st>>>(V.p, bsz, 0x5eed1234abcdULL);
+  HIP_OK(hipGetLastError());
+  std::vector<double> B0;
+  if (!orthonormalize(V.p, V.p, B0, 0.0)) return set_err("pivot block is rank deficient");
+  std::vector<std::vector<double>> A, B;  // diagonal and sub-diagonal blocks
+  std::vector<double> e, Y, e_prev;
+  int mv = 0, nb = 0;
+  auto ritz = [&](int nbl) {
+    const int n = nbl * K;
+    std::vector<double> T_((size_t)n * n, 0.0);
+    for (int j = 0; j < nbl; j++) {
+      for (int a = 0; a < K; a++)
+        for (int c = 0; c < K; c++) T_[(size_t)(j * K + a) * n + j * K + c] = A[j][a * K + c];
+      if (j + 1 < nbl)
+        for (int a = 0; a < K; a++)
+          for (int c = 0; c < K; c++) {
+            T_[(size_t)((j + 1) * K + a) * n + j * K + c] = B[j][a * K + c];
+            T_[(size_t)(j * K + c) * n + (j + 1) * K + a] = B[j][a * K + c];
+          }
+    }
+    jacobi_eigen(n, T_, e, Y);
+  };
+  for (int j = 0; j < nblk; j++) {
+    T* Vj = V.p + (int64_t)j * bsz;
+    launch_batched<T>(s, bv, Vj, W.p, st);
+    mv += K;
+    // C_i = V_i^T W for every stored block; A_j = C_j; W -= sum_i V_i C_i
+    const int m = j + 1;
+    k_block_dots<T, K><<<dim3(nbx, m), kBlock, 0, st>>>(V.p, D, m, W.p, part.p);
+    k_reduce<<<m * K * K, kBlock, 0, st>>>(part.p, nbx, m * K * K, C.p, 0);
+    HIP_OK(hipGetLastError());
+    std::vector<double> Aj(K * K);
+    HIP_OK(hipMemcpy(Aj.data(), C.p + (size_t)j * K * K, sizeof(double) * K * K, hipMemcpyDeviceToHost));
+    for (int a = 0; a < K; a++)
+      for (int c = 0; c < a; c++) Aj[a * K + c] = Aj[c * K + a] = 0.5 * (Aj[a * K + c] + Aj[c * K + a]);
+    A.push_back(Aj);
+    nb = j + 1;
+    k_block_update<T, K><<<g, kBlock, 0, st>>>(W.p, V.p, D, m, C.p);
+    // a second pass (classical Gram-Schmidt twice): once Ritz vectors converge,
+    // H V_j lies almost in the stored span and one pass leaves a remainder
+    // that is not orthogonal to it
+    k_block_dots<T, K><<<dim3(nbx, m), kBlock, 0, st>>>(V.p, D, m, W.p, part.p);
+    k_reduce<<<m * K * K, kBlock, 0, st>>>(part.p, nbx, m * K * K, C.p, 0);
+    k_block_update<T, K><<<g, kBlock, 0, st>>>(W.p, V.p, D, m, C.p);
+    HIP_OK(hipGetLastError());
+    for (int a = 0; a < K; a++) hnorm = std::max(hnorm, std::fabs(Aj[a * K + a]));
+    if (p->tol > 0.0 && nb * K >= nkeep) {
+      // (src/blanczoslib.f90:2660-2800) stop when no kept Ritz value moves by tol
+      ritz(nb);
+      if (!e_prev.empty()) {
+        double dmax = 0.0;
+        for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(e[k] - e_prev[k]));
+        if (dmax < p->tol) break;
+      }
+      e_prev.assign(e.begin(), e.begin() + nkeep);
+    }
+    if (j + 1 == nblk) break;
+    std::vector<double> Bj;
+    if (!orthonormalize(V.p + (int64_t)(j + 1) * bsz, W.p, Bj, 100.0 * eps * hnorm)) break;  // invariant span
+    B.push_back(Bj);
+  }
+  if (nb * K < nkeep) return set_err("Krylov space smaller than nkeep");
+  ritz(nb);
+  const int n = nb * K;
+  for (int k = 0; k < nkeep; k++) evals[k] = e[k];
+  if (evecs || resid || p->polish) {
+    const int nb1 = grid_for(D) < 1024 ? grid_for(D) : 1024;
+    DevBuf<double> dY, X64, HX, part1, dots;
+    std::vector<double> Yk((size_t)n * nkeep);
+    for (int c = 0; c < n; c++)
+      for (int k = 0; k < nkeep; k++) Yk[(size_t)c * nkeep + k] = Y[(size_t)c * n + k];
+    dY.upload(Yk);
+    X64.alloc((size_t)nkeep * D);
+    HX.alloc(D);
+    part1.alloc(nb1);
+    dots.alloc(4);
+    k_block_combine<T, K><<<g, kBlock, 0, st>>>(X64.p, V.p, D, n, dY.p, nkeep);
+    HIP_OK(hipGetLastError());
+    for (int k = 0; k < nkeep; k++) {
+      double* xk = X64.p + (int64_t)k * D;
+      k_multidot<double><<<nb1, kBlock, 0, st>>>(xk, D, 1, xk, D, part1.p);
+      k_reduce<<<1, kBlock, 0, st>>>(part1.p, nb1, 1, dots.p, 1);
+      k_normalize<double><<<g, kBlock, 0, st>>>(xk, xk, dots.p, D);
+      if (p->polish || resid) {
+        launch_apply<double>(s, xk, HX.p, 0, st);
+        mv++;
+        k_multidot<double><<<nb1, kBlock, 0, st>>>(xk, D, 1, HX.p, D, part1.p);
+        k_reduce<<<1, kBlock, 0, st>>>(part1.p, nb1, 1, dots.p + 1, 0);
+        double rq = 0.0;
+        HIP_OK(hipMemcpy(&rq, dots.p + 1, sizeof(double), hipMemcpyDeviceToHost));
+        if (p->polish) evals[k] = rq;
+        if (resid) {
+          k_sub_scaled<double><<<g, kBlock, 0, st>>>(HX.p, xk, dots.p + 1, D);
+          k_multidot<double><<<nb1, kBlock, 0, st>>>(HX.p, D, 1, HX.p, D, part1.p);
+          k_reduce<<<1, kBlock, 0, st>>>(part1.p, nb1, 1, dots.p + 2, 1);
+          HIP_OK(hipMemcpy(&resid[k], dots.p + 2, sizeof(double), hipMemcpyDeviceToHost));
+        }
+      }
+      if (evecs) {
+        if (p->dtype == BS_F64)
+          HIP_OK(hipMemcpyAsync((double*)evecs + (int64_t)k * D, xk, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
+        else
+          k_convert<double, float><<<g, kBlock, 0, st>>>((float*)evecs + (int64_t)k * D, xk, D);
+      }
+    }
+  }
+  HIP_OK(hipStreamSynchronize(st));
+  if (iters) *iters = mv;
+  return 0;
+}
+
+template <typename T>
+int run_block_lanczos_k(bs_system* s, const bs_lanczos_params* p, int K, double* evals, void* evecs,
+                        double* resid, int* iters) {
+  switch (K) {
+    case 1: return run_block_lanczos<T, 1>(s, p, evals, evecs, resid, iters);
+    case 2: return run_block_lanczos<T, 2>(s, p, evals, evecs, resid, iters);
+    case 3: return run_block_lanczos<T, 3>(s, p, evals, evecs, resid, iters);
+    case 4: return run_block_lanczos<T, 4>(s, p, evals, evecs, resid, iters);
+    case 5: return run_block_lanczos<T, 5>(s, p, evals, evecs, resid, iters);
+    case 6: return run_block_lanczos<T, 6>(s, p, evals, evecs, resid, iters);
+    case 7: return run_block_lanczos<T, 7>(s, p, evals, evecs, resid, iters);
+    case 8: return run_block_lanczos<T, 8>(s, p, evals, evecs, resid, iters);
+    default: return set_err("block size must be in 1..8");
+  }
+}
+
+
 // Lanczos steps i0..i1-1 for all K Hamiltonians in lockstep (FP64
 // alpha/beta and Gram-Schmidt dots per Hamiltonian), enqueued on st with no
 // host synchronisation and no memset, so that it can be captured.
@@ -3371,6 +3682,21 @@ int bs_strength(bs_system* s, const bs_lanczos_params* p, const void* pivot, dou
       return run_lanczos<double>(s, p, alpha, beta, evals, evecs, resid, iters, pr);
     }
     return set_err("dtype must be BS_F32 or BS_F64");
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_block_lanczos(bs_system* s, const bs_lanczos_params* p, int block, double* evals, void* evecs,
+                     double* resid, int* iters) {
+  try {
+    checked(s);
+    if (!p || !evals) return set_err("null argument");
+    if (block < 1 || block > 8) return set_err("block size must be in 1..8");
+    if (p->dtype != BS_F32 && p->dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (p->dtype == BS_F32) return run_block_lanczos_k<float>(s, p, block, evals, evecs, resid, iters);
+    return run_block_lanczos_k<double>(s, p, block, evals, evecs, resid, iters);
   } catch (const std::exception& e) {
     return set_err(e.what());
   }

@@ -58,6 +58,7 @@ def load_library():
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
+    L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_strength.argtypes = [vp, ctypes.POINTER(_Params), vp, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp,
                               vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
@@ -311,6 +312,26 @@ class System:
                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
+
+    def block_lanczos(self, nkeep, niter, block, dtype="f32", tol=0.0, polish=True, want_vectors=False):
+        """Block Lanczos with `block` vectors per block and a basis of niter
+        vectors (fixed, or stopped at tol); alpha/beta are not returned."""
+        import torch
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        p = _Params(nkeep, niter, 0, 0, dt, tol, int(polish))
+        ev = np.zeros(nkeep)
+        res = np.zeros(nkeep)
+        it = ctypes.c_int()
+        vec = None
+        if want_vectors:
+            vec = torch.empty((nkeep, self.dim), device="cuda",
+                              dtype=torch.float32 if dt == BS_F32 else torch.float64)
+        torch.cuda.synchronize()
+        _check(load_library().bs_block_lanczos(self.h, ctypes.byref(p), block, ev.ctypes.data,
+                                               ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
+                                               res.ctypes.data, ctypes.byref(it)))
+        return LanczosResult(ev, None, None, res, it.value, vec)
 
     def strength(self, pivot, nkeep, niter, polish=True, want_vectors=False):
         """Strength function: a fixed Lanczos run of niter steps started from

@@ -103,6 +103,15 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_lanczos
 
+    function bs_block_lanczos(sys, p, block, evals, evecs, resid, iters) bind(C, name='bs_block_lanczos') result(rc)
+      import :: c_ptr, c_int, bs_lanczos_params
+      type(c_ptr), value :: sys, evals, evecs, resid
+      type(bs_lanczos_params), intent(in) :: p
+      integer(c_int), value :: block
+      integer(c_int), intent(out) :: iters
+      integer(c_int) :: rc
+    end function bs_block_lanczos
+
     function bs_strength(sys, p, pivot, pivot_norm, ritz, strength, alpha, beta, evals, evecs, resid, iters) &
         bind(C, name='bs_strength') result(rc)
       import :: c_ptr, c_int, c_double, bs_lanczos_params

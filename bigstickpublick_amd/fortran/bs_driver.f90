@@ -3,7 +3,8 @@
 ! menu s, strength function from a pivot in a previous .wfn; menu x,
 ! expectation values in previous states; menu o, a one-body operator applied
 ! to previous states, below;
-! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td,
+! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td
+! and the block options bf/bd,
 ! src/blanczosmain.f90:380-745) and writes <name>.res (energy table in the
 ! format of src/bparticleocc.f90:256) and <name>.lcoef (alpha, beta per
 ! iteration in the format of src/blanczosmain.f90:1685).
@@ -45,7 +46,7 @@ contains
   subroutine diagonalize_run(write_wfn)
     logical, intent(in) :: write_wfn
     character(len=256) :: outname, spsname, intname, line, lopt
-    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu
+    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter
     real(c_float) :: spscale, a0, a, x
     type(c_ptr) :: sys, dvec
     integer(c_int64_t) :: dim, nbytes, esize
@@ -77,7 +78,15 @@ contains
     end if
     read(5, '(a)') lopt
     lopt = adjustl(lopt)
-    read(5, *) nkeep, niter
+    dimblock = 0
+    if (lopt(1:1) == 'b') then
+      ! block Lanczos bf / bd: nkeep, block size, block iterations; the basis
+      ! holds (iterations + 1) blocks (src/blanczosmain.f90:849-869)
+      read(5, *) nkeep, dimblock, nbiter
+      niter = nbiter * dimblock + dimblock
+    else
+      read(5, *) nkeep, niter
+    end if
     nthick = 0
     maxrest = 0
     if (lopt(1:1) == 't') then
@@ -109,8 +118,15 @@ contains
     p%polish = 1
     nbytes = int(nkeep, c_int64_t) * dim * esize
     dvec = device_alloc(nbytes)
-    rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
-    call bs_check(rc, 'bs_lanczos')
+    alpha = 0.0d0
+    beta = 0.0d0
+    if (dimblock > 0) then
+      rc = bs_block_lanczos(sys, p, dimblock, c_loc(evals), dvec, c_loc(resid), iters)
+      call bs_check(rc, 'bs_block_lanczos')
+    else
+      rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
+      call bs_check(rc, 'bs_lanczos')
+    end if
 
     open(newunit=iu, file=trim(first_word(outname)) // '.res', status='replace')
     write(iu, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
@@ -121,11 +137,13 @@ contains
     write(iu, '(a)') '  '
     call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
     close(iu)
-    open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
-    do k = 1, niter
-      write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
-    end do
-    close(iu)
+    if (dimblock == 0) then
+      open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
+      do k = 1, niter
+        write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
+      end do
+      close(iu)
+    end if
     write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', dim, ', ', iters, ' matvecs'
     do k = 1, nkeep
       write(6, '(i5,3x,f12.6,2x,es10.2)') k, evals(k), resid(k)

@@ -188,10 +188,12 @@ def write_case(case: Case, directory: str) -> Tuple[str, str]:
     return sps, case.name
 
 
-def menu_script(case: Case, option: str = "n", lanczos: str = None) -> str:
+def menu_script(case: Case, option: str = "n", lanczos: str = None, block=None) -> str:
     """stdin answers for the reference executable: menu option, output name,
     .sps, Z N, 2Jz, interaction, scaling `1 1 1 1` (no mass scaling,
-    src/binput_tbme.f90:1027-1031), `end`, then the diagonalization option."""
+    src/binput_tbme.f90:1027-1031), `end`, then the diagonalization option;
+    block = (dimblock, block iterations) for the block options bf / bd
+    (src/blanczosmain.f90:822-872)."""
     sps, intf = case.space, case.name
     lines = [option, case.name, sps, "%d %d" % (case.Z, case.N), str(case.jz2),
              intf, "1 1 1 1", "end"]
@@ -200,6 +202,9 @@ def menu_script(case: Case, option: str = "n", lanczos: str = None) -> str:
     if lanczos is None:
         lanczos = "tf" if case.thick else "lf"
     lines.append(lanczos)
+    if lanczos.startswith("b"):
+        lines.append("%d %d %d" % (case.nkeep, block[0], block[1]))
+        return "\n".join(lines) + "\n"
     lines.append("%d %d" % (case.nkeep, case.niter))
     if lanczos.startswith("t"):
         lines.append(str(case.thick))

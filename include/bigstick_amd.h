@@ -157,6 +157,19 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Block Lanczos: blocks of `block` (1..8) vectors, H applied to a block with
+ * the multi-vector kernel, full block Gram-Schmidt against every stored
+ * block (FP64 sums), each new block orthonormalized by the spectral
+ * decomposition of its Gram matrix, Ritz pairs of the block tridiagonal
+ * matrix.  p->niter is the basis size (rounded down to a multiple of block),
+ * p->thick must be 0, p->tol > 0 stops when no kept Ritz value moves by more
+ * than tol between block iterations; evals, evecs, resid, iters and the FP64
+ * polish as in bs_lanczos.  The pivot block is seeded pseudo-random.
+ * Replaces lanczos_block with option bf / bd (src/blanczosmain.f90:1804-2153,
+ * :822-872). */
+int bs_block_lanczos(bs_system *sys, const bs_lanczos_params *p, int block, double *evals, void *evecs,
+                     double *resid, int *iters);
+
 /* Strength function: bs_lanczos started from `pivot` (device, dim elements
  * in p->dtype; not modified) instead of the uniform vector, with a fixed
  * number of iterations (p->thick must be 0).  pivot_norm receives ||pivot||;

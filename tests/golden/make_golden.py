@@ -355,6 +355,27 @@ def apply1b(wfn_name, op):
         shutil.rmtree(wd)
 
 
+# block Lanczos (option bf) runs of the reference: (case, dimblock, block iterations)
+BLOCK = [("sd2p2n", 4, 60), ("pf2p2n", 3, 120)]
+
+
+def block(name, dimblock, nbiter):
+    """The reference's block-Lanczos .res (random pivot block, fixed block
+    iterations), kept as <name>_bf.res."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    case.nkeep = 5
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, menu_script(case, "n", "bf", block=(dimblock, nbiter)))
+        shutil.copy(os.path.join(wd, name + ".res"), os.path.join(HERE, name + "_bf.res"))
+        print(name, "bf", read_res(os.path.join(wd, name + ".res"))[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -403,6 +424,9 @@ if __name__ == "__main__":
     for c in APPLY1B:
         if "apply1b" in want:
             apply1b(*c)
+    for c in BLOCK:
+        if "block" in want:
+            block(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -256,3 +256,27 @@ def test_fortran_host_menu_o_matches_reference(name, op, casedir):
     for g, f in zip(got["states"], ref["states"]):
         assert (g["index"], g["e"], g["xj"], g["xt2"]) == (f["index"], f["e"], f["xj"], f["xt2"])
         assert np.abs(g["vec"] - f["vec"]).max() <= 1e-6 * np.abs(f["vec"]).max()
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,dimblock,nbiter", [("sd2p2n", 4, 60), ("pf2p2n", 3, 120)])
+def test_fortran_host_block_lanczos_matches_reference(name, dimblock, nbiter, casedir):
+    """Option bf (block Lanczos, fixed block iterations): the .res energies,
+    J and T agree with the reference's block run (both converged; the pivot
+    blocks differ: the reference draws its own random block)."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    meta = {"sd2p2n": ("sd", 2, 2, 0, 77), "pf2p2n": ("pf", 2, 2, 0, 84)}[name]
+    case = small_case(*meta[:4], seed=meta[4])
+    case.name = name
+    case.nkeep = 5
+    wd = os.path.join(casedir, "fb_" + name)
+    write_case(case, wd)
+    r = _run(menu_script(case, "ns", "bf", block=(dimblock, nbiter)), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_res(os.path.join(wd, name + ".res"))
+    ref = read_res(os.path.join(gold, name + "_bf.res"))
+    assert got.shape == ref.shape == (5, 3)
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 2e-5, (got, ref)
+    assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3

@@ -214,3 +214,40 @@ def test_strength_function_moments(dtype, tol, casedir):
         v = o.apply(v)
     # the lowest Ritz values equal bs_lanczos's from the same pivot direction
     assert np.abs(r.evals[:2] - r.ritz[:2]).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [1, 3, 4, 8])
+def test_block_lanczos_small(block, casedir):
+    """Block Lanczos on sd 2p2n (D = 640, a basis of 240 or 400 vectors,
+    past the point where the lowest states converge): the lowest five equal
+    the oracle's dense spectrum (FP64 1e-9, FP32 vectors with FP64 polish
+    1e-5), with FP64 residuals below 1e-6, and the Ritz vectors are
+    orthonormal."""
+    import torch
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    s = _sys(case, d)
+    ev = np.linalg.eigvalsh(Oracle(*paths(case, d), case.Z, case.N, case.jz2).dense())[:5]
+    n = 240 if block <= 4 else 400
+    r = s.block_lanczos(nkeep=5, niter=n, block=block, dtype="f64", want_vectors=True)
+    assert np.abs(r.evals - ev).max() < 1e-9, (r.evals, ev)
+    assert r.resid.max() < 1e-6
+    X = r.evecs.cpu().numpy()
+    assert np.abs(X @ X.T - np.eye(5)).max() < 1e-10
+    r32 = s.block_lanczos(nkeep=5, niter=n, block=block, dtype="f32")
+    assert np.abs(r32.evals - ev).max() < 1e-5
+
+
+@pytest.mark.gpu
+def test_block_lanczos_config0(casedir):
+    """sd 4p4n (D = 28,503): block 4 with 300 basis vectors gives the lowest
+    five of a 300-step single-vector FP64 run; with tol it stops earlier at
+    the same energies."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    single = s.lanczos(nkeep=5, niter=300, dtype="f64")
+    blk = s.block_lanczos(nkeep=5, niter=300, block=4, dtype="f64")
+    assert np.abs(blk.evals - single.evals).max() < 1e-7, (blk.evals, single.evals)
+    early = s.block_lanczos(nkeep=5, niter=300, block=4, dtype="f64", tol=1e-9)
+    assert s.block_lanczos(nkeep=5, niter=300, block=4, dtype="f32").evals.shape == (5,)
+    assert early.matvecs < blk.matvecs and np.abs(early.evals - single.evals).max() < 1e-6

@@ -145,3 +145,10 @@ def test_entry_points_refuse_bad_arguments(casedir, tmp_path):
         assert L.bs_strength(s.h, ctypes.byref(prm), piv, ctypes.byref(pn), None, None, None, None,
                              ev.ctypes.data, None, None, ctypes.byref(it)) != 0
         assert msg in L.bs_last_error(), L.bs_last_error()
+    # block Lanczos: block size out of range, bad dtype, no upload
+    for blk, prm, msg in ((0, _Params(2, 10, 0, 0, BS_F32, 0.0, 1), b"block size"),
+                          (9, _Params(2, 10, 0, 0, BS_F32, 0.0, 1), b"block size"),
+                          (2, _Params(2, 10, 0, 0, 5, 0.0, 1), b"dtype"),
+                          (2, _Params(2, 10, 0, 0, BS_F32, 0.0, 1), b"not uploaded")):
+        assert L.bs_block_lanczos(s.h, ctypes.byref(prm), blk, ev.ctypes.data, None, None, ctypes.byref(it)) != 0
+        assert msg in L.bs_last_error(), L.bs_last_error()
