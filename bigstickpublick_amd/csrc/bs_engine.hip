@@ -737,11 +737,11 @@ __global__ void k_block_transform(T* __restrict__ dst, const T* __restrict__ src
   }
 }
 
-// out_q[s] = sum_{c < n} col_c[s] * Y[c*nq + q], col_{iK+k} = V_i[., k]; FP64
-// output vectors at stride D (Ritz vectors of the block basis)
+// out_q[s] = sum_{c < n} col_c[s] * Y[c*ys + q] (q < nq <= 16), col_{iK+k} =
+// V_i[., k]; FP64 output vectors at stride D (Ritz vectors of the block basis)
 template <typename T, int K>
 __global__ void k_block_combine(double* __restrict__ out, const T* __restrict__ V, int64_t D, int n,
-                                const double* __restrict__ Y, int nq) {
+                                const double* __restrict__ Y, int ys, int nq) {
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < D; s += (int64_t)gridDim.x * blockDim.x) {
     double acc[16];
 #pragma unroll
@@ -750,7 +750,7 @@ __global__ void k_block_combine(double* __restrict__ out, const T* __restrict__ 
       const double v = (double)V[(int64_t)(c / K) * D * K + s * K + c % K];
 #pragma unroll
       for (int q = 0; q < 16; q++)
-        if (q < nq) acc[q] += v * Y[(int64_t)c * nq + q];
+        if (q < nq) acc[q] += v * Y[(int64_t)c * ys + q];
     }
 #pragma unroll
     for (int q = 0; q < 16; q++)
@@ -1892,7 +1892,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
   int niter = std::min<int64_t>(p->niter, D);
-  if (nkeep < 1 || nkeep > 16) return set_err("nkeep must be in 1..16");
+  if (nkeep < 1) return set_err("nkeep must be >= 1");
   if (niter < nkeep) return set_err("niter must be >= nkeep");
   const int thick = p->thick;
   if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter))) return set_err("thick must be 0 or in [nkeep, niter)");
@@ -2029,7 +2029,9 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     dY.upload(Yk);
     X64.alloc((size_t)nkeep * D);
     HX.alloc(D);
-    k_combine_mfma<T, double><<<mfma_grid(D), 256, 0, st>>>(X64.p, D, V.p, ld, m, dY.p, nkeep, nkeep, D);
+    for (int k0 = 0; k0 < nkeep; k0 += 16)
+      k_combine_mfma<T, double><<<mfma_grid(D), 256, 0, st>>>(X64.p + (int64_t)k0 * D, D, V.p, ld, m, dY.p + k0,
+                                                               nkeep, std::min(16, nkeep - k0), D);
     HIP_OK(hipGetLastError());
     for (int k = 0; k < nkeep; k++) {
       double* xk = X64.p + (int64_t)k * D;
@@ -2498,7 +2500,7 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   const int64_t D = s->t.dim;
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
-  if (nkeep < 1 || nkeep > 16) return set_err("nkeep must be in 1..16");
+  if (nkeep < 1) return set_err("nkeep must be >= 1");
   if (p->thick != 0) return set_err("block Lanczos runs fixed block iterations (thick must be 0)");
   if (D < K) return set_err("block larger than the basis");
   int nblk = (int)(std::min<int64_t>(p->niter, D) / K);
@@ -2622,7 +2624,9 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
     HX.alloc(D);
     part1.alloc(nb1);
     dots.alloc(4);
-    k_block_combine<T, K><<<g, kBlock, 0, st>>>(X64.p, V.p, D, n, dY.p, nkeep);
+    for (int k0 = 0; k0 < nkeep; k0 += 16)
+      k_block_combine<T, K><<<g, kBlock, 0, st>>>(X64.p + (int64_t)k0 * D, V.p, D, n, dY.p + k0, nkeep,
+                                                  std::min(16, nkeep - k0));
     HIP_OK(hipGetLastError());
     for (int k = 0; k < nkeep; k++) {
       double* xk = X64.p + (int64_t)k * D;

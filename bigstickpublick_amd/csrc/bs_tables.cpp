@@ -60,6 +60,39 @@ std::vector<std::string> data_tokens(std::istream& is, bool first_line_only_chec
   return toks;
 }
 
+// Fortran list-directed reads over the data lines of a file (comment lines
+// starting with ! or # skipped): each take(n) starts on a new line, continues
+// over the following lines until n values are read, and drops the rest of its
+// last line, as one READ(unit, *) statement does.
+struct RecordReader {
+  std::vector<std::vector<std::string>> lines;
+  size_t next = 0;
+  explicit RecordReader(std::istream& is) {
+    std::string line;
+    while (std::getline(is, line)) {
+      size_t p = line.find_first_not_of(" \t\r");
+      if (p == std::string::npos || line[p] == '!' || line[p] == '#') continue;
+      std::istringstream ss(line);
+      std::vector<std::string> toks;
+      std::string t;
+      while (ss >> t) {
+        if (t[0] == '!') break;
+        toks.push_back(t);
+      }
+      if (!toks.empty()) lines.push_back(toks);
+    }
+  }
+  bool take(size_t n, std::vector<std::string>& out) {
+    out.clear();
+    while (out.size() < n && next < lines.size()) {
+      for (const std::string& t : lines[next])
+        if (out.size() < n) out.push_back(t);
+      next++;
+    }
+    return out.size() == n;
+  }
+};
+
 }  // namespace
 
 // Clebsch-Gordan <j1 m1 j2 m2 | j3 m3> with 2j, 2m arguments: the Racah sum in
@@ -188,11 +221,20 @@ struct Builder {
     read_int_stream(f);
   }
   void read_int_stream(std::istream& f) {
-    auto tok = data_tokens(f);
-    if (tok.empty()) fail(".int: empty file");
-    long nme = std::strtol(tok[0].c_str(), nullptr, 10);
-    if (nme < 0) fail(".int: autoscaled (negative count) files are not supported");
-    if ((long)tok.size() < 1 + norb + 7 * nme) fail(".int: truncated");
+    // one record: the count and the SPEs (src/binput_tbme.f90:1086); one per
+    // element: a b c d J T V (:1171); values past those on a line are ignored
+    RecordReader rr(f);
+    std::vector<std::string> head, rec;
+    if (rr.lines.empty()) fail(".int: empty file");
+    const long nme0 = std::strtol(rr.lines[0][0].c_str(), nullptr, 10);
+    if (nme0 < 0) fail(".int: autoscaled (negative count) files are not supported");
+    if (!rr.take(1 + norb, head)) fail(".int: truncated SPE line");
+    std::vector<std::string> tok(head);
+    for (long n = 0; n < nme0; n++) {
+      if (!rr.take(7, rec)) fail(".int: truncated");
+      tok.insert(tok.end(), rec.begin(), rec.end());
+    }
+    long nme = nme0;
     const float spscale = t.in.spscale, vscale = t.in.vscale;
     t.pspe.assign(norb, 0.0f);
     t.nspe.assign(norb, 0.0f);

@@ -137,7 +137,7 @@ int bs_density(bs_system *sys, const void *psi, int dtype, double *rho_pp, doubl
 int bs_gradient(bs_system *sys, const void *psi, int dtype, double *grad);
 
 typedef struct {
-  int nkeep;        /* eigenpairs wanted */
+  int nkeep;        /* eigenpairs wanted (<= niter; <= 16 for the batched solve) */
   int niter;        /* Lanczos iterations (plain) or restart length (thick) */
   int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart */
   int max_restarts; /* thick restart: maximum number of restarts */

@@ -376,6 +376,21 @@ def block(name, dimblock, nbiter):
         shutil.rmtree(wd)
 
 
+# the reference's own ²⁵Mg / USDB example (examples/sd): inputs and the
+# outputs its authors committed (v7.8.4 runs), copied as data
+MG25_FILES = ["sd.sps", "usdb.int", "sdspin.opme", "output/mg25ref.res", "output/mg25expectH.res",
+              "output/mg25spinstrength1.res"]
+
+
+def examples():
+    src = "/root/reference/examples/sd"
+    out = os.path.join(HERE, "mg25")
+    os.makedirs(out, exist_ok=True)
+    for f in MG25_FILES:
+        shutil.copy(os.path.join(src, f), os.path.join(out, os.path.basename(f)))
+        print("copied", f)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -427,6 +442,8 @@ if __name__ == "__main__":
     for c in BLOCK:
         if "block" in want:
             block(*c)
+    if "examples" in want:
+        examples()
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

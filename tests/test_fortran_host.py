@@ -37,14 +37,17 @@ def test_fortran_host_rejects_bad_menu(casedir):
 
 
 def read_res(path):
+    """E, J, T of the first State/E/Ex/J/T table (its consecutive rows)."""
     rows = []
-    lines = open(path).read().splitlines()
+    lines = open(path, errors="replace").read().splitlines()
     for i, l in enumerate(lines):
-        if l.strip().startswith("State"):
+        if l.strip().startswith("State") and "Ex" in l:
             for l2 in lines[i + 1:]:
                 p = l2.split()
-                if len(p) >= 5 and p[0].isdigit():
-                    rows.append([float(p[1]), float(p[3]), float(p[4])])
+                if len(p) < 5 or not p[0].isdigit():
+                    break
+                rows.append([float(p[1]), float(p[3]), float(p[4])])
+            break
     return np.array(rows)
 
 
@@ -280,3 +283,70 @@ def test_fortran_host_block_lanczos_matches_reference(name, dimblock, nbiter, ca
     assert got.shape == ref.shape == (5, 3)
     assert np.abs(got[:, 0] - ref[:, 0]).max() < 2e-5, (got, ref)
     assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3
+
+
+def _two_float_rows(path):
+    """Energy / strength rows after the first underline of a strength table
+    (either reference version's layout)."""
+    lines = open(path, errors="replace").read().splitlines()
+    i0 = [i for i, l in enumerate(lines) if l.strip().startswith("______")][0]
+    rows = []
+    for l in lines[i0 + 1:]:
+        p = l.split()
+        try:
+            if len(p) != 2:
+                raise ValueError
+            rows.append([float(p[0]), float(p[1])])
+        except ValueError:
+            break
+    return np.array(rows)
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
+    """The reference's own 25Mg / USDB example (examples/sd, outputs committed
+    by its authors) replayed with this host's menus: n (10 states, ld 400),
+    o (the spin operator sdspin.opme on every state), s (30 iterations from
+    O|state 1>), x (<H> in the ten states).  Energies, J, T to the printed
+    digits; the strength table's total, and the energies and strengths of the
+    rows that carry at least 1e-3 of it."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mg25")
+    wd = os.path.join(casedir, "mg25")
+    os.makedirs(wd, exist_ok=True)
+    for f in ("sd.sps", "usdb.int", "sdspin.opme"):
+        shutil.copy(os.path.join(gold, f), wd)
+    scal = "1.0000 18.0000 25.0000 0.3000"
+    steps = [
+        "n\nmg25ref\nsd\n4 5\n1\nusdb\n%s\nend\nld\n10 400\n" % scal,
+        "o\nmg25ref\nmg25spin\nsdspin\n",
+        "s\nmg25spin\nmg25spinstrength1\nusdb\n%s\nend\n30 30\n1\n" % scal,
+        "x\nmg25ref\nmg25expectH\nusdb\n%s\nend\n" % scal,
+    ]
+    for script in steps:
+        r = _run(script, wd)
+        assert r.returncode == 0, script + r.stdout + r.stderr
+    got = read_res(os.path.join(wd, "mg25ref.res"))
+    ref = read_res(os.path.join(gold, "mg25ref.res"))
+    assert got.shape == (10, 3) and ref.shape[0] >= 10
+    assert np.abs(got[:, 0] - ref[:10, 0]).max() < 1.5e-5, (got, ref[:10])
+    assert np.abs(got[:, 1:] - ref[:10, 1:]).max() < 2e-3
+    # strength function from the spin operator applied to the ground state
+    tot = [float(l.split()[0]) for l in open(os.path.join(wd, "mg25spinstrength1.res"))
+           if l.strip().endswith("= total strength")][0]
+    rtot = [float(l.split()[0]) for l in open(os.path.join(gold, "mg25spinstrength1.res"))
+            if l.strip().endswith("= total strength")][0]
+    assert abs(tot - rtot) < 1e-5 * rtot, (tot, rtot)
+    rows = _two_float_rows(os.path.join(wd, "mg25spinstrength1.res"))
+    rrows = _two_float_rows(os.path.join(gold, "mg25spinstrength1.res"))
+    assert rows.shape == rrows.shape == (30, 2)
+    big = rrows[:, 1] >= 1e-3
+    assert np.abs(rows[big, 0] - rrows[big, 0]).max() < 2e-4, (rows[big], rrows[big])
+    assert np.abs(rows[:, 1] - rrows[:, 1]).max() < 2e-5 + 1e-4 * rtot
+    # expectation values of H in the stored states
+    gx = read_x_table(os.path.join(wd, "mg25expectH.res"))
+    rx = read_x_table(os.path.join(gold, "mg25expectH.res"))
+    assert len(gx) == len(rx) == 10
+    for g, f in zip(gx, rx):
+        assert abs(float(g[1]) - float(f[1])) < 1.5e-4 and abs(float(g[2]) - float(f[2])) < 2e-3
+        assert abs(float(g[4]) - float(f[4])) < 2e-5, (g, f)
