@@ -191,9 +191,29 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
     if (r < nr) {
       const int f = f0 + r;
       // source rows through a scalar buffer descriptor per entry (no vector
-      // address math), two entries' loads in flight together
+      // address math), four (then two) entries' loads in flight together
       const int64_t e0 = tb.pp_ptr[f], e1 = tb.pp_ptr[f + 1];
       int64_t e = e0;
+      for (; e + 4 <= e1; e += 4) {
+        __amdgpu_buffer_rsrc_t rr[4];
+        T vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          rr[u] = buf_rsrc(x + tb.pstart[tb.pp_src[e + u]]);
+          vv[u] = vload<T>(tb.pp_val32, tb.pp_val64, e + u);
+        }
+        T xx[4][CPT];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int q = 0; q < CPT; q++) {
+            if constexpr (sizeof(T) == 4) xx[u][q] = __uint_as_float(buf_u32(rr[u], cl[q] * 4u, 0u));
+            else xx[u][q] = buf_f64(rr[u], cl[q] * 8u, 0u);
+          }
+#pragma unroll
+        for (int q = 0; q < CPT; q++)
+          acc[r][q] = fma(vv[3], xx[3][q], fma(vv[2], xx[2][q], fma(vv[1], xx[1][q], fma(vv[0], xx[0][q], acc[r][q]))));
+      }
       for (; e + 2 <= e1; e += 2) {
         const __amdgpu_buffer_rsrc_t ra = buf_rsrc(x + tb.pstart[tb.pp_src[e]]);
         const __amdgpu_buffer_rsrc_t rb = buf_rsrc(x + tb.pstart[tb.pp_src[e + 1]]);
