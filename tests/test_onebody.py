@@ -123,3 +123,30 @@ def test_onebody_apply_matches_oracle_and_reference(name, op, J, T, tmp_path):
     assert np.abs(y32.cpu().numpy() - want).max() <= 1e-6 * np.abs(want).max()
     ref = np.stack([st["vec"] for st in read_wfn(os.path.join(GOLD, "%s_%s.wfn" % (name, op)))["states"]])
     assert np.abs(y32.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_onebody_apply_single_species_and_strength_pivot(tmp_path):
+    """Z = 0 (sd 0p3n): only the neutron part acts, equal to the oracle; the
+    result as a strength-function pivot has total strength ||O x||^2; a zero
+    pivot is refused."""
+    import torch
+    from bigstickpublick_amd.engine import System
+    from bigstickpublick_amd.inputs import SD_ORBITS
+    case = small_case("sd", 0, 3, 3, seed=80)
+    case.name = "sd0p3n"
+    d = str(tmp_path)
+    write_case(case, d)
+    sps, intf = os.path.join(d, "sd.sps"), os.path.join(d, "sd0p3n.int")
+    o = Oracle(sps, intf, 0, 3, 3)
+    s = System(sps, intf, 0, 3, 3, upload=True)
+    op_p, op_n, _, _ = o.opme(os.path.join(GOLD, "sd_gt.opme"))
+    x = np.random.default_rng(8).standard_normal(s.dim)
+    y, _, _ = s.apply_onebody(os.path.join(GOLD, "sd_gt.opme"), torch.tensor(x, device="cuda"))
+    want = o.apply_onebody(op_p, op_n, x)
+    assert np.abs(y.cpu().numpy() - want).max() <= 1e-12 * np.abs(want).max()
+    r = s.strength(y, nkeep=2, niter=min(20, s.dim))
+    assert abs(r.pivot_norm ** 2 - float(want @ want)) < 1e-10 * float(want @ want)
+    assert abs(r.strength.sum() - float(want @ want)) < 1e-9 * float(want @ want)
+    with pytest.raises(RuntimeError, match="zero"):
+        s.strength(torch.zeros(s.dim, dtype=torch.float64, device="cuda"), nkeep=2, niter=10)

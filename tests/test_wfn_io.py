@@ -133,3 +133,16 @@ def test_reader_refusals(tmp_path):
     with pytest.raises(RuntimeError):
         System.from_wfn(src, pf_int)
     shutil.rmtree(d2)
+
+
+def test_empty_wfn_round_trip(tmp_path):
+    """A .wfn with no states: written, counted as 0, and a read of one state
+    refused."""
+    sps, intf = _case(*WFN[0], str(tmp_path))
+    s = System(sps, intf, 2, 2, 0)
+    p = str(tmp_path / "empty.wfn")
+    s.write_wfn(p, [], [], [], np.zeros((0, s.dim), np.float32))
+    assert s.wfn_count(p) == 0
+    assert System.from_wfn(p).wfn_count(p) == 0
+    with pytest.raises(RuntimeError, match="file holds 0"):
+        s.read_wfn(p, first=0, count=1)
