@@ -389,6 +389,9 @@ def examples():
     for f in MG25_FILES:
         shutil.copy(os.path.join(src, f), os.path.join(out, os.path.basename(f)))
         print("copied", f)
+    # the isospin operator T^2 as an iso interaction on the same orbits
+    shutil.copy("/root/reference/util/examples/sdt2.int", os.path.join(out, "sdt2.int"))
+    print("copied util/examples/sdt2.int")
 
 
 def ensemble(members=8):

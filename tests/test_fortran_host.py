@@ -318,7 +318,7 @@ def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mg25")
     wd = os.path.join(casedir, "mg25")
     os.makedirs(wd, exist_ok=True)
-    for f in ("sd.sps", "usdb.int", "sdspin.opme"):
+    for f in ("sd.sps", "usdb.int", "sdspin.opme", "sdt2.int"):
         shutil.copy(os.path.join(gold, f), wd)
     scal = "1.0000 18.0000 25.0000 0.3000"
     steps = [
@@ -326,6 +326,7 @@ def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
         "o\nmg25ref\nmg25spin\nsdspin\n",
         "s\nmg25spin\nmg25spinstrength1\nusdb\n%s\nend\n30 30\n1\n" % scal,
         "x\nmg25ref\nmg25expectH\nusdb\n%s\nend\n" % scal,
+        "x\nmg25ref\nmg25T2\nsdt2\n1 1 1 1\nend\n",
     ]
     for script in steps:
         r = _run(script, wd)
@@ -356,3 +357,6 @@ def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
     for g, f in zip(gx, rx):
         assert abs(float(g[1]) - float(f[1])) < 1.5e-4 and abs(float(g[2]) - float(f[2])) < 2e-3
         assert abs(float(g[4]) - float(f[4])) < 2e-5, (g, f)
+    # the reference's T^2 operator file (util/examples/sdt2.int): T = 1/2
+    for g in read_x_table(os.path.join(wd, "mg25T2.res")):
+        assert abs(float(g[4]) - 0.75) < 1e-5, g
