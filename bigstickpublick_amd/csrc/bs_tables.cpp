@@ -227,14 +227,31 @@ struct Builder {
     std::vector<std::string> head, rec;
     if (rr.lines.empty()) fail(".int: empty file");
     const long nme0 = std::strtol(rr.lines[0][0].c_str(), nullptr, 10);
-    if (nme0 < 0) fail(".int: autoscaled (negative count) files are not supported");
-    if (!rr.take(1 + norb, head)) fail(".int: truncated SPE line");
+    // a negative count marks an autoscaled file (src/binput_tbme.f90:1005-1011,
+    // 1075-1080, 1119-1140): three more values on the SPE record (core A, A0,
+    // x), SPEs unscaled, TBMEs scaled by (A0 / (core + Z + N))^x, or by 1
+    // when A0 or x is 0; the scaling line of the menu is not used
+    const bool autoscale = nme0 < 0;
+    if (!rr.take(1 + norb + (autoscale ? 3 : 0), head)) fail(".int: truncated SPE line");
+    const long nme = autoscale ? -nme0 : nme0;
+    if (autoscale) {
+      const float core = std::strtof(head[1 + norb].c_str(), nullptr);
+      const float ax = std::strtof(head[2 + norb].c_str(), nullptr);
+      const float x = std::strtof(head[3 + norb].c_str(), nullptr);
+      t.in.spscale = 1.0f;
+      if (ax == 0.0f || x == 0.0f) {
+        t.in.vscale = 1.0f;
+      } else {
+        const float bx = core + (float)t.in.Z + (float)t.in.N;
+        t.in.vscale = std::pow(ax / bx, x);
+      }
+      head.resize(1 + norb);
+    }
     std::vector<std::string> tok(head);
-    for (long n = 0; n < nme0; n++) {
+    for (long n = 0; n < nme; n++) {
       if (!rr.take(7, rec)) fail(".int: truncated");
       tok.insert(tok.end(), rec.begin(), rec.end());
     }
-    long nme = nme0;
     const float spscale = t.in.spscale, vscale = t.in.vscale;
     t.pspe.assign(norb, 0.0f);
     t.nspe.assign(norb, 0.0f);

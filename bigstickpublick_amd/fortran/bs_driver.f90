@@ -60,7 +60,13 @@ contains
     read(5, *) z, n
     read(5, *) jz2
     read(5, '(a)') intname
-    read(5, *) spscale, a0, a, x
+    spscale = 1.0
+    a0 = 1.0
+    a = 1.0
+    x = 1.0
+    ! an autoscaled .int (negative count) carries its own scaling; no line
+    ! is read for it (src/binput_tbme.f90:1015)
+    if (.not. int_autoscaled(trim(first_word(intname)) // '.int')) read(5, *) spscale, a0, a, x
     read(5, '(a)') line
     if (adjustl(line) /= 'end') then
       write(6, *) ' bs_driver: one interaction file is supported'
@@ -177,7 +183,11 @@ contains
     read(5, '(a)') wfnname
     read(5, '(a)') oname
     read(5, '(a)') iname
-    read(5, *) s1, s2, s3, s4
+    s1 = 1.0
+    s2 = 1.0
+    s3 = 1.0
+    s4 = 1.0
+    if (.not. int_autoscaled(trim(first_word(iname)) // '.int')) read(5, *) s1, s2, s3, s4
     read(5, '(a)') l
     if (adjustl(l) /= 'end') then
       write(6, *) ' bs_driver: one interaction file is supported'
@@ -378,7 +388,11 @@ contains
     read(5, '(a)') wfnname
     read(5, '(a)') oname
     read(5, '(a)') iname
-    read(5, *) s1, s2, s3, s4
+    s1 = 1.0
+    s2 = 1.0
+    s3 = 1.0
+    s4 = 1.0
+    if (.not. int_autoscaled(trim(first_word(iname)) // '.int')) read(5, *) s1, s2, s3, s4
     read(5, '(a)') l
     if (adjustl(l) /= 'end') then
       write(6, *) ' bs_driver: one interaction file is supported'
@@ -424,6 +438,27 @@ contains
     r = hipFree(dv)
     r = bs_finalize(sy)
   end subroutine expectation_run
+
+  ! .int files whose first data line starts with a negative count are
+  ! autoscaled (src/binput_tbme.f90:1005-1011)
+  logical function int_autoscaled(path)
+    character(len=*), intent(in) :: path
+    character(len=512) :: l
+    integer :: u, ios, nme
+    int_autoscaled = .false.
+    open(newunit=u, file=path, status='old', action='read', iostat=ios)
+    if (ios /= 0) return
+    do
+      read(u, '(a)', iostat=ios) l
+      if (ios /= 0) exit
+      l = adjustl(l)
+      if (len_trim(l) == 0 .or. l(1:1) == '!' .or. l(1:1) == '#') cycle
+      read(l, *, iostat=ios) nme
+      if (ios == 0) int_autoscaled = nme < 0
+      exit
+    end do
+    close(u)
+  end function int_autoscaled
 
   ! the reference's menu lines may carry a trailing '! comment'
   function first_word(s) result(w)

@@ -394,6 +394,34 @@ def examples():
     print("copied util/examples/sdt2.int")
 
 
+def autoscaled(name="sd2p2n", core=16.0, a0=18.0, x=0.3):
+    """<name>_auto.int: the seeded interaction of <name> rewritten as an
+    autoscaled file (negative count; core, A0, x after the SPEs), and the
+    reference's .res / .lcoef for it (menu n, no scaling line, lf 5 60),
+    kept as <name>_auto.res / <name>_auto.lcoef."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        lines = [l for l in open(os.path.join(wd, name + ".int")).read().splitlines()
+                 if l.strip() and not l.lstrip().startswith(("!", "#"))]
+        head = lines[0].split()
+        lines[0] = " ".join(["-" + head[0]] + head[1:] + ["%.1f" % core, "%.1f" % a0, "%.1f" % x])
+        auto = name + "_auto"
+        with open(os.path.join(HERE, auto + ".int"), "w") as f:
+            f.write("! autoscaled copy of a seeded synthetic interaction\n" + "\n".join(lines) + "\n")
+        shutil.copy(os.path.join(HERE, auto + ".int"), wd)
+        script = "n\n%s\n%s\n%d %d\n%d\n%s\nend\nlf\n5 60\n" % (auto, case.space, case.Z, case.N, case.jz2, auto)
+        run_ref(wd, script)
+        shutil.copy(os.path.join(wd, auto + ".res"), os.path.join(HERE, auto + ".res"))
+        shutil.copy(os.path.join(wd, auto + ".lcoef"), os.path.join(HERE, auto + ".lcoef"))
+        print(auto, read_res(os.path.join(wd, auto + ".res"))[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -447,6 +475,8 @@ if __name__ == "__main__":
             block(*c)
     if "examples" in want:
         examples()
+    if "autoscaled" in want:
+        autoscaled()
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -360,3 +360,28 @@ def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
     # the reference's T^2 operator file (util/examples/sdt2.int): T = 1/2
     for g in read_x_table(os.path.join(wd, "mg25T2.res")):
         assert abs(float(g[4]) - 0.75) < 1e-5, g
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_autoscaled_int(casedir):
+    """An autoscaled .int: no scaling line is read (the file carries core, A0
+    and x), and the .res / .lcoef agree with the reference's run on the same
+    file."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    wd = os.path.join(casedir, "fauto")
+    write_case(case, wd)
+    shutil.copy(os.path.join(gold, "sd2p2n_auto.int"), wd)
+    r = _run("n\nsd2p2n_auto\nsd\n2 2\n0\nsd2p2n_auto\nend\nlf\n5 60\n", wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_res(os.path.join(wd, "sd2p2n_auto.res"))
+    ref = read_res(os.path.join(gold, "sd2p2n_auto.res"))
+    conv = np.abs(ref[:, 1] - np.round(ref[:, 1])) < 1e-3
+    assert got.shape == ref.shape == (5, 3) and conv.sum() >= 3
+    assert np.abs(got[conv, 0] - ref[conv, 0]).max() < 2e-5, (got, ref)
+    lc = np.loadtxt(os.path.join(wd, "sd2p2n_auto.lcoef"))
+    rlc = np.loadtxt(os.path.join(gold, "sd2p2n_auto.lcoef"))
+    assert np.abs(lc[:20, 1] - rlc[:20, 1]).max() < 2e-4

@@ -216,3 +216,39 @@ def test_oracle_mg25_usdb_matches_reference_example():
     _, _, e, _ = lanczos(o.apply, o.dim, 220, 10)
     assert ref.shape == (10, 3)
     assert np.abs(e - ref[:, 0]).max() < 6e-6, (e, ref[:, 0])
+
+
+def test_autoscaled_int_matches_reference_run(tmp_path):
+    """An autoscaled .int (negative count; core 16, A0 18, x 0.3 after the
+    SPEs): the engine's tables equal, bit for bit, those of the same file
+    read with the scaling line (1, 18, 16 + Z + N, 0.3), and the oracle with
+    that scale reproduces the reference's own run on the autoscaled file
+    (alpha of the first 20 Lanczos steps, energies)."""
+    from bigstickpublick_amd.engine import System
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    c = small_case("sd", 2, 2, 0, seed=77)
+    c.name = "sd2p2n"
+    d = str(tmp_path)
+    write_case(c, d)
+    sps = os.path.join(d, "sd.sps")
+    auto = System(sps, os.path.join(gold, "sd2p2n_auto.int"), 2, 2, 0)
+    plain = System(sps, os.path.join(d, "sd2p2n.int"), 2, 2, 0, scaling=(1.0, 18.0, 20.0, 0.3))
+    for k in range(3):
+        assert np.array_equal(auto.me_table(k).view(np.uint32), plain.me_table(k).view(np.uint32))
+    vs = float(np.float32(18.0) / np.float32(20.0)) ** np.float32(0.3)
+    o = Oracle(sps, os.path.join(d, "sd2p2n.int"), 2, 2, 0, 1.0, float(np.float32(vs)))
+    a, _, e, _ = lanczos(o.apply, o.dim, 60, 5)
+    lc = np.loadtxt(os.path.join(gold, "sd2p2n_auto.lcoef"))
+    assert np.abs(a[:20] - lc[:20, 1]).max() < 1e-4
+    res = []
+    for l in open(os.path.join(gold, "sd2p2n_auto.res")).read().splitlines():
+        p = l.split()
+        if len(p) == 6 and p[0].isdigit():
+            res.append((float(p[1]), float(p[3])))
+    res = np.array(res[:5])
+    # the states the 60-step runs have converged (integer J in the
+    # reference's table); the others still mix and follow the rounding
+    conv = np.abs(res[:, 1] - np.round(res[:, 1])) < 1e-3
+    assert conv.sum() >= 3
+    assert np.abs(e - res[:, 0])[conv].max() < 2e-5, (e, res)
