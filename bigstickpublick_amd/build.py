@@ -21,7 +21,7 @@ def build_native(verbose=False):
     srcs = [os.path.join(CSRC, "bs_engine.hip"), os.path.join(CSRC, "bs_tables.cpp")]
     tmp = LIB + ".tmp"
     cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(os.path.dirname(HERE), "include"),
-                                 "-o", tmp] + srcs
+                                 "-o", tmp] + srcs + ["-lrocsolver", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -33,6 +33,7 @@
 
 #include "../../include/bigstick_amd.h"
 #include "bs_tables.hpp"
+#include <rocsolver/rocsolver.h>
 
 namespace {
 
@@ -775,6 +776,24 @@ __global__ void k_block_combine(double* __restrict__ out, const T* __restrict__ 
 #pragma unroll
     for (int q = 0; q < 16; q++)
       if (q < nq) out[(int64_t)q * D + s] = acc[q];
+  }
+}
+
+// X[s*64 + k] = 1 if s == j0 + k (k < kb) else 0: a block of unit vectors
+__global__ void k_unit_block(double* __restrict__ X, int64_t D, int64_t j0, int kb) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D * 64; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i / 64;
+    const int k = (int)(i % 64);
+    X[i] = (k < kb && s == j0 + k) ? 1.0 : 0.0;
+  }
+}
+
+// column j0 + k of the dense (column-major) H from the interleaved block H X
+__global__ void k_scatter_cols(double* __restrict__ H, int64_t D, const double* __restrict__ Y, int64_t j0, int kb) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D * 64; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i / 64;
+    const int k = (int)(i % 64);
+    if (k < kb) H[(j0 + k) * D + s] = Y[i];
   }
 }
 
@@ -3706,6 +3725,83 @@ int bs_strength(bs_system* s, const bs_lanczos_params* p, const void* pivot, dou
       return run_lanczos<double>(s, p, alpha, beta, evals, evecs, resid, iters, pr);
     }
     return set_err("dtype must be BS_F32 or BS_F64");
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_exact(bs_system* s, int nkeep, int dtype, double* evals, void* evecs, double* resid, int* iters) {
+  try {
+    checked(s);
+    if (!evals) return set_err("null argument");
+    if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    const int64_t D = s->t.dim;
+    if (nkeep < 1 || nkeep > D) return set_err("nkeep must be in 1..dim");
+    if (D > 65536) return set_err("exact diagonalization is limited to dim <= 65536 (dense H in HBM)");
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    const hipStream_t st = nullptr;
+    // (src/blanczosmain.f90:2171-2471) H column by column: here 64 unit
+    // vectors at a time through the multi-vector kernel, FP64
+    DevBuf<double> Hd, X, Y, W, E;
+    DevBuf<int> info;
+    Hd.alloc((size_t)D * D);
+    X.alloc((size_t)D * 64);
+    Y.alloc((size_t)D * 64);
+    W.alloc(D);
+    E.alloc(D);
+    info.alloc(1);
+    const BatchVals bv = shared_vals(s, 64);
+    const int g = grid_for(D * 64);
+    for (int64_t j0 = 0; j0 < D; j0 += 64) {
+      const int kb = (int)std::min<int64_t>(64, D - j0);
+      k_unit_block<<<g, kBlock, 0, st>>>(X.p, D, j0, kb);
+      launch_batched<double>(s, bv, X.p, Y.p, st);
+      k_scatter_cols<<<g, kBlock, 0, st>>>(Hd.p, D, Y.p, j0, kb);
+      HIP_OK(hipGetLastError());
+    }
+    // the dense symmetric eigenproblem (the reference calls DSYEV): rocSOLVER
+    // divide and conquer, eigenvectors in the columns of H
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return set_err("rocblas_create_handle failed");
+    std::unique_ptr<std::remove_pointer<rocblas_handle>::type, rocblas_status (*)(rocblas_handle)> guard(
+        h, rocblas_destroy_handle);
+    if (rocblas_set_stream(h, st) != rocblas_status_success) return set_err("rocblas_set_stream failed");
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)D, Hd.p, (rocblas_int)D, W.p,
+                         E.p, info.p) != rocblas_status_success)
+      return set_err("rocsolver_dsyevd failed");
+    int inf = 0;
+    HIP_OK(hipMemcpy(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (inf != 0) return set_err("rocsolver_dsyevd did not converge (info " + std::to_string(inf) + ")");
+    std::vector<double> ev(D);
+    HIP_OK(hipMemcpy(ev.data(), W.p, sizeof(double) * D, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nkeep; k++) evals[k] = ev[k];
+    const int gd = grid_for(D);
+    const int nb1 = gd < 1024 ? gd : 1024;
+    DevBuf<double> HX, part1, dots;
+    HX.alloc(D);
+    part1.alloc(nb1);
+    dots.alloc(4);
+    for (int k = 0; k < nkeep; k++) {
+      double* xk = Hd.p + (int64_t)k * D;
+      if (resid) {
+        // ||H x - E x|| of the eigenvector, one more apply
+        launch_apply<double>(s, xk, HX.p, 0, st);
+        HIP_OK(hipMemcpy(dots.p + 1, &evals[k], sizeof(double), hipMemcpyHostToDevice));
+        k_sub_scaled<double><<<gd, kBlock, 0, st>>>(HX.p, xk, dots.p + 1, D);
+        k_multidot<double><<<nb1, kBlock, 0, st>>>(HX.p, D, 1, HX.p, D, part1.p);
+        k_reduce<<<1, kBlock, 0, st>>>(part1.p, nb1, 1, dots.p + 2, 1);
+        HIP_OK(hipMemcpy(&resid[k], dots.p + 2, sizeof(double), hipMemcpyDeviceToHost));
+      }
+      if (evecs) {
+        if (dtype == BS_F64)
+          HIP_OK(hipMemcpyAsync((double*)evecs + (int64_t)k * D, xk, sizeof(double) * D, hipMemcpyDeviceToDevice, st));
+        else
+          k_convert<double, float><<<gd, kBlock, 0, st>>>((float*)evecs + (int64_t)k * D, xk, D);
+      }
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    if (iters) *iters = (int)D;
+    return 0;
   } catch (const std::exception& e) {
     return set_err(e.what());
   }

@@ -59,6 +59,7 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
+    L.bs_exact.argtypes = [vp, i32, i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_strength.argtypes = [vp, ctypes.POINTER(_Params), vp, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp,
                               vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_apply_block.argtypes = [vp, vp, vp, i32, i32]
@@ -312,6 +313,25 @@ class System:
                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
+
+    def exact(self, nkeep, dtype="f64", want_vectors=False):
+        """Exact diagonalization of the dense H (dim <= 65536): the nkeep
+        lowest eigenpairs and their residuals."""
+        import torch
+        self.upload()
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        ev = np.zeros(nkeep)
+        res = np.zeros(nkeep)
+        it = ctypes.c_int()
+        vec = None
+        if want_vectors:
+            vec = torch.empty((nkeep, self.dim), device="cuda",
+                              dtype=torch.float32 if dt == BS_F32 else torch.float64)
+        torch.cuda.synchronize()
+        _check(load_library().bs_exact(self.h, nkeep, dt, ev.ctypes.data,
+                                       ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
+                                       res.ctypes.data, ctypes.byref(it)))
+        return LanczosResult(ev, None, None, res, it.value, vec)
 
     def block_lanczos(self, nkeep, niter, block, dtype="f32", tol=0.0, polish=True, want_vectors=False):
         """Block Lanczos with `block` vectors per block and a basis of niter

@@ -103,6 +103,14 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_lanczos
 
+    function bs_exact(sys, nkeep, dtype, evals, evecs, resid, iters) bind(C, name='bs_exact') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: sys, evals, evecs, resid
+      integer(c_int), value :: nkeep, dtype
+      integer(c_int), intent(out) :: iters
+      integer(c_int) :: rc
+    end function bs_exact
+
     function bs_block_lanczos(sys, p, block, evals, evecs, resid, iters) bind(C, name='bs_block_lanczos') result(rc)
       import :: c_ptr, c_int, bs_lanczos_params
       type(c_ptr), value :: sys, evals, evecs, resid

@@ -3,8 +3,8 @@
 ! menu s, strength function from a pivot in a previous .wfn; menu x,
 ! expectation values in previous states; menu o, a one-body operator applied
 ! to previous states, below;
-! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td
-! and the block options bf/bd,
+! interaction menu, src/binput_tbme.f90:178-573; Lanczos options lf/ld/tf/td,
+! the block options bf/bd and exact diagonalization ex,
 ! src/blanczosmain.f90:380-745) and writes <name>.res (energy table in the
 ! format of src/bparticleocc.f90:256) and <name>.lcoef (alpha, beta per
 ! iteration in the format of src/blanczosmain.f90:1685).
@@ -85,7 +85,11 @@ contains
     read(5, '(a)') lopt
     lopt = adjustl(lopt)
     dimblock = 0
-    if (lopt(1:1) == 'b') then
+    if (lopt(1:2) == 'ex') then
+      ! exact diagonalization: nkeep only (src/blanczosmain.f90:754-763)
+      read(5, *) nkeep
+      niter = nkeep
+    else if (lopt(1:1) == 'b') then
       ! block Lanczos bf / bd: nkeep, block size, block iterations; the basis
       ! holds (iterations + 1) blocks (src/blanczosmain.f90:849-869)
       read(5, *) nkeep, dimblock, nbiter
@@ -126,7 +130,10 @@ contains
     dvec = device_alloc(nbytes)
     alpha = 0.0d0
     beta = 0.0d0
-    if (dimblock > 0) then
+    if (lopt(1:2) == 'ex') then
+      rc = bs_exact(sys, nkeep, dtype, c_loc(evals), dvec, c_loc(resid), iters)
+      call bs_check(rc, 'bs_exact')
+    else if (dimblock > 0) then
       rc = bs_block_lanczos(sys, p, dimblock, c_loc(evals), dvec, c_loc(resid), iters)
       call bs_check(rc, 'bs_block_lanczos')
     else
@@ -143,7 +150,7 @@ contains
     write(iu, '(a)') '  '
     call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
     close(iu)
-    if (dimblock == 0) then
+    if (dimblock == 0 .and. lopt(1:2) /= 'ex') then
       open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
       do k = 1, niter
         write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)

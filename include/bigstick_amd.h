@@ -157,6 +157,15 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Exact diagonalization: the dense H (dim <= 65536, dim^2 doubles in HBM)
+ * built 64 columns at a time with the multi-vector kernel in FP64, all
+ * eigenpairs by rocSOLVER's symmetric divide-and-conquer solver; evals (nkeep
+ * lowest), evecs (device, may be NULL: nkeep vectors in dtype), resid (may be
+ * NULL: ||H x - E x|| of each), iters (may be NULL: dim, the matvec count).
+ * Replaces exactdiag_p, option ex (src/blanczosmain.f90:2171-2471, :754-763),
+ * which builds H with dim calls of applyHbundled_g and calls DSYEV. */
+int bs_exact(bs_system *sys, int nkeep, int dtype, double *evals, void *evecs, double *resid, int *iters);
+
 /* Block Lanczos: blocks of `block` (1..8) vectors, H applied to a block with
  * the multi-vector kernel, full block Gram-Schmidt against every stored
  * block (FP64 sums), each new block orthonormalized by the spectral

@@ -422,6 +422,25 @@ def autoscaled(name="sd2p2n", core=16.0, a0=18.0, x=0.3):
         shutil.rmtree(wd)
 
 
+def exact(name="sd2p2n"):
+    """The reference's exact diagonalization (option ex, 5 states) of a small
+    system, kept as <name>_ex.res."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    case.nkeep = 5
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        # one thread: like the wh path, the reference's dense build gives
+        # wrong matrices with several OpenMP threads
+        run_ref(wd, menu_script(case, "n", "ex"), env_extra={"OMP_NUM_THREADS": "1"})
+        shutil.copy(os.path.join(wd, name + ".res"), os.path.join(HERE, name + "_ex.res"))
+        print(name, "ex", read_res(os.path.join(wd, name + ".res"))[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -477,6 +496,8 @@ if __name__ == "__main__":
         examples()
     if "autoscaled" in want:
         autoscaled()
+    if "exact" in want:
+        exact()
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

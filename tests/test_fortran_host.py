@@ -385,3 +385,26 @@ def test_fortran_host_autoscaled_int(casedir):
     lc = np.loadtxt(os.path.join(wd, "sd2p2n_auto.lcoef"))
     rlc = np.loadtxt(os.path.join(gold, "sd2p2n_auto.lcoef"))
     assert np.abs(lc[:20, 1] - rlc[:20, 1]).max() < 2e-4
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_exact_matches_reference(casedir):
+    """Option ex (exact diagonalization): the .res energies, J, T equal the
+    reference's own ex run."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    case.nkeep = 5
+    wd = os.path.join(casedir, "fex")
+    write_case(case, wd)
+    r = _run(menu_script(case, "ns", "ex"), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_res(os.path.join(wd, "sd2p2n.res"))
+    ref = read_res(os.path.join(gold, "sd2p2n_ex.res"))
+    assert got.shape == ref.shape == (5, 3)
+    # one unit of the printed f10.5: the reference's dense solve is FP32
+    # (state 5: -28.240175 exactly, printed -28.24018 here, -28.24017 there)
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 1.1e-5, (got, ref)
+    assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3

@@ -251,3 +251,26 @@ def test_block_lanczos_config0(casedir):
     early = s.block_lanczos(nkeep=5, niter=300, block=4, dtype="f64", tol=1e-9)
     assert s.block_lanczos(nkeep=5, niter=300, block=4, dtype="f32").evals.shape == (5,)
     assert early.matvecs < blk.matvecs and np.abs(early.evals - single.evals).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sd2p2n", "pf2p1n_m3", "sd1p0n_m1"])
+def test_exact_diagonalization(name, casedir):
+    """Option ex: the dense H from 64-column blocks and rocSOLVER give the
+    oracle's dense spectrum (all states, 1e-10) with eigenvector residuals at
+    rounding level, FP32 output vectors equal to the FP64 ones rounded."""
+    case, d, fx = small_fixture(name, casedir)
+    s = _sys(case, d)
+    ev = np.linalg.eigvalsh(Oracle(*paths(case, d), case.Z, case.N, case.jz2).dense())
+    n = min(10, s.dim)
+    r = s.exact(nkeep=n, want_vectors=True)
+    assert r.matvecs == s.dim
+    assert np.abs(r.evals - ev[:n]).max() < 1e-10, (r.evals, ev[:n])
+    assert r.resid.max() < 1e-10
+    X = r.evecs.cpu().numpy()
+    assert np.abs(X @ X.T - np.eye(n)).max() < 1e-12
+    r32 = s.exact(nkeep=n, dtype="f32", want_vectors=True)
+    assert np.abs(r32.evals - r.evals).max() == 0.0
+    x32 = r32.evecs.cpu().numpy().astype(np.float64)
+    ok = np.abs(np.abs(np.einsum("ij,ij->i", x32, X)) - 1.0)
+    assert ok[np.diff(np.r_[r.evals, np.inf]) > 1e-6].max() < 1e-6
