@@ -1442,6 +1442,42 @@ void jacobi_eigen(int m, std::vector<double> A, std::vector<double>& e, std::vec
   Y.swap(Ys);
 }
 
+// The nk lowest eigenvalues of the symmetric tridiagonal matrix with diagonal
+// a[0..n) and off-diagonal b[0..n-1) (b[j] couples j and j+1), by Sturm
+// sequence bisection (the count of negative pivots of T - xI is the number of
+// eigenvalues below x) inside the Gershgorin interval.
+std::vector<double> tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, int nk) {
+  const int n = (int)a.size();
+  double lo = a[0], hi = a[0];
+  for (int j = 0; j < n; j++) {
+    const double r = (j > 0 ? std::fabs(b[j - 1]) : 0.0) + (j + 1 < n ? std::fabs(b[j]) : 0.0);
+    lo = std::min(lo, a[j] - r);
+    hi = std::max(hi, a[j] + r);
+  }
+  auto below = [&](double x) {
+    int c = 0;
+    double q = a[0] - x;
+    for (int j = 0;; j++) {
+      if (q == 0.0) q = -1e-300;
+      if (q < 0.0) c++;
+      if (j + 1 == n) break;
+      q = a[j + 1] - x - b[j] * b[j] / q;
+    }
+    return c;
+  };
+  std::vector<double> e(std::min(nk, n));
+  for (int k = 0; k < (int)e.size(); k++) {
+    double l = lo, h = hi;
+    for (int it = 0; it < 200 && h - l > 1e-14 * std::max(1.0, std::fabs(l) + std::fabs(h)); it++) {
+      const double m = 0.5 * (l + h);
+      if (below(m) > k) h = m;
+      else l = m;
+    }
+    e[k] = 0.5 * (l + h);
+  }
+  return e;
+}
+
 // one wave per 16-row strip, at most 2048 workgroups of 4 waves
 int mfma_grid(int64_t D) {
   const int64_t waves = (D + 15) / 16;
@@ -1965,7 +2001,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (pr.pivot_norm) *pr.pivot_norm = pnorm;
 
   std::vector<double> alpha_h, beta_h;  // current arrowhead/tridiagonal data
-  std::vector<double> e_prev;
+  std::vector<double> e_prev, e_check;
   int total_mv = 0;
   int kstart = 0;           // number of locked (thick-restart) vectors at the front
   std::vector<double> arrow;  // coupling of the locked vectors to v_kstart
@@ -1990,6 +2026,27 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
       k_normalize<T><<<g, kBlock, 0, st>>>(V.p + (int64_t)(i + 1) * ld, w.p, d_beta + i, D);
       HIP_OK(hipGetLastError());
+      // plain runs with a tolerance (options ld, lc): every 10 steps the
+      // lowest nkeep Ritz values of the tridiagonal so far; stop when none
+      // moved by more than tol since the previous check (the reference tests
+      // its criterion every itskipeig = 10 steps, src/blanczoslib.f90:2660-2800)
+      if (thick == 0 && p->tol > 0.0 && (i + 1) % 10 == 0 && i + 1 >= nkeep + 2 && i + 1 < niter) {
+        std::vector<double> a(i + 1), b(i + 1);
+        HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * (i + 1), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * (i + 1), hipMemcpyDeviceToHost));
+        std::vector<double> ek = tridiag_lowest(a, b, nkeep);
+        bool stop = false;
+        if (!e_check.empty()) {
+          double dmax = 0.0;
+          for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(ek[k] - e_check[k]));
+          stop = dmax < p->tol;
+        }
+        e_check = ek;
+        if (stop) {
+          niter = i + 1;
+          break;
+        }
+      }
     }
     alpha_h.assign(mtot, 0.0);
     beta_h.assign(mtot, 0.0);

@@ -142,7 +142,8 @@ typedef struct {
   int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart */
   int max_restarts; /* thick restart: maximum number of restarts */
   int dtype;        /* BS_F32: FP32 vectors/matvec, FP64 coefficients; BS_F64: all FP64 */
-  double tol;       /* thick restart: stop when max |E_i - E_i(previous)| < tol (0: run all) */
+  double tol;       /* stop when max |E_i - E_i(previous)| < tol, checked at each thick restart, or
+                       every 10 steps of a plain run (0: run all) */
   int polish;       /* 1: FP64 Rayleigh quotient / residual polish of the Ritz vectors */
 } bs_lanczos_params;
 

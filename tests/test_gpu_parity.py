@@ -274,3 +274,17 @@ def test_exact_diagonalization(name, casedir):
     x32 = r32.evecs.cpu().numpy().astype(np.float64)
     ok = np.abs(np.abs(np.einsum("ij,ij->i", x32, X)) - 1.0)
     assert ok[np.diff(np.r_[r.evals, np.inf]) > 1e-6].max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_plain_lanczos_stops_at_tol(casedir):
+    """A plain run with a tolerance (options ld / lc) checks its Ritz values
+    every 10 steps and stops once none moves by tol: fewer matvecs than the
+    iteration cap, the same energies as the full run."""
+    case, d, fx = config_fixture(0, casedir)
+    s = _sys(case, d)
+    full = s.lanczos(nkeep=5, niter=300, dtype="f64")
+    early = s.lanczos(nkeep=5, niter=300, dtype="f64", tol=1e-8)
+    assert early.matvecs < full.matvecs and early.matvecs % 10 == 5  # 10k steps + 5 polish applies
+    assert np.abs(early.evals - full.evals).max() < 1e-6
+    assert np.count_nonzero(early.alpha) == early.matvecs - 5
