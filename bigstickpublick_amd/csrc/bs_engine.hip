@@ -3169,6 +3169,14 @@ int bs_basis(const bs_system* s, int it, uint64_t* mask, int64_t* start) {
   return 0;
 }
 
+int bs_parity(const bs_system* s, int* parity) {
+  if (!s || !parity) return set_err("null argument");
+  // one-parity spaces only: every SD has parity (-1)^(l * (Z + N))
+  const int l = s->t.in.orbits.empty() ? 0 : s->t.in.orbits[0].l;
+  *parity = ((l * (s->t.in.Z + s->t.in.N)) % 2) ? -1 : 1;
+  return 0;
+}
+
 int bs_num_sps(const bs_system* s, int* nsps) {
   if (!s || !nsps) return set_err("null argument");
   *nsps = s->t.nsps;

@@ -103,6 +103,13 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_lanczos
 
+    function bs_parity(sys, parity) bind(C, name='bs_parity') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: sys
+      integer(c_int), intent(out) :: parity
+      integer(c_int) :: rc
+    end function bs_parity
+
     function bs_exact(sys, nkeep, dtype, evals, evecs, resid, iters) bind(C, name='bs_exact') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: sys, evals, evecs, resid

@@ -28,7 +28,9 @@ program bs_driver
   case ('s', 'S')
     call strength_run(opt(2:2) == 's' .or. opt(2:2) == 'S')
   case ('x', 'X')
-    call expectation_run()
+    call expectation_run(.false.)
+  case ('h', 'H')
+    call expectation_run(.true.)
   case ('o', 'O')
     if (len_trim(opt) > 1) then
       write(6, *) ' bs_driver: of the o options only o itself is supported'
@@ -37,7 +39,7 @@ program bs_driver
     call onebody_run()
   case default
     write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function),', &
-                ' x (expectation values) and o (one-body operator applied to states)'
+                ' x (expectation values), h (operator matrix) and o (one-body operator applied to states)'
     stop 1
   end select
 
@@ -328,11 +330,12 @@ contains
     real(c_double) :: j2, t2
     real :: xj, xt
     integer(c_int64_t) :: esize
-    integer(c_int) :: rc
+    integer(c_int) :: rc, ipar
     integer :: k
 
     esize = 4
     if (dtype == BS_F64) esize = 8
+    rc = bs_parity(sy, ipar)
     write(iu, '(a)') '  State      E        Ex         J       T      par'
     allocate(we(nk), wj(nk), wt2(nk))
     do k = 1, nk
@@ -342,7 +345,7 @@ contains
       call bs_check(rc, 'bs_observable T^2')
       xj = real(-0.5d0 + sqrt(0.25d0 + max(j2, 0.0d0)))
       xt = real(-0.5d0 + sqrt(0.25d0 + max(t2, 0.0d0)))
-      write(iu, 100) k, ev(k), ev(k) - ev(1), xj, xt, 1
+      write(iu, 100) k, ev(k), ev(k) - ev(1), xj, xt, ipar
       ! .wfn carries E, J and T(T+1) in single precision (src/blanczoslib.f90:1330-1350)
       we(k) = real(ev(k), c_float)
       wj(k) = xj
@@ -380,7 +383,12 @@ contains
   ! src/bexpect.f90:16-74).  stdin: x, .wfn name, output name, .int name,
   ! scaling line, end.  Each state is normalized first (dnormvec_p,
   ! src/bvectorlib.f90:200-264); <name>.res gets the reference's table.
-  subroutine expectation_run()
+  !
+  ! Menu h (matrix): the matrix <f|O|i> of the raw stored states instead
+  ! (hmatrixelements_boss, src/bexpect.f90:84-180): <name>.xme gets the state
+  ! table and, for f <= i with equal J, every element above 1e-6, then -1 -1 0.
+  subroutine expectation_run(matrix)
+    logical, intent(in) :: matrix
     character(len=256) :: wfnname, oname, iname, l
     character(len=:), allocatable :: wpath
     real(c_float) :: s1, s2, s3, s4
@@ -390,7 +398,9 @@ contains
     real(c_float), allocatable, target :: e(:), ej(:), et(:)
     real(c_double), allocatable, target :: hv(:, :), hme(:, :)
     real(c_double), allocatable :: dn(:)
-    integer :: i, u
+    real :: h4
+    integer(c_int) :: ipar
+    integer :: i, f, u
 
     read(5, '(a)') wfnname
     read(5, '(a)') oname
@@ -420,7 +430,7 @@ contains
     call bs_check(r, 'bs_read_wfn')
     do i = 1, nk
       dn(i) = sqrt(sum(hv(:, i)**2))
-      if (dn(i) > 0.0d0) hv(:, i) = hv(:, i) / dn(i)
+      if (dn(i) > 0.0d0 .and. .not. matrix) hv(:, i) = hv(:, i) / dn(i)
     end do
     dv = device_alloc(8 * d * nk)
     r = hipMemcpy(dv, c_loc(hv), int(8 * d * nk, c_size_t), 1_c_int)
@@ -431,6 +441,32 @@ contains
     ! hme(f, i) = <v_f | O | v_i>; the diagonal is the expectation value
     r = bs_operator_matrix(sy, c_null_ptr, 1.0, 1.0, 1.0, 1.0, dv, nk, BS_F64, c_loc(hme))
     call bs_check(r, 'bs_operator_matrix')
+    if (matrix) then
+      open(newunit=u, file=trim(first_word(oname)) // '.xme', status='replace')
+      write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+      write(u, '(a,a)') '  states read from ', wpath
+      write(u, *) ' '
+      write(u, '(a)') '  State      E        Ex         J       T      par'
+      r = bs_parity(sy, ipar)
+      do i = 1, nk
+        write(u, 111) i, e(i), e(i) - e(1), ej(i), et(i), ipar
+      end do
+111   format(i5,3x,2f10.5,2x,2f8.3,3x,i2)
+      write(u, *) ' '
+      do i = 1, nk
+        do f = 1, i
+          if (abs(ej(i) - ej(f)) > 0.0001) cycle
+          h4 = real(hme(f, i))
+          if (abs(h4) > 1e-6) write(u, *) i, f, h4
+        end do
+      end do
+      write(u, *) -1, -1, 0.0
+      close(u)
+      write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', nk, ' states'
+      r = hipFree(dv)
+      r = bs_finalize(sy)
+      return
+    end if
     open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
     write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
     write(u, '(a,a)') '  states read from ', wpath

@@ -56,6 +56,11 @@ int bs_init(const char *sps_path, const char *int_path, int Z, int N, int jz2,
 int bs_finalize(bs_system *sys);
 const char *bs_last_error(void);
 
+/* Parity (+1 / -1) of every basis state (the spaces this engine builds have
+ * one parity).  Replaces find_parity (src/boutputlib.f90:1673-1873), which
+ * reads it from the vector's components. */
+int bs_parity(const bs_system *sys, int *parity);
+
 /* ---- host-side inspection (no device needed) ---- */
 int bs_dim(const bs_system *sys, int64_t *dim);
 /* it = 0 protons, 1 neutrons */

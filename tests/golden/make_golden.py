@@ -441,6 +441,25 @@ def exact(name="sd2p2n"):
         shutil.rmtree(wd)
 
 
+def hmatrix(name, op_seed):
+    """The reference's menu-h table (src/bexpect.f90:84-180) for the states of
+    <name>.wfn and the seeded operator op<seed>.int, kept as <name>_h.xme.
+    One OpenMP thread: with several, the reference's applyHbundled_g path
+    (menus h, ex, wh) gives wrong matrices."""
+    case = small_case(*[c for c in SMALL if c[0] == name][0][1:5], seed=op_seed)
+    case.name = "op%d" % op_seed
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        shutil.copy(os.path.join(HERE, name + ".wfn"), wd)
+        run_ref(wd, "h\n%s\n%s_h\n%s\n1 1 1 1\nend\n" % (name, name, case.name),
+                env_extra={"OMP_NUM_THREADS": "1"})
+        shutil.copy(os.path.join(wd, name + "_h.xme"), os.path.join(HERE, name + "_h.xme"))
+        print(name, "menu h with", case.name)
+    finally:
+        shutil.rmtree(wd)
+
+
 def ensemble(members=8):
     """Batched ensemble: the reference run on each of the first `members`
     Hamiltonians of the sd 4p4n ensemble (energies, J, T and the Lanczos
@@ -498,6 +517,9 @@ if __name__ == "__main__":
         autoscaled()
     if "exact" in want:
         exact()
+    for c in EXPECT:
+        if "h_" + c[0] in want:
+            hmatrix(*c)
     if "config2" in want:
         ensemble()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -408,3 +408,52 @@ def test_fortran_host_exact_matches_reference(casedir):
     # (state 5: -28.240175 exactly, printed -28.24018 here, -28.24017 there)
     assert np.abs(got[:, 0] - ref[:, 0]).max() < 1.1e-5, (got, ref)
     assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3
+
+
+def read_xme(path):
+    """State rows (E, J, T column, par) and the (i, f) -> <f|O|i> elements of
+    a menu-h .xme file."""
+    lines = open(path, errors="replace").read().splitlines()
+    i0 = [i for i, l in enumerate(lines) if l.strip().startswith("State")][0]
+    states, elems = [], {}
+    k = i0 + 1
+    while k < len(lines):
+        p = lines[k].split()
+        if len(p) == 6 and p[0].isdigit():
+            states.append([float(p[1]), float(p[3]), float(p[4]), int(p[5])])
+            k += 1
+        else:
+            break
+    for l in lines[k:]:
+        p = l.split()
+        if len(p) != 3:
+            continue
+        i, f = int(p[0]), int(p[1])
+        if i == -1:
+            break
+        elems[(i, f)] = float(p[2])
+    return np.array(states), elems
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,space,Z,N,jz2,op_seed", [("sd2p2n", "sd", 2, 2, 0, 5), ("pf2p1n_m3", "pf", 2, 1, 3, 6)])
+def test_fortran_host_menu_h_matches_reference(name, space, Z, N, jz2, op_seed, casedir):
+    """Menu h on the reference's own .wfn: the state table (with parity) and
+    the printed <f|O|i> elements equal the reference's single-thread run."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    op = small_case(space, Z, N, jz2, seed=op_seed)
+    op.name = "op%d" % op_seed
+    wd = os.path.join(casedir, "fh_" + name)
+    write_case(op, wd)
+    shutil.copy(os.path.join(gold, name + ".wfn"), wd)
+    r = _run("h\n%s\n%s_h\n%s\n1 1 1 1\nend\n" % (name, name, op.name), wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    gs, ge = read_xme(os.path.join(wd, name + "_h.xme"))
+    rs, re_ = read_xme(os.path.join(gold, name + "_h.xme"))
+    assert gs.shape == rs.shape == (5, 4)
+    assert np.abs(gs[:, :3] - rs[:, :3]).max() < 1e-5 and (gs[:, 3] == rs[:, 3]).all()
+    assert set(ge) == set(re_), (ge, re_)
+    for key, v in re_.items():
+        assert abs(ge[key] - v) <= 2e-6 * max(1.0, abs(v)), (key, ge[key], v)
