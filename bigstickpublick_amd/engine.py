@@ -139,6 +139,7 @@ class System:
 
     def _adopt(self, h, upload):
         L = load_library()
+        self._lib = L  # held so that close() still works at interpreter exit
         self.h = h
         d = ctypes.c_int64()
         _check(L.bs_dim(h, ctypes.byref(d)))
@@ -152,7 +153,7 @@ class System:
 
     def close(self):
         if getattr(self, "h", None):
-            load_library().bs_finalize(self.h)
+            self._lib.bs_finalize(self.h)
             self.h = None
 
     __del__ = close
@@ -502,6 +503,7 @@ class Batch:
         h = ctypes.c_void_p()
         _check(L.bs_batch_init(str(sps).encode(), names, len(intfiles), Z, N, jz2,
                                *[float(v) for v in scaling], ctypes.byref(h)))
+        self._lib = L  # held so that close() still works at interpreter exit
         self.h = h
         k, d = ctypes.c_int(), ctypes.c_int64()
         _check(L.bs_batch_size(h, ctypes.byref(k), ctypes.byref(d)))
@@ -510,7 +512,7 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            load_library().bs_batch_finalize(self.h)
+            self._lib.bs_batch_finalize(self.h)
             self.h = None
 
     __del__ = close
