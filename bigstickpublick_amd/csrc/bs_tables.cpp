@@ -14,6 +14,7 @@
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 
@@ -719,6 +720,20 @@ struct Builder {
       }
   }
 
+  // Runs body(f0, f1, chunk) over contiguous chunks of [0, n) on up to 16
+  // host threads; chunk c covers the c-th range, so results gathered per
+  // chunk and concatenated in chunk order equal a serial loop's.
+  template <typename F>
+  static int parallel_chunks(int64_t n, F body) {
+    int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    for (int c = 1; c < nt; c++) th.emplace_back(body, n * c / nt, n * (c + 1) / nt, c);
+    body(0, n / nt, 0);
+    for (auto& x : th) x.join();
+    return nt;
+  }
+
   void build_jumps(int it) {
     Species& S = t.sp[it];
     const int ns = t.nsps;
@@ -726,32 +741,44 @@ struct Builder {
     idx.reserve(S.mask.size() * 2);
     for (size_t i = 0; i < S.mask.size(); i++) idx[S.mask[i]] = (int32_t)i;
     const int64_t nsd = (int64_t)S.mask.size();
-    // one-body
+    // one-body: rows of final SD f, entries sorted by (class, source, op)
     OneBody& O = t.ob[it];
     O = OneBody();
     O.rowptr.assign(nsd + 1, 0);
     struct E1 { int cls; int32_t src; int16_t op; int8_t ph; };
-    std::vector<E1> row;
-    for (int64_t f = 0; f < nsd; f++) {
-      row.clear();
-      uint64_t fm = S.mask[f];
-      for (uint64_t ta = fm; ta; ta &= ta - 1) {
-        int a = __builtin_ctzll(ta);
-        uint64_t m1 = fm & ~(1ULL << a);
-        for (int c = 0; c < ns; c++) {
-          if (c != a && ((m1 >> c) & 1)) continue;
-          uint64_t im = m1 | (1ULL << c);
-          auto itr = idx.find(im);
-          if (itr == idx.end()) continue;  // outside the basis (cannot happen for full spaces)
-          int ph = fsign(im, c) * fsign(im & ~(1ULL << c), a);
-          row.push_back({t.class_of_op[a * ns + c], itr->second, (int16_t)(a * ns + c), (int8_t)ph});
+    struct Chunk1 { std::vector<int32_t> src; std::vector<int16_t> op; std::vector<int8_t> ph; };
+    std::vector<Chunk1> c1(16);
+    const int nt1 = parallel_chunks(nsd, [&](int64_t f0, int64_t f1, int c) {
+      std::vector<E1> row;
+      Chunk1& out = c1[c];
+      for (int64_t f = f0; f < f1; f++) {
+        row.clear();
+        uint64_t fm = S.mask[f];
+        for (uint64_t ta = fm; ta; ta &= ta - 1) {
+          int a = __builtin_ctzll(ta);
+          uint64_t m1 = fm & ~(1ULL << a);
+          for (int cc = 0; cc < ns; cc++) {
+            if (cc != a && ((m1 >> cc) & 1)) continue;
+            uint64_t im = m1 | (1ULL << cc);
+            auto itr = idx.find(im);
+            if (itr == idx.end()) continue;  // outside the basis (cannot happen for full spaces)
+            int ph = fsign(im, cc) * fsign(im & ~(1ULL << cc), a);
+            row.push_back({t.class_of_op[a * ns + cc], itr->second, (int16_t)(a * ns + cc), (int8_t)ph});
+          }
         }
+        std::sort(row.begin(), row.end(), [](const E1& x, const E1& y) {
+          return x.cls != y.cls ? x.cls < y.cls : (x.src != y.src ? x.src < y.src : x.op < y.op);
+        });
+        for (auto& e : row) { out.src.push_back(e.src); out.op.push_back(e.op); out.ph.push_back(e.ph); }
+        O.rowptr[f + 1] = (int64_t)row.size();
       }
-      std::sort(row.begin(), row.end(), [](const E1& x, const E1& y) {
-        return x.cls != y.cls ? x.cls < y.cls : (x.src != y.src ? x.src < y.src : x.op < y.op);
-      });
-      for (auto& e : row) { O.src.push_back(e.src); O.op.push_back(e.op); O.phase.push_back(e.ph); }
-      O.rowptr[f + 1] = (int64_t)O.src.size();
+    });
+    for (int64_t f = 0; f < nsd; f++) O.rowptr[f + 1] += O.rowptr[f];
+    O.src.reserve(O.rowptr[nsd]);
+    for (int c = 0; c < nt1; c++) {
+      O.src.insert(O.src.end(), c1[c].src.begin(), c1[c].src.end());
+      O.op.insert(O.op.end(), c1[c].op.begin(), c1[c].op.end());
+      O.phase.insert(O.phase.end(), c1[c].ph.begin(), c1[c].ph.end());
     }
     // two-body, merged per source
     TwoBody& T = t.tb[it];
@@ -761,53 +788,66 @@ struct Builder {
     if (S.np >= 2) {
       const std::vector<float>& v = it == 0 ? t.vpp : t.vnn;
       const int np = t.npairxx;
-      std::vector<std::pair<int32_t, double>> acc;
-      for (int64_t f = 0; f < nsd; f++) {
-        acc.clear();
-        uint64_t fm = S.mask[f];
-        for (uint64_t ti = fm; ti; ti &= ti - 1) {
-          int i = __builtin_ctzll(ti);
-          for (uint64_t tj = fm & ((1ULL << i) - 1); tj; tj &= tj - 1) {
-            int j = __builtin_ctzll(tj);
-            uint64_t m2 = fm & ~(1ULL << i) & ~(1ULL << j);
-            int mij = t.sm2[i] + t.sm2[j];
-            int cidx = i * (i - 1) / 2 + j;
-            for (int k = 1; k < ns; k++) {
-              if ((m2 >> k) & 1) continue;
-              for (int l = 0; l < k; l++) {
-                if ((m2 >> l) & 1) continue;
-                if (t.sm2[k] + t.sm2[l] != mij) continue;
-                float me = v[(size_t)cidx * np + (k * (k - 1) / 2 + l)];
-                uint64_t sm = m2 | (1ULL << k) | (1ULL << l);
-                // the reference stores and counts jumps whatever their element value
-                t.tb_unmerged[it][f] += (sm == fm) ? 2 : 1;
-                if (me == 0.0f && !t.keep_structural) continue;
-                // sign of a+_i a+_j a_l a_k acting on |s>
-                int sg = fsign(sm, k);
-                uint64_t s1 = sm & ~(1ULL << k);
-                sg *= fsign(s1, l);
-                uint64_t s2 = s1 & ~(1ULL << l);
-                sg *= fsign(s2, j);
-                uint64_t s3 = s2 | (1ULL << j);
-                sg *= fsign(s3, i);
-                auto itr = idx.find(sm);
-                if (itr == idx.end()) continue;
-                acc.push_back({itr->second, (double)me * (double)sg});
+      struct Chunk2 { std::vector<int32_t> src; std::vector<double> val; };
+      std::vector<Chunk2> c2(16);
+      const int nt2 = parallel_chunks(nsd, [&](int64_t f0, int64_t f1, int c) {
+        std::vector<std::pair<int32_t, double>> acc;
+        Chunk2& out = c2[c];
+        for (int64_t f = f0; f < f1; f++) {
+          acc.clear();
+          const size_t before = out.src.size();
+          uint64_t fm = S.mask[f];
+          for (uint64_t ti = fm; ti; ti &= ti - 1) {
+            int i = __builtin_ctzll(ti);
+            for (uint64_t tj = fm & ((1ULL << i) - 1); tj; tj &= tj - 1) {
+              int j = __builtin_ctzll(tj);
+              uint64_t m2 = fm & ~(1ULL << i) & ~(1ULL << j);
+              int mij = t.sm2[i] + t.sm2[j];
+              int cidx = i * (i - 1) / 2 + j;
+              for (int k = 1; k < ns; k++) {
+                if ((m2 >> k) & 1) continue;
+                for (int l = 0; l < k; l++) {
+                  if ((m2 >> l) & 1) continue;
+                  if (t.sm2[k] + t.sm2[l] != mij) continue;
+                  float me = v[(size_t)cidx * np + (k * (k - 1) / 2 + l)];
+                  uint64_t sm = m2 | (1ULL << k) | (1ULL << l);
+                  // the reference stores and counts jumps whatever their element value
+                  t.tb_unmerged[it][f] += (sm == fm) ? 2 : 1;
+                  if (me == 0.0f && !t.keep_structural) continue;
+                  // sign of a+_i a+_j a_l a_k acting on |s>
+                  int sg = fsign(sm, k);
+                  uint64_t s1 = sm & ~(1ULL << k);
+                  sg *= fsign(s1, l);
+                  uint64_t s2 = s1 & ~(1ULL << l);
+                  sg *= fsign(s2, j);
+                  uint64_t s3 = s2 | (1ULL << j);
+                  sg *= fsign(s3, i);
+                  auto itr = idx.find(sm);
+                  if (itr == idx.end()) continue;
+                  acc.push_back({itr->second, (double)me * (double)sg});
+                }
               }
             }
           }
+          std::sort(acc.begin(), acc.end(), [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) {
+            return x.first < y.first;
+          });
+          for (size_t q = 0; q < acc.size();) {
+            size_t r = q;
+            double sum = 0.0;
+            while (r < acc.size() && acc[r].first == acc[q].first) { sum += acc[r].second; r++; }
+            if (sum != 0.0 || t.keep_structural) { out.src.push_back(acc[q].first); out.val.push_back(sum); }
+            q = r;
+          }
+          T.rowptr[f + 1] = (int64_t)(out.src.size() - before);
         }
-        std::sort(acc.begin(), acc.end(), [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) {
-          return x.first < y.first;
-        });
-        for (size_t q = 0; q < acc.size();) {
-          size_t r = q;
-          double s = 0.0;
-          while (r < acc.size() && acc[r].first == acc[q].first) { s += acc[r].second; r++; }
-          if (s != 0.0 || t.keep_structural) { T.src.push_back(acc[q].first); T.val.push_back(s); }
-          q = r;
-        }
-        T.rowptr[f + 1] = (int64_t)T.src.size();
+      });
+      for (int64_t f = 0; f < nsd; f++) T.rowptr[f + 1] += T.rowptr[f];
+      T.src.reserve(T.rowptr[nsd]);
+      T.val.reserve(T.rowptr[nsd]);
+      for (int c = 0; c < nt2; c++) {
+        T.src.insert(T.src.end(), c2[c].src.begin(), c2[c].src.end());
+        T.val.insert(T.val.end(), c2[c].val.begin(), c2[c].val.end());
       }
     }
   }
