@@ -32,14 +32,15 @@ program bs_driver
   case ('h', 'H')
     call expectation_run(.true.)
   case ('o', 'O')
-    if (len_trim(opt) > 1) then
-      write(6, *) ' bs_driver: of the o options only o itself is supported'
+    ! oo runs the same applicator1b as o (src/bmenu.f90:465-478)
+    if (len_trim(opt) > 2 .or. (len_trim(opt) == 2 .and. opt(2:2) /= 'o' .and. opt(2:2) /= 'O')) then
+      write(6, *) ' bs_driver: of the o options only o and oo are supported'
       stop 1
     end if
     call onebody_run()
   case default
     write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function),', &
-                ' x (expectation values), h (operator matrix) and o (one-body operator applied to states)'
+                ' x (expectation values), h (operator matrix) and o, oo (one-body operator applied to states)'
     stop 1
   end select
 

@@ -339,16 +339,17 @@ def opme(name, space, fmt, J, T, seed):
         f.write("\n".join(out) + "\n")
 
 
-def apply1b(wfn_name, op):
+def apply1b(wfn_name, op, option="o"):
     """The reference's menu o (src/bdenslib1.f90:1645-1693): the one-body
     operator <op>.opme applied to every state of <wfn_name>.wfn, kept as
-    <wfn_name>_<op>.wfn."""
+    <wfn_name>_<op>.wfn (menu oo, src/bmenu.f90:465-478, which runs the same
+    applicator1b: <wfn_name>_<op>_oo.wfn)."""
     wd = tempfile.mkdtemp(prefix="bsgold_")
     try:
         shutil.copy(os.path.join(HERE, wfn_name + ".wfn"), wd)
         shutil.copy(os.path.join(HERE, op + ".opme"), wd)
-        out = "%s_%s" % (wfn_name, op)
-        run_ref(wd, "o\n%s\n%s\n%s\n" % (wfn_name, out, op))
+        out = "%s_%s" % (wfn_name, op) + ("" if option == "o" else "_" + option)
+        run_ref(wd, "%s\n%s\n%s\n%s\n" % (option, wfn_name, out, op))
         shutil.copy(os.path.join(wd, out + ".wfn"), os.path.join(HERE, out + ".wfn"))
         print(out)
     finally:
@@ -508,6 +509,8 @@ if __name__ == "__main__":
     for c in APPLY1B:
         if "apply1b" in want:
             apply1b(*c)
+    if "apply1b_oo" in want:
+        apply1b("sd2p2n", "sd_e2", "oo")
     for c in BLOCK:
         if "block" in want:
             block(*c)

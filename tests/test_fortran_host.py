@@ -239,19 +239,20 @@ def test_fortran_host_strength_matches_reference(name, menu, pivot, nkeep, niter
 
 @needs_driver
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,op", [("sd2p2n", "sd_e2"), ("pf2p1n_m3", "pf_xpn1")])
-def test_fortran_host_menu_o_matches_reference(name, op, casedir):
-    """Menu o writes O|state> for every state of the .wfn: header, E, J,
-    T(T+1) equal to the reference's menu-o file, vectors within FP32
-    rounding."""
+@pytest.mark.parametrize("name,op,option", [("sd2p2n", "sd_e2", "o"), ("pf2p1n_m3", "pf_xpn1", "o"),
+                                            ("sd2p2n", "sd_e2", "oo")])
+def test_fortran_host_menu_o_matches_reference(name, op, option, casedir):
+    """Menu o (and oo, the same application in the reference) writes
+    O|state> for every state of the .wfn: header, E, J, T(T+1) equal to the
+    reference's menu-o / menu-oo file, vectors within FP32 rounding."""
     from wfn_io import read_wfn
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    wd = os.path.join(casedir, "fo_%s_%s" % (name, op))
+    wd = os.path.join(casedir, "fo_%s_%s_%s" % (name, op, option))
     os.makedirs(wd, exist_ok=True)
     shutil.copy(os.path.join(gold, name + ".wfn"), wd)
     shutil.copy(os.path.join(gold, op + ".opme"), wd)
-    out = "%s_%s" % (name, op)
-    r = _run("o\n%s\n%s\n%s\n" % (name, out, op), wd)
+    out = "%s_%s" % (name, op) + ("" if option == "o" else "_" + option)
+    r = _run("%s\n%s\n%s\n%s\n" % (option, name, out, op), wd)
     assert r.returncode == 0, r.stdout + r.stderr
     got = read_wfn(os.path.join(wd, out + ".wfn"))
     ref = read_wfn(os.path.join(gold, out + ".wfn"))
