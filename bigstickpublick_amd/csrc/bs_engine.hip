@@ -2582,6 +2582,27 @@ BatchVals shared_vals(const bs_system* s, int K) {
   return v;
 }
 
+// A caller's pivot block for a block strength run (menu sb: the pivot block
+// read from a .wfn, read_pivot_block_choose, src/blanczoslib.f90): K vectors
+// [k][D] of dtype `dtype`; out: strength[i * nkeep + k] = |<Ritz k|pivot i>|^2
+// and pnorm2[i] = |pivot i|^2.
+struct BlockPivot {
+  const void* piv = nullptr;
+  int dtype = BS_F32;
+  double* strength = nullptr;
+  double* pnorm2 = nullptr;
+};
+
+// V[d * K + k] = piv[k * D + d]: K separate vectors into the interleaved block.
+template <typename T, typename S, int K>
+__global__ __launch_bounds__(kBlock) void k_load_block(T* __restrict__ V, const S* __restrict__ piv, int64_t D) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D * K; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = i / K;
+    const int k = (int)(i - d * K);
+    V[i] = (T)piv[(int64_t)k * D + d];
+  }
+}
+
 // Block Lanczos (lanczos_block, src/blanczosmain.f90:1804-2153): blocks of K
 // vectors, H applied to a block with the multi-vector kernel, full block
 // classical Gram-Schmidt against every stored block (FP64 sums, one summed
@@ -2589,10 +2610,11 @@ BatchVals shared_vals(const bs_system* s, int K) {
 // of its Gram matrix (spectral_decomp in the reference), Ritz pairs from the
 // block tridiagonal matrix, then the same FP64 normalization / Rayleigh
 // quotient / residual as run_lanczos.  The pivot block is seeded
-// pseudo-random (the reference draws it with random_number).
+// pseudo-random (the reference draws it with random_number), or the caller's
+// (bp): then the strength of each pivot in every Ritz state is returned too.
 template <typename T, int K>
 int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, void* evecs, double* resid,
-                      int* iters) {
+                      int* iters, const BlockPivot* bp = nullptr) {
   const int64_t D = s->t.dim;
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
@@ -2644,8 +2666,19 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
     HIP_OK(hipGetLastError());
     return true;
   };
-  k_pivot_random<T><<<g, kBlock, 0, st>>>(V.p, bsz, 0x5eed1234abcdULL);
-  HIP_OK(hipGetLastError());
+  if (bp && bp->piv) {
+    if (bp->dtype == BS_F32) k_load_block<T, float, K><<<g, kBlock, 0, st>>>(V.p, (const float*)bp->piv, D);
+    else k_load_block<T, double, K><<<g, kBlock, 0, st>>>(V.p, (const double*)bp->piv, D);
+    HIP_OK(hipGetLastError());
+    std::vector<double> S0;
+    gram(V.p, S0);
+    if (bp->pnorm2)
+      for (int k = 0; k < K; k++) bp->pnorm2[k] = S0[k * K + k];
+  } else {
+    k_pivot_random<T><<<g, kBlock, 0, st>>>(V.p, bsz, 0x5eed1234abcdULL);
+    HIP_OK(hipGetLastError());
+  }
+  // pivot block = V_0 B0 (B0 from the spectral orthonormalization)
   std::vector<double> B0;
   if (!orthonormalize(V.p, V.p, B0, 0.0)) return set_err("pivot block is rank deficient");
   std::vector<std::vector<double>> A, B;  // diagonal and sub-diagonal blocks
@@ -2709,6 +2742,13 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   ritz(nb);
   const int n = nb * K;
   for (int k = 0; k < nkeep; k++) evals[k] = e[k];
+  if (bp && bp->strength)
+    for (int i = 0; i < K; i++)
+      for (int k = 0; k < nkeep; k++) {
+        double o = 0.0;  // <Ritz k | pivot i> = sum_c Y(c, k) B0(c, i)
+        for (int c = 0; c < K; c++) o += Y[(size_t)c * n + k] * B0[c * K + i];
+        bp->strength[(size_t)i * nkeep + k] = o * o;
+      }
   if (evecs || resid || p->polish) {
     const int nb1 = grid_for(D) < 1024 ? grid_for(D) : 1024;
     DevBuf<double> dY, X64, HX, part1, dots;
@@ -2759,16 +2799,16 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
 
 template <typename T>
 int run_block_lanczos_k(bs_system* s, const bs_lanczos_params* p, int K, double* evals, void* evecs,
-                        double* resid, int* iters) {
+                        double* resid, int* iters, const BlockPivot* bp = nullptr) {
   switch (K) {
-    case 1: return run_block_lanczos<T, 1>(s, p, evals, evecs, resid, iters);
-    case 2: return run_block_lanczos<T, 2>(s, p, evals, evecs, resid, iters);
-    case 3: return run_block_lanczos<T, 3>(s, p, evals, evecs, resid, iters);
-    case 4: return run_block_lanczos<T, 4>(s, p, evals, evecs, resid, iters);
-    case 5: return run_block_lanczos<T, 5>(s, p, evals, evecs, resid, iters);
-    case 6: return run_block_lanczos<T, 6>(s, p, evals, evecs, resid, iters);
-    case 7: return run_block_lanczos<T, 7>(s, p, evals, evecs, resid, iters);
-    case 8: return run_block_lanczos<T, 8>(s, p, evals, evecs, resid, iters);
+    case 1: return run_block_lanczos<T, 1>(s, p, evals, evecs, resid, iters, bp);
+    case 2: return run_block_lanczos<T, 2>(s, p, evals, evecs, resid, iters, bp);
+    case 3: return run_block_lanczos<T, 3>(s, p, evals, evecs, resid, iters, bp);
+    case 4: return run_block_lanczos<T, 4>(s, p, evals, evecs, resid, iters, bp);
+    case 5: return run_block_lanczos<T, 5>(s, p, evals, evecs, resid, iters, bp);
+    case 6: return run_block_lanczos<T, 6>(s, p, evals, evecs, resid, iters, bp);
+    case 7: return run_block_lanczos<T, 7>(s, p, evals, evecs, resid, iters, bp);
+    case 8: return run_block_lanczos<T, 8>(s, p, evals, evecs, resid, iters, bp);
     default: return set_err("block size must be in 1..8");
   }
 }
@@ -3882,6 +3922,27 @@ int bs_block_lanczos(bs_system* s, const bs_lanczos_params* p, int block, double
     if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
     if (p->dtype == BS_F32) return run_block_lanczos_k<float>(s, p, block, evals, evecs, resid, iters);
     return run_block_lanczos_k<double>(s, p, block, evals, evecs, resid, iters);
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_block_strength(bs_system* s, const bs_lanczos_params* p, int block, const void* pivots, int pivot_dtype,
+                      double* pnorm2, double* strength, double* evals, void* evecs, double* resid, int* iters) {
+  try {
+    checked(s);
+    if (!p || !evals || !pivots) return set_err("null argument");
+    if (block < 1 || block > 8) return set_err("block size must be in 1..8");
+    if (p->dtype != BS_F32 && p->dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    if (pivot_dtype != BS_F32 && pivot_dtype != BS_F64) return set_err("pivot dtype must be BS_F32 or BS_F64");
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    BlockPivot bp;
+    bp.piv = pivots;
+    bp.dtype = pivot_dtype;
+    bp.strength = strength;
+    bp.pnorm2 = pnorm2;
+    if (p->dtype == BS_F32) return run_block_lanczos_k<float>(s, p, block, evals, evecs, resid, iters, &bp);
+    return run_block_lanczos_k<double>(s, p, block, evals, evecs, resid, iters, &bp);
   } catch (const std::exception& e) {
     return set_err(e.what());
   }

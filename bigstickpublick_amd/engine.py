@@ -59,6 +59,8 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
+    L.bs_block_strength.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, i32, vp, vp, vp, vp, vp,
+                                    ctypes.POINTER(i32)]
     L.bs_exact.argtypes = [vp, i32, i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_strength.argtypes = [vp, ctypes.POINTER(_Params), vp, ctypes.POINTER(ctypes.c_double), vp, vp, vp, vp,
                               vp, vp, vp, ctypes.POINTER(i32)]

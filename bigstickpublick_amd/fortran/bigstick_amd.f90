@@ -127,6 +127,16 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_block_lanczos
 
+    function bs_block_strength(sys, p, block, pivots, pivot_dtype, pnorm2, strength, evals, evecs, resid, iters) &
+        bind(C, name='bs_block_strength') result(rc)
+      import :: c_ptr, c_int, bs_lanczos_params
+      type(c_ptr), value :: sys, pivots, pnorm2, strength, evals, evecs, resid
+      type(bs_lanczos_params), intent(in) :: p
+      integer(c_int), value :: block, pivot_dtype
+      integer(c_int), intent(out) :: iters
+      integer(c_int) :: rc
+    end function bs_block_strength
+
     function bs_strength(sys, p, pivot, pivot_norm, ritz, strength, alpha, beta, evals, evecs, resid, iters) &
         bind(C, name='bs_strength') result(rc)
       import :: c_ptr, c_int, c_double, bs_lanczos_params

@@ -26,7 +26,12 @@ program bs_driver
   case ('n', 'N')
     call diagonalize_run(.not. (opt(2:2) == 's' .or. opt(2:2) == 'S'))
   case ('s', 'S')
-    call strength_run(opt(2:2) == 's' .or. opt(2:2) == 'S')
+    if (opt(2:2) == 'b' .or. opt(2:2) == 'B') then
+      ! sb, sbs: block strength function (src/bmenu.f90:612-627)
+      call block_strength_run(opt(3:3) == 's' .or. opt(3:3) == 'S')
+    else
+      call strength_run(opt(2:2) == 's' .or. opt(2:2) == 'S')
+    end if
   case ('x', 'X')
     call expectation_run(.false.)
   case ('h', 'H')
@@ -39,7 +44,7 @@ program bs_driver
     end if
     call onebody_run()
   case default
-    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss (strength function),', &
+    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss, sb, sbs (strength function),', &
                 ' x (expectation values), h (operator matrix) and o, oo (one-body operator applied to states)'
     stop 1
   end select
@@ -265,6 +270,127 @@ contains
     r = hipFree(dpiv)
     r = bs_finalize(sy)
   end subroutine strength_run
+
+  ! Menus sb / sbs: block Lanczos from a pivot block of states of a previous
+  ! .wfn (read_pivot_block_choose, src/blanczoslib.f90:522-707), fixed block
+  ! iterations; per pivot, its total strength and its Energy/Strength table
+  ! over every Ritz value (src/blanczoslib.f90:1146-1181), then (sb) the
+  ! state table and the .wfn of the Ritz vectors.
+  ! stdin: sb, .wfn name, output name, .int name, scaling line, end,
+  ! nkeep dimblock iterations (nkeep raised to dimblock*(iterations+1)),
+  ! c + first last (a contiguous list) or s + dimblock state numbers.
+  subroutine block_strength_run(suppress)
+    logical, intent(in) :: suppress
+    character(len=256) :: wfnname, oname, iname, l
+    character(len=:), allocatable :: wpath
+    real(c_float) :: s1, s2, s3, s4
+    type(c_ptr) :: sy, dpiv, dvec
+    integer(c_int64_t) :: d, esize
+    integer(c_int) :: r, nk, nfile, kb, nbiter, iters, dtype, variant, nf
+    integer(c_int), allocatable :: choice(:)
+    type(bs_lanczos_params) :: p
+    real(c_float), target :: pe(1), pj(1), pt(1)
+    real(c_double), allocatable, target :: stren(:), evals(:), resid(:), pn2(:)
+    character(len=16) :: listchar
+    integer :: i, j, u, c0, c1
+
+    read(5, '(a)') wfnname
+    read(5, '(a)') oname
+    read(5, '(a)') iname
+    s1 = 1.0
+    s2 = 1.0
+    s3 = 1.0
+    s4 = 1.0
+    if (.not. int_autoscaled(trim(first_word(iname)) // '.int')) read(5, *) s1, s2, s3, s4
+    read(5, '(a)') l
+    if (adjustl(l) /= 'end') then
+      write(6, *) ' bs_driver: one interaction file is supported'
+      stop 1
+    end if
+    read(5, *) nk, kb, nbiter
+    if (kb < 1 .or. kb > 8 .or. nbiter < 0) then
+      write(6, *) ' bs_driver: block dimension must be 1..8 and iterations >= 0'
+      stop 1
+    end if
+    nk = max(nk, kb * (nbiter + 1))
+    wpath = trim(first_word(wfnname)) // '.wfn'
+    r = bs_init_wfn(cstr(wpath), cstr(trim(first_word(iname)) // '.int'), s1, s2, s3, s4, sy)
+    call bs_check(r, 'bs_init_wfn')
+    r = bs_dim(sy, d)
+    r = bs_read_wfn(sy, cstr(wpath), 0, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, BS_F32, 0, nfile)
+    call bs_check(r, 'bs_read_wfn')
+    allocate(choice(kb))
+    read(5, '(a)') listchar
+    select case (adjustl(listchar))
+    case ('c', 'C')
+      read(5, *) c0, c1
+      if (c1 - c0 + 1 /= kb .or. c0 < 1 .or. c1 > nfile) then
+        write(6, '(a,i0,a,i0,a)') ' bs_driver: the pivot list must name ', kb, ' of the ', nfile, &
+                                  ' states (random fill is not supported)'
+        stop 1
+      end if
+      choice = [(c0 + i - 1, i = 1, kb)]
+    case ('s', 'S')
+      read(5, *) choice
+      if (any(choice < 1) .or. any(choice > nfile)) then
+        write(6, '(a,i0,a)') ' bs_driver: pivot states must lie in 1:', nfile, ' (random fill is not supported)'
+        stop 1
+      end if
+    case default
+      write(6, *) ' bs_driver: pivot list must be c (contiguous) or s (specified)'
+      stop 1
+    end select
+    dtype = dtype_from_env()
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    dpiv = device_alloc(int(kb, c_int64_t) * d * esize)
+    do i = 1, kb
+      r = bs_read_wfn(sy, cstr(wpath), choice(i) - 1, 1, c_loc(pe), c_loc(pj), c_loc(pt), &
+                      ptr_offset(dpiv, int(i - 1, c_int64_t) * d * esize), dtype, 1, nf)
+      call bs_check(r, 'bs_read_wfn')
+    end do
+    r = bs_autotune(sy, dtype, variant)
+    call bs_check(r, 'bs_autotune')
+    ! basis size dimblock*(iterations+1), capped by the dimension
+    nk = int(min(int(nk, c_int64_t), (d / kb) * kb))
+    allocate(stren(int(kb) * nk), evals(nk), resid(nk), pn2(kb))
+    p%nkeep = nk
+    p%niter = kb * (nbiter + 1)
+    p%thick = 0
+    p%max_restarts = 0
+    p%dtype = dtype
+    p%tol = 0.0d0
+    p%polish = 0     ! the tables print Ritz values, as the reference does
+    dvec = device_alloc(int(nk, c_int64_t) * d * esize)
+    r = bs_block_strength(sy, p, kb, dpiv, dtype, c_loc(pn2), c_loc(stren), c_loc(evals), dvec, c_loc(resid), iters)
+    call bs_check(r, 'bs_block_strength')
+
+    open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
+    write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(u, '(a,a)') '  pivot block read from ', wpath
+    write(u, '(1x,i0,a)') nk, '  iterations '
+    write(u, '(a)') '  '
+    do i = 1, kb
+      write(u, *) '###################'
+      write(u, *) ' For pivot state ', choice(i), ' total strength = ', pn2(i)
+      write(u, *) ' Energy   Strength  '
+      write(u, *) ' ______   ________  '
+      do j = 1, nk
+        write(u, 88) evals(j), stren((i - 1) * nk + j)
+      end do
+    end do
+88  format(3f12.6)
+    if (suppress) then
+      write(u, *) ' Note: not writing wfns to file, nor computing J, T '
+    else
+      call state_table(u, sy, dvec, nk, evals, dtype, d, .true., trim(first_word(oname)) // '.wfn')
+    end if
+    close(u)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', iters, ' matvecs'
+    r = hipFree(dvec)
+    r = hipFree(dpiv)
+    r = bs_finalize(sy)
+  end subroutine block_strength_run
 
   ! Menu o: the one-body operator of an .opme file applied to every state of
   ! a previous .wfn; <name>.wfn gets O|state>, not normalized, with each

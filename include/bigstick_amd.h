@@ -185,6 +185,20 @@ int bs_exact(bs_system *sys, int nkeep, int dtype, double *evals, void *evecs, d
 int bs_block_lanczos(bs_system *sys, const bs_lanczos_params *p, int block, double *evals, void *evecs,
                      double *resid, int *iters);
 
+/* Block strength function: bs_block_lanczos started from the caller's pivot
+ * block instead of a random one: `pivots` holds `block` device vectors one
+ * after another ([k][dim], pivot_dtype BS_F32 or BS_F64; not modified).
+ * pnorm2 (may be NULL) receives |pivot i|^2; strength (may be NULL)
+ * receives strength[i * nkeep + k] = |<Ritz k|pivot i>|^2 for every pivot i
+ * and kept Ritz state k; evals, evecs, resid, iters as in bs_block_lanczos
+ * (p->polish = 0 gives the Ritz values themselves, as the reference prints).
+ * Replaces menu sb / sbs (src/bmenu.f90:612-627): read_pivot_block_choose
+ * (src/blanczoslib.f90:522-707), lanczos_block from that block
+ * (src/blanczosmain.f90:1945-1953) and the per-pivot Energy/Strength tables
+ * (src/blanczoslib.f90:1146-1181). */
+int bs_block_strength(bs_system *sys, const bs_lanczos_params *p, int block, const void *pivots, int pivot_dtype,
+                      double *pnorm2, double *strength, double *evals, void *evecs, double *resid, int *iters);
+
 /* Strength function: bs_lanczos started from `pivot` (device, dim elements
  * in p->dtype; not modified) instead of the uniform vector, with a fixed
  * number of iterations (p->thick must be 0).  pivot_norm receives ||pivot||;

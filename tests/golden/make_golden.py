@@ -298,6 +298,36 @@ def strength(name, menu, pivot, nkeep, niter):
         shutil.rmtree(wd)
 
 
+# block strength function runs (menu sb / sbs, src/bmenu.f90:612-627):
+# (output, menu, .wfn, space, Z, N, 2Jz, interaction seed, nkeep, dimblock,
+#  block iterations, pivot list answer)
+BLOCK_STRENGTH = [("sd2p2n_sb", "sb", "sd2p2n_pivot", "sd", 2, 2, 0, 5, 8, 2, 3, "c\n1 2"),
+                  ("pf2p1n_m3_sb", "sb", "pf2p1n_m3", "pf", 2, 1, 3, 6, 9, 3, 2, "s\n1 3 5"),
+                  ("sd2p2n_sbs", "sbs", "sd2p2n", "sd", 2, 2, 0, 5, 6, 3, 1, "c\n2 4")]
+
+
+def block_strength(out, menu, wfn_name, space, Z, N, jz2, seed, nkeep, dimblock, nbiter, pick):
+    """The reference's block strength function: block Lanczos from a pivot
+    block of states of <wfn_name>.wfn with the seeded interaction
+    int<seed>.int, kept as <out>.res (per pivot state: total strength and the
+    Energy/Strength table, src/blanczoslib.f90:1146-1181)."""
+    case = small_case(space, Z, N, jz2, seed=seed)
+    case.name = "int%d" % seed
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        shutil.copy(os.path.join(HERE, wfn_name + ".wfn"), wd)
+        # one OpenMP thread: with several, the reference's block apply gives
+        # Ritz values below the lowest eigenvalue of H
+        run_ref(wd, "%s\n%s\n%s\n%s\n1 1 1 1\nend\n%d %d %d\n%s\n" %
+                (menu, wfn_name, out, case.name, nkeep, dimblock, nbiter, pick),
+                env_extra={"OMP_NUM_THREADS": "1"})
+        shutil.copy(os.path.join(wd, out + ".res"), os.path.join(HERE, out + ".res"))
+        print(out)
+    finally:
+        shutil.rmtree(wd)
+
+
 # one-body operators for menu o: (name, space, format, J, T, seed)
 OPME = [("sd_e2", "sd", "iso", 2, 0, 31), ("sd_gt", "sd", "iso", 1, 1, 32), ("sd_pns2", "sd", "pns", 2, 0, 33),
         ("pf_xpn1", "pf", "xpn", 1, 0, 34)]
@@ -509,6 +539,9 @@ if __name__ == "__main__":
     for c in APPLY1B:
         if "apply1b" in want:
             apply1b(*c)
+    for c in BLOCK_STRENGTH:
+        if "block_strength" in want:
+            block_strength(*c)
     if "apply1b_oo" in want:
         apply1b("sd2p2n", "sd_e2", "oo")
     for c in BLOCK:

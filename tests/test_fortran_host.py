@@ -1,6 +1,7 @@
 """The Fortran host (bigstickpublick_amd/fortran/bs_driver, iso_c_binding to the
 C ABI) driven by the reference's own menu answers."""
 import os
+import re
 import shutil
 import subprocess
 
@@ -458,3 +459,116 @@ def test_fortran_host_menu_h_matches_reference(name, space, Z, N, jz2, op_seed, 
     assert set(ge) == set(re_), (ge, re_)
     for key, v in re_.items():
         assert abs(ge[key] - v) <= 2e-6 * max(1.0, abs(v)), (key, ge[key], v)
+
+
+def read_block_strength(path):
+    """Per pivot state of a block-strength .res (src/blanczoslib.f90:
+    1146-1181): {state: (total strength, rows of (Energy, Strength))}."""
+    lines = open(path, errors="replace").read().splitlines()
+    out, k = {}, 0
+    while k < len(lines):
+        m = re.match(r"\s*For pivot state\s+(\d+)\s+total strength =\s*(\S+)", lines[k])
+        if not m:
+            k += 1
+            continue
+        rows, k = [], k + 3
+        while k < len(lines) and re.match(r"^\s*-?\d+\.\d{6}\s+-?\d+\.\d{6}\s*$", lines[k]):
+            rows.append((float(lines[k][:12]), float(lines[k][12:24])))
+            k += 1
+        out[int(m.group(1))] = (float(m.group(2)), np.array(rows))
+    return out
+
+
+BLOCK_STRENGTH = [
+    ("sd2p2n_sb", "sb", "sd2p2n_pivot", "sd", 2, 2, 0, 5, 8, 2, 3, [1, 2], "c\n1 2"),
+    ("pf2p1n_m3_sb", "sb", "pf2p1n_m3", "pf", 2, 1, 3, 6, 9, 3, 2, [1, 3, 5], "s\n1 3 5"),
+    ("sd2p2n_sbs", "sbs", "sd2p2n", "sd", 2, 2, 0, 5, 6, 3, 1, [2, 3, 4], "c\n2 4")]
+
+
+def block_krylov_oracle(wfn, space, Z, N, jz2, seed, nbiter, picks, wd):
+    """Dense FP64 block Krylov space span{P, HP, ..., H^nbiter P} of the
+    pivot states P with the oracle's H: its Ritz values, the strength
+    |<Ritz k|p_i>|^2 of every pivot, |p_i|^2, and H's lowest eigenvalue."""
+    from oracle_api import Oracle
+    from wfn_io import read_wfn
+    o = Oracle(os.path.join(wd, space + ".sps"), os.path.join(wd, "int%d.int" % seed), Z, N, jz2)
+    H = o.dense()
+    w = read_wfn(os.path.join(GOLD, wfn + ".wfn"))
+    P = np.stack([w["states"][i - 1]["vec"].astype(np.float64) for i in picks], 1)
+    blocks = [np.linalg.qr(P)[0]]
+    for _ in range(nbiter):
+        W = H @ blocks[-1]
+        B = np.hstack(blocks)
+        for _ in range(2):
+            W -= B @ (B.T @ W)
+        blocks.append(np.linalg.qr(W)[0])
+    B = np.hstack(blocks)
+    e, Y = np.linalg.eigh(B.T @ H @ B)
+    ov = (B @ Y).T @ P
+    return e, (ov ** 2).T, (P * P).sum(0), np.linalg.eigvalsh(H)[0]
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _block_case(out, space, Z, N, jz2, seed, wfn, casedir):
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case(space, Z, N, jz2, seed=seed)
+    case.name = "int%d" % seed
+    wd = os.path.join(casedir, "fbs_" + out)
+    write_case(case, wd)
+    shutil.copy(os.path.join(GOLD, wfn + ".wfn"), wd)
+    return case, wd
+
+
+@pytest.mark.parametrize("out,menu,wfn,space,Z,N,jz2,seed,nkeep,kb,nbiter,picks,pick", BLOCK_STRENGTH)
+def test_reference_block_strength_is_not_a_ritz_spectrum(out, menu, wfn, space, Z, N, jz2, seed, nkeep, kb, nbiter,
+                                                         picks, pick, casedir):
+    """The reference's sb / sbs tables (one OpenMP thread) print the right
+    total strength |p_i|^2 for each pivot, but energies that are not the
+    Ritz values of the pivots' block Krylov space (all three cases are well
+    conditioned: every residual block has singular values above 4), and on
+    the pf case with orthonormal pivots the lowest lies below H's lowest
+    eigenvalue, which no Ritz value can.  So the menu is pinned to the
+    oracle's dense block Krylov space, not to these printed tables."""
+    _, wd = _block_case(out, space, Z, N, jz2, seed, wfn, casedir)
+    e, st, pn2, e0 = block_krylov_oracle(wfn, space, Z, N, jz2, seed, nbiter, picks, wd)
+    ref = read_block_strength(os.path.join(GOLD, out + ".res"))
+    assert sorted(ref) == sorted(picks)
+    for n, i in enumerate(picks):
+        assert abs(ref[i][0] - pn2[n]) < 1e-6 * max(1.0, pn2[n])
+    rows = ref[picks[0]][1]
+    if space == "pf":
+        assert rows[0, 0] < e0 - 0.05, (rows[0, 0], e0)
+    assert np.abs(rows[:, 0] - e).max() > 0.1
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("out,menu,wfn,space,Z,N,jz2,seed,nkeep,kb,nbiter,picks,pick", BLOCK_STRENGTH)
+def test_fortran_host_block_strength_matches_oracle(out, menu, wfn, space, Z, N, jz2, seed, nkeep, kb, nbiter,
+                                                     picks, pick, casedir):
+    """Menus sb / sbs: block Lanczos from a pivot block of stored states
+    (contiguous or listed).  Each pivot's total strength equals the
+    reference's; the Ritz energies (unconverged ones included: the Krylov
+    space is fixed by the pivot block) and every strength equal the oracle's
+    dense FP64 block Krylov space; sbs writes no state table."""
+    _, wd = _block_case(out, space, Z, N, jz2, seed, wfn, casedir)
+    r = _run("%s\n%s\n%s\nint%d\n1 1 1 1\nend\n%d %d %d\n%s\n" % (menu, wfn, out, seed, nkeep, kb, nbiter, pick),
+             wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_block_strength(os.path.join(wd, out + ".res"))
+    ref = read_block_strength(os.path.join(GOLD, out + ".res"))
+    e, st, pn2, _ = block_krylov_oracle(wfn, space, Z, N, jz2, seed, nbiter, picks, wd)
+    assert sorted(got) == sorted(picks)
+    for n, i in enumerate(picks):
+        gt, gr = got[i]
+        assert abs(gt - ref[i][0]) <= 2e-6 * max(1.0, gt), (i, gt, ref[i][0])
+        assert gr.shape == (kb * (nbiter + 1), 2)
+        assert np.abs(gr[:, 0] - e).max() < 1e-4, (gr[:, 0], e)
+        assert np.abs(gr[:, 1] - st[n]).max() < 2e-5, (gr[:, 1], st[n])
+    table = read_res(os.path.join(wd, out + ".res"))
+    if menu == "sb":
+        assert table.shape == (kb * (nbiter + 1), 3) and np.abs(table[:, 0] - e).max() < 1e-4
+    else:
+        assert table.size == 0

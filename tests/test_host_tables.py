@@ -152,3 +152,11 @@ def test_entry_points_refuse_bad_arguments(casedir, tmp_path):
                           (2, _Params(2, 10, 0, 0, BS_F32, 0.0, 1), b"not uploaded")):
         assert L.bs_block_lanczos(s.h, ctypes.byref(prm), blk, ev.ctypes.data, None, None, ctypes.byref(it)) != 0
         assert msg in L.bs_last_error(), L.bs_last_error()
+    # block strength: no pivots, block size, pivot dtype, no upload
+    piv = np.zeros(4, np.float32)
+    for blk, pv, pdt, msg in ((2, None, BS_F32, b"null argument"), (9, piv.ctypes.data, BS_F32, b"block size"),
+                              (2, piv.ctypes.data, 7, b"pivot dtype"), (2, piv.ctypes.data, BS_F32, b"not uploaded")):
+        prm = _Params(2, 10, 0, 0, BS_F32, 0.0, 0)
+        assert L.bs_block_strength(s.h, ctypes.byref(prm), blk, pv, pdt, None, None, ev.ctypes.data, None, None,
+                                   ctypes.byref(it)) != 0
+        assert msg in L.bs_last_error(), L.bs_last_error()
