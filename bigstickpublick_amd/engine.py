@@ -356,6 +356,31 @@ class System:
                                                res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, None, None, res, it.value, vec)
 
+    def block_strength(self, pivots, nblocks, dtype="f64", polish=False):
+        """Block strength function: block Lanczos from the pivot block
+        `pivots` (a (K, dim) CUDA tensor, float32 or float64) with nblocks
+        blocks; returns (Ritz values, strengths (K, K*nblocks) with
+        strength[i, k] = |<Ritz k|pivot i>|^2, |pivot i|^2, matvecs)."""
+        import torch
+        self.upload()
+        K = int(pivots.shape[0])
+        if pivots.dim() != 2 or pivots.shape[1] != self.dim or not pivots.is_cuda:
+            raise ValueError("pivots must be a (K, dim) CUDA tensor")
+        pv = pivots.contiguous()
+        pdt = BS_F32 if pv.dtype == torch.float32 else BS_F64
+        if pv.dtype not in (torch.float32, torch.float64):
+            raise ValueError("pivots must be float32 or float64")
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        n = K * nblocks
+        p = _Params(n, n, 0, 0, dt, 0.0, int(polish))
+        ev, st, pn2 = np.zeros(n), np.zeros(K * n), np.zeros(K)
+        it = ctypes.c_int()
+        torch.cuda.synchronize()
+        _check(load_library().bs_block_strength(self.h, ctypes.byref(p), K, ctypes.c_void_p(pv.data_ptr()), pdt,
+                                                pn2.ctypes.data, st.ctypes.data, ev.ctypes.data, None, None,
+                                                ctypes.byref(it)))
+        return ev, st.reshape(K, n), pn2, it.value
+
     def strength(self, pivot, nkeep, niter, polish=True, want_vectors=False):
         """Strength function: a fixed Lanczos run of niter steps started from
         `pivot` (a CUDA float32/float64 tensor of dim elements; its dtype is

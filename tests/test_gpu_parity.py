@@ -239,6 +239,34 @@ def test_block_lanczos_small(block, casedir):
 
 
 @pytest.mark.gpu
+def test_block_strength_moments(casedir):
+    """bs_block_strength on sd 2p2n from three seeded pivots, 6 blocks: the
+    strengths of each pivot sum to |p|^2 and reproduce its moments
+    sum_k S_ik E_k^n = <p_i|H^n|p_i> for n <= 2 * nblocks - 1 (Gauss block
+    quadrature) against the oracle's dense H, in FP64; the FP32 run gives
+    the same Ritz values to 1e-4."""
+    import torch
+    case, d, fx = small_fixture("sd2p2n", casedir)
+    s = _sys(case, d)
+    H = Oracle(*paths(case, d), case.Z, case.N, case.jz2).dense()
+    rng = np.random.default_rng(11)
+    P = rng.standard_normal((3, s.dim))
+    ev, st, pn2, mv = s.block_strength(torch.tensor(P, device="cuda"), 6, dtype="f64")
+    assert mv == 18
+    assert np.abs(pn2 - (P * P).sum(1)).max() < 1e-9 * pn2.max()
+    assert np.abs(st.sum(1) - pn2).max() < 1e-9 * pn2.max()
+    for i in range(3):
+        v = P[i].copy()
+        for n in range(12):
+            mom = P[i] @ v
+            scale = (st[i] * np.abs(ev) ** n).sum()
+            assert abs((st[i] * ev ** n).sum() - mom) < 1e-9 * max(1.0, scale), (i, n)
+            v = H @ v
+    ev32, st32, _, _ = s.block_strength(torch.tensor(P, device="cuda", dtype=torch.float32), 6, dtype="f32")
+    assert np.abs(ev32 - ev).max() < 1e-4
+
+
+@pytest.mark.gpu
 def test_block_lanczos_config0(casedir):
     """sd 4p4n (D = 28,503): block 4 with 300 basis vectors gives the lowest
     five of a 300-step single-vector FP64 run; with tol it stops earlier at
