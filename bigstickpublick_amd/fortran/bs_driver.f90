@@ -54,7 +54,8 @@ contains
   subroutine diagonalize_run(write_wfn)
     logical, intent(in) :: write_wfn
     character(len=256) :: outname, spsname, intname, line, lopt
-    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter
+    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter, ncheck, ctest
+    real(c_double) :: ctol
     real(c_float) :: spscale, a0, a, x
     type(c_ptr) :: sys, dvec
     integer(c_int64_t) :: dim, nbytes, esize
@@ -112,6 +113,24 @@ contains
       read(5, *, iostat=ios) maxrest
       if (ios /= 0) maxrest = 20
     end if
+    ctol = 1.0d-6
+    if (lopt(2:2) == 'c' .or. lopt(2:2) == 'C') then
+      ! lc / bc / tc: [additional states for the test,] test choice, tolerance
+      ! (src/blanczosmain.f90:943-985, :873-921, :1117-1175).  The test here is
+      ! the largest change of a kept energy; choice 0 (the reference's mean
+      ! change) is served by it, the wave-function tests 2 and 3 are not.
+      if (lopt(1:1) /= 't') read(5, *) ncheck
+      read(5, *) ctest
+      if (ctest /= 0 .and. ctest /= 1) then
+        write(6, *) ' bs_driver: only the energy convergence tests (0, 1) are supported'
+        stop 1
+      end if
+      read(5, *) ctol
+      if (.not. (ctol > 0.0d0)) then
+        write(6, *) ' bs_driver: the tolerance must be positive'
+        stop 1
+      end if
+    end if
 
     rc = bs_init(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
                  z, n, jz2, spscale, a0, a, x, sys)
@@ -132,7 +151,7 @@ contains
     p%max_restarts = maxrest
     p%dtype = dtype
     p%tol = 0.0d0
-    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c') p%tol = 1.0d-6
+    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'C') p%tol = ctol
     p%polish = 1
     nbytes = int(nkeep, c_int64_t) * dim * esize
     dvec = device_alloc(nbytes)

@@ -328,6 +328,32 @@ def block_strength(out, menu, wfn_name, space, Z, N, jz2, seed, nkeep, dimblock,
         shutil.rmtree(wd)
 
 
+# Lanczos with a user tolerance (options lc, tc, bc): (case, option, answers
+# after the option)
+CONVERGE = [("sd2p2n", "lc", ["5 300", "2", "1", "1e-6"]),
+            ("sd2p2n", "tc", ["5 30", "12", "40", "1", "1e-6"]),
+            ("pf2p2n", "bc", ["5 3 100", "2", "0", "1e-6"])]
+
+
+def converge(name, option, answers):
+    """The reference's run of option lc / tc / bc (src/blanczosmain.f90:
+    943-985, :1117-1175, :873-921) to its tolerance, kept as
+    <name>_<option>.res."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    lines = ["n", name, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), name, "1 1 1 1", "end",
+             option] + answers
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, "\n".join(lines) + "\n")
+        shutil.copy(os.path.join(wd, name + ".res"), os.path.join(HERE, "%s_%s.res" % (name, option)))
+        print(name, option, read_res(os.path.join(wd, name + ".res"))[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 # one-body operators for menu o: (name, space, format, J, T, seed)
 OPME = [("sd_e2", "sd", "iso", 2, 0, 31), ("sd_gt", "sd", "iso", 1, 1, 32), ("sd_pns2", "sd", "pns", 2, 0, 33),
         ("pf_xpn1", "pf", "xpn", 1, 0, 34)]
@@ -542,6 +568,9 @@ if __name__ == "__main__":
     for c in BLOCK_STRENGTH:
         if "block_strength" in want:
             block_strength(*c)
+    for c in CONVERGE:
+        if "converge" in want:
+            converge(*c)
     if "apply1b_oo" in want:
         apply1b("sd2p2n", "sd_e2", "oo")
     for c in BLOCK:

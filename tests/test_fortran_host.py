@@ -572,3 +572,45 @@ def test_fortran_host_block_strength_matches_oracle(out, menu, wfn, space, Z, N,
         assert table.shape == (kb * (nbiter + 1), 3) and np.abs(table[:, 0] - e).max() < 1e-4
     else:
         assert table.size == 0
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,option,answers", [
+    ("sd2p2n", "lc", ["5 300", "2", "1", "1e-6"]),
+    ("sd2p2n", "tc", ["5 30", "12", "40", "1", "1e-6"]),
+    ("pf2p2n", "bc", ["5 3 100", "2", "0", "1e-6"])])
+def test_fortran_host_user_tolerance_matches_reference(name, option, answers, casedir):
+    """Options lc, tc, bc (Lanczos, thick-restart, block Lanczos to a user
+    tolerance: the reference's extra answers are read, the energy tests 0 /
+    1 honoured): converged energies, J and T equal the reference's run of
+    the same menu script."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    meta = {"sd2p2n": ("sd", 2, 2, 0, 77), "pf2p2n": ("pf", 2, 2, 0, 84)}[name]
+    case = small_case(*meta[:4], seed=meta[4])
+    case.name = name
+    wd = os.path.join(casedir, "fconv_%s_%s" % (name, option))
+    write_case(case, wd)
+    lines = ["n", name, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), name, "1 1 1 1", "end",
+             option] + answers
+    r = _run("\n".join(lines) + "\n", wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_res(os.path.join(wd, name + ".res"))
+    ref = read_res(os.path.join(GOLD, "%s_%s.res" % (name, option)))
+    assert got.shape == ref.shape == (5, 3)
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 1e-4, (got[:, 0], ref[:, 0])
+    assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3, (got, ref)
+
+
+@needs_driver
+def test_fortran_host_refuses_wavefunction_convergence_test(casedir):
+    """Option lc with the wave-function convergence tests (2, 3) is refused
+    with a message, before any table is built."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    wd = os.path.join(casedir, "fconv_refuse")
+    write_case(case, wd)
+    lines = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end", "lc", "5 300", "2", "3", "1e-6"]
+    r = _run("\n".join(lines) + "\n", wd)
+    assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
