@@ -55,6 +55,7 @@ contains
     logical, intent(in) :: write_wfn
     character(len=256) :: outname, spsname, intname, line, lopt
     integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter, ncheck, ctest
+    integer(8) :: t0, t1, t2, trate
     real(c_double) :: ctol
     real(c_float) :: spscale, a0, a, x
     type(c_ptr) :: sys, dvec
@@ -94,6 +95,30 @@ contains
     read(5, '(a)') lopt
     lopt = adjustl(lopt)
     dimblock = 0
+    if (lopt(1:2) == 'sk') then
+      ! sk: set up only, to time it (src/blanczosmain.f90:1175-1179): tables
+      ! built, uploaded and the kernel variant tuned; no diagonalization
+      call system_clock(t0, trate)
+      rc = bs_init(cstr(trim(first_word(spsname)) // '.sps'), cstr(trim(first_word(intname)) // '.int'), &
+                   z, n, jz2, spscale, a0, a, x, sys)
+      call bs_check(rc, 'bs_init')
+      call system_clock(t1)
+      rc = bs_autotune(sys, dtype_from_env(), variant)
+      call bs_check(rc, 'bs_autotune')
+      call system_clock(t2)
+      rc = bs_dim(sys, dim)
+      open(newunit=iu, file=trim(first_word(outname)) // '.res', status='replace')
+      write(iu, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+      write(iu, '(1x,i0,1x,i0)') z, n
+      write(iu, '(1x,i0,a)') jz2, ' +'
+      write(iu, '(a,i0)') '  Total basis dimension: ', dim
+      write(iu, '(a,f10.3)') '  Time to build and upload the tables (s): ', real(t1 - t0) / real(trate)
+      write(iu, '(a,f10.3)') '  Time to tune the kernel (s): ', real(t2 - t1) / real(trate)
+      close(iu)
+      write(6, '(a,i0,a,f8.3,a)') ' bs_driver: dimension ', dim, ', setup ', real(t1 - t0) / real(trate), ' s'
+      rc = bs_finalize(sys)
+      return
+    end if
     if (lopt(1:2) == 'ex') then
       ! exact diagonalization: nkeep only (src/blanczosmain.f90:754-763)
       read(5, *) nkeep

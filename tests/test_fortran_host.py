@@ -614,3 +614,20 @@ def test_fortran_host_refuses_wavefunction_convergence_test(casedir):
     lines = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end", "lc", "5 300", "2", "3", "1e-6"]
     r = _run("\n".join(lines) + "\n", wd)
     assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_skip_lanczos_times_setup(casedir):
+    """Option sk (set up only, for timing): the .res names the basis
+    dimension and the setup times, and no diagonalization runs."""
+    case, d, fx = config_fixture(0, casedir)
+    wd = os.path.join(casedir, "fsk")
+    shutil.copytree(d, wd, dirs_exist_ok=True)
+    lines = menu_script(case, "n", "lf").splitlines()
+    i = lines.index("lf")
+    r = _run("\n".join(lines[:i] + ["sk"]) + "\n", wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    text = open(os.path.join(wd, case.name + ".res")).read()
+    assert "Total basis dimension: %d" % int(fx["dim"]) in text
+    assert "Time to build and upload the tables" in text and "State" not in text
