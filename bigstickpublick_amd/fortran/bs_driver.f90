@@ -123,6 +123,11 @@ contains
       ! exact diagonalization: nkeep only (src/blanczosmain.f90:754-763)
       read(5, *) nkeep
       niter = nkeep
+    else if (lopt(1:2) == 'li') then
+      ! li: Lanczos coefficients only, # iterations (src/blanczosmain.f90:787-795);
+      ! the .lcoef and the .res header are written, no eigenstates
+      read(5, *) niter
+      nkeep = 1
     else if (lopt(1:1) == 'b') then
       ! block Lanczos bf / bd: nkeep, block size, block iterations; the basis
       ! holds (iterations + 1) blocks (src/blanczosmain.f90:849-869)
@@ -188,6 +193,11 @@ contains
     else if (dimblock > 0) then
       rc = bs_block_lanczos(sys, p, dimblock, c_loc(evals), dvec, c_loc(resid), iters)
       call bs_check(rc, 'bs_block_lanczos')
+    else if (lopt(1:2) == 'li') then
+      p%polish = 0
+      resid = 0.0d0
+      rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), c_null_ptr, c_null_ptr, iters)
+      call bs_check(rc, 'bs_lanczos')
     else
       rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
       call bs_check(rc, 'bs_lanczos')
@@ -200,7 +210,8 @@ contains
     write(iu, '(1x,i0,a)') jz2, ' +'
     write(iu, '(1x,i0,a)') iters, '  iterations '
     write(iu, '(a)') '  '
-    call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
+    if (lopt(1:2) /= 'li') &
+      call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
     close(iu)
     if (dimblock == 0 .and. lopt(1:2) /= 'ex') then
       open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')

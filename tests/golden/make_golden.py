@@ -354,6 +354,23 @@ def converge(name, option, answers):
         shutil.rmtree(wd)
 
 
+def coefficients_only():
+    """The reference's hidden option li (src/blanczosmain.f90:787-795): 20
+    Lanczos iterations on sd 2p2n, coefficients only, kept as
+    sd2p2n_li.lcoef."""
+    meta = [c for c in SMALL if c[0] == "sd2p2n"][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = "sd2p2n"
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, "n\nsd2p2n_li\nsd\n2 2\n0\nsd2p2n\n1 1 1 1\nend\nli\n20\n")
+        shutil.copy(os.path.join(wd, "sd2p2n_li.lcoef"), os.path.join(HERE, "sd2p2n_li.lcoef"))
+        print("sd2p2n_li")
+    finally:
+        shutil.rmtree(wd)
+
+
 # one-body operators for menu o: (name, space, format, J, T, seed)
 OPME = [("sd_e2", "sd", "iso", 2, 0, 31), ("sd_gt", "sd", "iso", 1, 1, 32), ("sd_pns2", "sd", "pns", 2, 0, 33),
         ("pf_xpn1", "pf", "xpn", 1, 0, 34)]
@@ -571,6 +588,8 @@ if __name__ == "__main__":
     for c in CONVERGE:
         if "converge" in want:
             converge(*c)
+    if "li" in want:
+        coefficients_only()
     if "apply1b_oo" in want:
         apply1b("sd2p2n", "sd_e2", "oo")
     for c in BLOCK:

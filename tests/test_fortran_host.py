@@ -631,3 +631,23 @@ def test_fortran_host_skip_lanczos_times_setup(casedir):
     text = open(os.path.join(wd, case.name + ".res")).read()
     assert "Total basis dimension: %d" % int(fx["dim"]) in text
     assert "Time to build and upload the tables" in text and "State" not in text
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_coefficients_only_matches_reference(casedir):
+    """Option li (Lanczos coefficients only): the .lcoef of 20 iterations
+    from the uniform pivot equals the reference's (FP32 vectors, FP64
+    coefficients), and the .res holds no state table."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    wd = os.path.join(casedir, "fli")
+    write_case(case, wd)
+    r = _run("n\nsd2p2n_li\nsd\n2 2\n0\nsd2p2n\n1 1 1 1\nend\nli\n20\n", wd)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lc = np.loadtxt(os.path.join(wd, "sd2p2n_li.lcoef"))
+    rlc = np.loadtxt(os.path.join(GOLD, "sd2p2n_li.lcoef"))
+    assert lc.shape == rlc.shape == (20, 3)
+    assert np.abs(lc[:, 1:] - rlc[:, 1:]).max() < 2e-4, (lc, rlc)
+    assert read_res(os.path.join(wd, "sd2p2n_li.res")).size == 0
