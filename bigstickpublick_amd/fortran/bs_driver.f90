@@ -24,7 +24,12 @@ program bs_driver
   ! a second letter s suppresses the .wfn (menus ns, ss: src/bmenu.f90:572-580, :680-687)
   select case (opt(1:1))
   case ('n', 'N')
-    call diagonalize_run(.not. (opt(2:2) == 's' .or. opt(2:2) == 'S'))
+    if (opt(2:2) == 'p' .or. opt(2:2) == 'P') then
+      ! np: the spectrum from a pivot read from a previous .wfn (src/bmenu.f90:700-707)
+      call pivot_run()
+    else
+      call diagonalize_run(.not. (opt(2:2) == 's' .or. opt(2:2) == 'S'))
+    end if
   case ('s', 'S')
     if (opt(2:2) == 'b' .or. opt(2:2) == 'B') then
       ! sb, sbs: block strength function (src/bmenu.f90:612-627)
@@ -325,6 +330,97 @@ contains
     r = hipFree(dpiv)
     r = bs_finalize(sy)
   end subroutine strength_run
+
+  ! Menu np with option lf: Lanczos from one state of a previous .wfn as the
+  ! pivot (readpivot, src/bwfnlib.f90:298-366) for a fixed number of
+  ! iterations, then the usual .res state table, .wfn and .lcoef.
+  ! stdin: np, .wfn name, output name, .int name, scaling line, end, lf,
+  ! nkeep niter, which state is the pivot.
+  subroutine pivot_run()
+    character(len=256) :: wfnname, oname, iname, l, lopt
+    character(len=:), allocatable :: wpath
+    real(c_float) :: s1, s2, s3, s4
+    type(c_ptr) :: sy, dpiv, dvec
+    integer(c_int64_t) :: d, esize
+    integer(c_int) :: r, nk, nfile, ipiv, niter, iters, dtype, variant
+    type(bs_lanczos_params) :: p
+    real(c_float), target :: pe(1), pj(1), pt(1)
+    real(c_double), allocatable, target :: ritz(:), stren(:), alpha(:), beta(:), evals(:), resid(:)
+    real(c_double), target :: pnorm
+    integer :: j, u
+
+    read(5, '(a)') wfnname
+    read(5, '(a)') oname
+    read(5, '(a)') iname
+    s1 = 1.0
+    s2 = 1.0
+    s3 = 1.0
+    s4 = 1.0
+    if (.not. int_autoscaled(trim(first_word(iname)) // '.int')) read(5, *) s1, s2, s3, s4
+    read(5, '(a)') l
+    if (adjustl(l) /= 'end') then
+      write(6, *) ' bs_driver: one interaction file is supported'
+      stop 1
+    end if
+    read(5, '(a)') lopt
+    lopt = adjustl(lopt)
+    if (lopt(1:2) /= 'lf' .and. lopt(1:2) /= 'LF' .and. lopt(1:2) /= 'Lf') then
+      write(6, *) ' bs_driver: menu np supports the fixed-iteration option lf'
+      stop 1
+    end if
+    read(5, *) nk, niter
+    read(5, *) ipiv
+    wpath = trim(first_word(wfnname)) // '.wfn'
+    r = bs_init_wfn(cstr(wpath), cstr(trim(first_word(iname)) // '.int'), s1, s2, s3, s4, sy)
+    call bs_check(r, 'bs_init_wfn')
+    r = bs_dim(sy, d)
+    r = bs_read_wfn(sy, cstr(wpath), 0, 0, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, BS_F32, 0, nfile)
+    call bs_check(r, 'bs_read_wfn')
+    if (ipiv < 1 .or. ipiv > nfile) then
+      write(6, '(a,i0)') ' Selected vector is not in range 1:', nfile
+      stop 1
+    end if
+    dtype = dtype_from_env()
+    esize = 4
+    if (dtype == BS_F64) esize = 8
+    dpiv = device_alloc(d * esize)
+    r = bs_read_wfn(sy, cstr(wpath), ipiv - 1, 1, c_loc(pe), c_loc(pj), c_loc(pt), dpiv, dtype, 1, nfile)
+    call bs_check(r, 'bs_read_wfn')
+    r = bs_autotune(sy, dtype, variant)
+    call bs_check(r, 'bs_autotune')
+    niter = int(min(int(niter, c_int64_t), d))
+    nk = min(nk, niter)
+    allocate(ritz(niter), stren(niter), alpha(niter), beta(niter), evals(nk), resid(nk))
+    p%nkeep = nk
+    p%niter = niter
+    p%thick = 0
+    p%max_restarts = 0
+    p%dtype = dtype
+    p%tol = 0.0d0
+    p%polish = 1
+    alpha = 0.0d0
+    beta = 0.0d0
+    dvec = device_alloc(int(nk, c_int64_t) * d * esize)
+    r = bs_strength(sy, p, dpiv, pnorm, c_loc(ritz), c_loc(stren), c_loc(alpha), c_loc(beta), c_loc(evals), &
+                    dvec, c_loc(resid), iters)
+    call bs_check(r, 'bs_strength')
+    open(newunit=u, file=trim(first_word(oname)) // '.res', status='replace')
+    write(u, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
+    write(u, '(a,a,a,i0)') '  pivot read from ', wpath, ', state ', ipiv
+    write(u, '(1x,i0,a)') niter, '  iterations '
+    write(u, '(a)') '  '
+    call state_table(u, sy, dvec, nk, evals, dtype, d, .true., trim(first_word(oname)) // '.wfn')
+    close(u)
+    open(newunit=u, file=trim(first_word(oname)) // '.lcoef', status='replace')
+    do j = 1, niter
+      write(u, '(i10,2(2x,f20.15))') j, alpha(j), beta(j)
+    end do
+    close(u)
+    write(6, '(a,i0,a,i0,a)') ' bs_driver: dimension ', d, ', ', iters, ' matvecs'
+    r = hipFree(dvec)
+    r = hipFree(dpiv)
+    r = bs_finalize(sy)
+  end subroutine pivot_run
 
   ! Menus sb / sbs: block Lanczos from a pivot block of states of a previous
   ! .wfn (read_pivot_block_choose, src/blanczoslib.f90:522-707), fixed block

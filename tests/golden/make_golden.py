@@ -354,6 +354,25 @@ def converge(name, option, answers):
         shutil.rmtree(wd)
 
 
+def pivot_spectrum():
+    """The reference's menu np with option lf (src/bmenu.f90:700-707): 40
+    Lanczos iterations on sd 2p2n from state 1 of sd2p2n_pivot.wfn, kept as
+    sd2p2n_np.res and sd2p2n_np.lcoef."""
+    meta = [c for c in SMALL if c[0] == "sd2p2n"][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = "sd2p2n"
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        shutil.copy(os.path.join(HERE, "sd2p2n_pivot.wfn"), wd)
+        run_ref(wd, "np\nsd2p2n_pivot\nsd2p2n_np\nsd2p2n\n1 1 1 1\nend\nlf\n5 40\n1\n")
+        for ext in (".res", ".lcoef"):
+            shutil.copy(os.path.join(wd, "sd2p2n_np" + ext), os.path.join(HERE, "sd2p2n_np" + ext))
+        print("sd2p2n_np")
+    finally:
+        shutil.rmtree(wd)
+
+
 def coefficients_only():
     """The reference's hidden option li (src/blanczosmain.f90:787-795): 20
     Lanczos iterations on sd 2p2n, coefficients only, kept as
@@ -588,6 +607,8 @@ if __name__ == "__main__":
     for c in CONVERGE:
         if "converge" in want:
             converge(*c)
+    if "np" in want:
+        pivot_spectrum()
     if "li" in want:
         coefficients_only()
     if "apply1b_oo" in want:
