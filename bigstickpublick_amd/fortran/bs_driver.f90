@@ -98,7 +98,14 @@ contains
       return
     end if
     read(5, '(a)') lopt
-    lopt = adjustl(lopt)
+    lopt = lower(adjustl(lopt))
+    select case (lopt(1:2))
+    case ('ex', 'ld', 'lf', 'lc', 'td', 'tf', 'tc', 'bd', 'bf', 'bc', 'sk', 'li')
+    case default
+      write(6, *) ' bs_driver: diagonalization option ', trim(lopt), ' is not supported', &
+                  ' (supported: ex, ld, lf, lc, td, tf, tc, bd, bf, bc, sk, li)'
+      stop 1
+    end select
     dimblock = 0
     if (lopt(1:2) == 'sk') then
       ! sk: set up only, to time it (src/blanczosmain.f90:1175-1179): tables
@@ -782,6 +789,18 @@ contains
   end function int_autoscaled
 
   ! the reference's menu lines may carry a trailing '! comment'
+  ! ASCII lower case (the reference accepts LF, Lf and lf alike)
+  function lower(s) result(w)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: w
+    integer :: i, c
+    w = s
+    do i = 1, len(s)
+      c = ichar(s(i:i))
+      if (c >= ichar('A') .and. c <= ichar('Z')) w(i:i) = char(c + 32)
+    end do
+  end function lower
+
   function first_word(s) result(w)
     character(len=*), intent(in) :: s
     character(len=len(s)) :: w

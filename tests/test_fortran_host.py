@@ -676,3 +676,20 @@ def test_fortran_host_pivot_spectrum_matches_reference(casedir):
     rlc = np.loadtxt(os.path.join(GOLD, "sd2p2n_np.lcoef"))
     assert lc.shape == rlc.shape == (40, 3)
     assert np.abs(lc[:20, 1:] - rlc[:20, 1:]).max() < 2e-4, (lc[:20], rlc[:20])
+
+
+@needs_driver
+def test_fortran_host_refuses_unsupported_option(casedir):
+    """A diagonalization option the driver does not implement (tx: thick
+    restart near a target energy) is refused with a message, not misread;
+    upper-case spellings of supported ones pass that check."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    wd = os.path.join(casedir, "fopt_refuse")
+    write_case(case, wd)
+    head = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end"]
+    r = _run("\n".join(head + ["tx", "5 30"]) + "\n", wd)
+    assert r.returncode != 0 and "is not supported" in r.stdout, r.stdout + r.stderr
+    r = _run("\n".join(head + ["LC", "5 300", "2", "3", "1e-6"]) + "\n", wd)
+    assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
