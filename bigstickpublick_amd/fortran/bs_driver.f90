@@ -49,7 +49,7 @@ program bs_driver
     end if
     call onebody_run()
   case default
-    write(6, *) ' bs_driver: supported menu options are n, ns (diagonalization), s, ss, sb, sbs (strength function),', &
+    write(6, *) ' bs_driver: supported menu options are n, ns, np (diagonalization), s, ss, sb, sbs (strength function),', &
                 ' x (expectation values), h (operator matrix) and o, oo (one-body operator applied to states)'
     stop 1
   end select
