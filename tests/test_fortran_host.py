@@ -693,3 +693,21 @@ def test_fortran_host_refuses_unsupported_option(casedir):
     assert r.returncode != 0 and "is not supported" in r.stdout, r.stdout + r.stderr
     r = _run("\n".join(head + ["LC", "5 300", "2", "3", "1e-6"]) + "\n", wd)
     assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_block_strength_fp64(casedir):
+    """Menu sb with FP64 vectors (BS_DTYPE=f64): the printed energies and
+    strengths equal the oracle's block Krylov space to the print precision."""
+    out, menu, wfn, space, Z, N, jz2, seed, nkeep, kb, nbiter, picks, pick = BLOCK_STRENGTH[1]
+    _, wd = _block_case(out + "_f64", space, Z, N, jz2, seed, wfn, casedir)
+    r = _run("%s\n%s\n%s\nint%d\n1 1 1 1\nend\n%d %d %d\n%s\n" % (menu, wfn, out, seed, nkeep, kb, nbiter, pick),
+             wd, {"BS_DTYPE": "f64"})
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_block_strength(os.path.join(wd, out + ".res"))
+    e, st, pn2, _ = block_krylov_oracle(wfn, space, Z, N, jz2, seed, nbiter, picks, wd)
+    for n, i in enumerate(picks):
+        gt, gr = got[i]
+        assert abs(gt - pn2[n]) < 1e-9 * max(1.0, pn2[n])
+        assert np.abs(gr[:, 0] - e).max() < 2e-6 and np.abs(gr[:, 1] - st[n]).max() < 2e-6, (gr, e, st[n])
