@@ -23,13 +23,15 @@ class EigenEnergies(torch.autograd.Function):
         norb = len(template.orbits)
         it = Interaction(template.orbits, tuple(float(v) for v in th[:norb]),
                          [l[:6] + (float(v),) for l, v in zip(template.tbmes, th[norb:])])
-        d = tempfile.mkdtemp(prefix="bsgrad_")
-        sps, intf = os.path.join(d, "space.sps"), os.path.join(d, "h.int")
-        with open(sps, "w") as f:
-            f.write(sps_text(it.orbits))
-        with open(intf, "w") as f:
-            f.write(it.int_text())
-        s = System(sps, intf, Z, N, jz2, upload=True)
+        # the files are read once, by the host table build; the directory
+        # goes away as soon as the system holds its tables
+        with tempfile.TemporaryDirectory(prefix="bsgrad_") as d:
+            sps, intf = os.path.join(d, "space.sps"), os.path.join(d, "h.int")
+            with open(sps, "w") as f:
+                f.write(sps_text(it.orbits))
+            with open(intf, "w") as f:
+                f.write(it.int_text())
+            s = System(sps, intf, Z, N, jz2, upload=True)
         r = s.lanczos(nkeep=nkeep, niter=niter, thick=thick, max_restarts=50 if thick else 0,
                       dtype=dtype, tol=1e-10 if thick else 0.0, polish=True, want_vectors=True)
         ctx.system = s

@@ -92,6 +92,44 @@ def _check(rc):
         raise RuntimeError("bigstick_amd: " + load_library().bs_last_error().decode())
 
 
+def _dev(t, name, dim=None, shape=None, dtype=None, numel=None, contiguous=True):
+    """Checks a tensor handed to a device entry point before any library call:
+    a CUDA float32/float64 tensor (of `dtype` when given) with `numel`
+    elements or the given shape or trailing dimension `dim`, contiguous when
+    it is written by the library.  Raises ValueError otherwise, so that a host
+    tensor or a short buffer never reaches a kernel."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s must be a torch tensor" % name)
+    if not t.is_cuda:
+        raise ValueError("%s must be a CUDA tensor" % name)
+    if t.dtype not in (torch.float32, torch.float64):
+        raise ValueError("%s must be float32 or float64" % name)
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError("%s must have dtype %s (got %s)" % (name, dtype, t.dtype))
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError("%s must have shape %s (got %s)" % (name, tuple(shape), tuple(t.shape)))
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s must have %d elements (got %d)" % (name, numel, t.numel()))
+    if dim is not None and (t.dim() == 0 or t.shape[-1] != dim or t.numel() % dim):
+        raise ValueError("%s must have dim = %d columns" % (name, dim))
+    if contiguous and not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    return t
+
+
+def _out_like(y, x, name="y"):
+    """An output buffer for x: a new tensor, or the caller's y checked to have
+    x's dtype, device and size."""
+    import torch
+    if y is None:
+        return torch.empty_like(x)
+    _dev(y, name, dtype=x.dtype, numel=x.numel())
+    if y.device != x.device:
+        raise ValueError("%s must be on %s" % (name, x.device))
+    return y
+
+
 @dataclass
 class LanczosResult:
     evals: np.ndarray
@@ -192,12 +230,11 @@ class System:
     # ---- device side: torch tensors are used only as device buffers ----
     def apply(self, x, y=None, accumulate=False, stream=None):
         import torch
+        x = _dev(x, "x", numel=self.dim, contiguous=False).contiguous()
+        y = _out_like(y, x)
+        if y.data_ptr() == x.data_ptr():
+            raise ValueError("x and y must not overlap")
         self.upload()
-        if x.dtype not in (torch.float32, torch.float64) or not x.is_cuda or x.numel() != self.dim:
-            raise ValueError("x must be a contiguous CUDA float32/float64 tensor of length dim")
-        x = x.contiguous()
-        if y is None:
-            y = torch.empty_like(x)
         dt = BS_F32 if x.dtype == torch.float32 else BS_F64
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(load_library().bs_apply_h(self.h, ctypes.c_void_p(x.data_ptr()),
@@ -252,7 +289,7 @@ class System:
 
     def observable(self, which, psi):
         """<psi|J^2|psi> (which='J2') or <psi|T^2|psi> (which='T2') for a CUDA vector."""
-        import torch  # noqa: F401
+        psi = _dev(psi, "psi", numel=self.dim, contiguous=False).contiguous()
         self.upload()
         dt = BS_F32 if psi.dtype.itemsize == 4 else BS_F64
         v = ctypes.c_double()
@@ -267,6 +304,7 @@ class System:
 
     def density(self, psi):
         """(rho_pp, rho_nn, rho_pn, <psi|psi>) for a CUDA vector psi."""
+        psi = _dev(psi, "psi", numel=self.dim, contiguous=False).contiguous()
         self.upload()
         npx = self.me_table(0).shape[0]
         npn = self.me_table(2).shape[0]
@@ -281,6 +319,7 @@ class System:
 
     def gradient(self, psi):
         """dE/dtheta for theta = (SPEs, TBME lines in .int order), psi a CUDA vector."""
+        psi = _dev(psi, "psi", numel=self.dim, contiguous=False).contiguous()
         self.upload()
         nspe, ntb = self.num_params()
         g = np.zeros(nspe + ntb)
@@ -362,14 +401,12 @@ class System:
         blocks; returns (Ritz values, strengths (K, K*nblocks) with
         strength[i, k] = |<Ritz k|pivot i>|^2, |pivot i|^2, matvecs)."""
         import torch
-        self.upload()
-        K = int(pivots.shape[0])
-        if pivots.dim() != 2 or pivots.shape[1] != self.dim or not pivots.is_cuda:
+        if not isinstance(pivots, torch.Tensor) or pivots.dim() != 2:
             raise ValueError("pivots must be a (K, dim) CUDA tensor")
-        pv = pivots.contiguous()
+        K = int(pivots.shape[0])
+        pv = _dev(pivots, "pivots", shape=(K, self.dim), contiguous=False).contiguous()
         pdt = BS_F32 if pv.dtype == torch.float32 else BS_F64
-        if pv.dtype not in (torch.float32, torch.float64):
-            raise ValueError("pivots must be float32 or float64")
+        self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         n = K * nblocks
         p = _Params(n, n, 0, 0, dt, 0.0, int(polish))
@@ -386,10 +423,8 @@ class System:
         `pivot` (a CUDA float32/float64 tensor of dim elements; its dtype is
         the run's vector precision)."""
         import torch
+        pv = _dev(pivot, "pivot", numel=self.dim, contiguous=False).contiguous().reshape(-1)
         self.upload()
-        pv = pivot.contiguous().reshape(-1)
-        if pv.numel() != self.dim or pv.dtype not in (torch.float32, torch.float64):
-            raise ValueError("pivot must be a float32/float64 CUDA tensor of dim elements")
         dt = BS_F32 if pv.dtype == torch.float32 else BS_F64
         p = _Params(nkeep, niter, 0, 0, dt, 0.0, int(polish))
         m = min(niter, self.dim)
@@ -418,9 +453,9 @@ class System:
         """<v_f|O|v_i> for the two-body operator in `intfile` (iso .int; None:
         this system's Hamiltonian) and an (nvec, dim) CUDA tensor of vectors;
         returns an (nvec, nvec) array [i, f]."""
+        v = _dev(vecs, "vecs", dim=self.dim, contiguous=False).contiguous()
+        n = v.numel() // self.dim
         self.upload()
-        v = vecs.contiguous()
-        n = v.shape[0]
         out = np.zeros((n, n))
         dt = BS_F32 if v.dtype.itemsize == 4 else BS_F64
         name = None if intfile is None else str(intfile).encode()
@@ -432,9 +467,8 @@ class System:
         """O x for the one-body operator of an .opme file and a (dim,) or
         (nvec, dim) CUDA float32/float64 tensor; returns (y, J, T)."""
         import torch
-        xv = x.contiguous()
-        if xv.shape[-1] != self.dim or xv.dtype not in (torch.float32, torch.float64):
-            raise ValueError("x must be a float32/float64 CUDA tensor with dim columns")
+        xv = _dev(x, "x", dim=self.dim, contiguous=False).contiguous()
+        self.upload()
         y = torch.empty_like(xv)
         nvec = xv.numel() // self.dim
         j, t = ctypes.c_int(), ctypes.c_int()
@@ -488,9 +522,7 @@ class System:
                 raise ValueError("vecs must be float32/float64 of shape (nkeep, dim)")
             ptr, dt, dev = v.ctypes.data, (BS_F32 if v.dtype == np.float32 else BS_F64), 0
         else:
-            v = vecs.contiguous()
-            if v.numel() != nk * self.dim:
-                raise ValueError("vecs must have shape (nkeep, dim)")
+            v = _dev(vecs, "vecs", numel=nk * self.dim, contiguous=False).contiguous()
             ptr, dt, dev = v.data_ptr(), (BS_F32 if v.dtype.itemsize == 4 else BS_F64), 1
         _check(load_library().bs_write_wfn(self.h, str(path).encode(), nk, e.ctypes.data, j.ctypes.data,
                                            t.ctypes.data, ctypes.c_void_p(ptr), dt, dev))
@@ -498,12 +530,15 @@ class System:
     def apply_block(self, x, y=None):
         """y[i, k] = (H x)[i, k] for a (dim, K) CUDA tensor of K vectors, K <= 64."""
         import torch
-        self.upload()
-        if x.dim() != 2 or x.shape[0] != self.dim or x.dtype not in (torch.float32, torch.float64):
+        if not isinstance(x, torch.Tensor) or x.dim() != 2:
             raise ValueError("x must be a (dim, K) CUDA float32/float64 tensor")
-        x = x.contiguous()
-        if y is None:
-            y = torch.empty_like(x)
+        x = _dev(x, "x", shape=(self.dim, x.shape[1]), contiguous=False).contiguous()
+        if not 1 <= x.shape[1] <= 64:
+            raise ValueError("K must be in 1..64")
+        y = _out_like(y, x)
+        if y.data_ptr() == x.data_ptr():
+            raise ValueError("x and y must not overlap")
+        self.upload()
         dt = BS_F32 if x.dtype == torch.float32 else BS_F64
         _check(load_library().bs_apply_block(self.h, ctypes.c_void_p(x.data_ptr()),
                                              ctypes.c_void_p(y.data_ptr()), x.shape[1], dt))
@@ -567,11 +602,10 @@ class Batch:
     def apply(self, x, y=None):
         """y[:, k] = H_k x[:, k] for a (dim, K) CUDA tensor."""
         import torch
-        if tuple(x.shape) != (self.dim, self.K) or x.dtype not in (torch.float32, torch.float64):
-            raise ValueError("x must be a (dim, K) CUDA float32/float64 tensor")
-        x = x.contiguous()
-        if y is None:
-            y = torch.empty_like(x)
+        x = _dev(x, "x", shape=(self.dim, self.K), contiguous=False).contiguous()
+        y = _out_like(y, x)
+        if y.data_ptr() == x.data_ptr():
+            raise ValueError("x and y must not overlap")
         dt = BS_F32 if x.dtype == torch.float32 else BS_F64
         torch.cuda.synchronize()
         _check(load_library().bs_batch_apply(self.h, ctypes.c_void_p(x.data_ptr()),
@@ -605,9 +639,7 @@ class Batch:
         """dE_k/dtheta for each Hamiltonian: psi is a (dim, K) CUDA tensor, one
         eigenvector per Hamiltonian; returns (K, nparams)."""
         import torch
-        if tuple(psi.shape) != (self.dim, self.K):
-            raise ValueError("psi must be (dim, K)")
-        psi = psi.contiguous()
+        psi = _dev(psi, "psi", shape=(self.dim, self.K), contiguous=False).contiguous()
         g = np.zeros((self.K, self.num_params()))
         dt = BS_F32 if psi.dtype == torch.float32 else BS_F64
         torch.cuda.synchronize()
