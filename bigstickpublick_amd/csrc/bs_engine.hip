@@ -1248,6 +1248,129 @@ __global__ void k_bnormalize(T* __restrict__ v, const T* __restrict__ w, const d
     v[i] = (T)((double)w[i] * (1.0 / beta[i % K]));
 }
 
+// Breakdown handling of the batched recurrence, per Hamiltonian k, on the
+// device so that whole cycles stay captured (the rule of lanczos_p,
+// src/blanczosmain.f90:1634-1658): flags[k] = 2 when beta_i < 1e-4 of the
+// previous beta (|H v_i| on a cycle's first step i == kstart: sqrt(alpha^2 +
+// beta^2 + sum arrow^2)), 1 when beta_i < 0.1 of it, else 0; flags[K] = the
+// largest, so that the gated kernels below exit at once when nothing broke.
+__global__ void k_bbreak(const double* __restrict__ alpha, const double* __restrict__ beta,
+                         const double* __restrict__ arrow, int i, int kstart, int K, int* __restrict__ flags) {
+  const int k = threadIdx.x;  // one wave, K <= 64
+  int f = 0;
+  if (k < K) {
+    const double a = alpha[(int64_t)i * K + k], b = beta[(int64_t)i * K + k];
+    double scale;
+    if (i == kstart) {
+      double s2 = a * a + b * b;
+      for (int j = 0; j < kstart; j++) s2 += arrow[(int64_t)j * K + k] * arrow[(int64_t)j * K + k];
+      scale = sqrt(s2);
+    } else {
+      scale = beta[(int64_t)(i - 1) * K + k];
+    }
+    f = !(b >= 1e-4 * scale) || b == 0.0 ? 2 : (b < 0.1 * scale ? 1 : 0);
+    flags[k] = f;
+  }
+  for (int o = 32; o > 0; o >>= 1) f = max(f, __shfl_down(f, o, 64));
+  if (k == 0) flags[K] = f;
+}
+
+// v_{i+1}: w / beta for members without breakdown, a seeded random vector
+// (SplitMix64 of the element index) for members with flag 2, zero when the
+// Krylov space already spans the basis (last = 1)
+template <typename T>
+__global__ void k_bnext(T* __restrict__ v, const T* __restrict__ w, const double* __restrict__ beta, int64_t D, int K,
+                        const int* __restrict__ flags, uint64_t seed, int last) {
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % K);
+    T out;
+    if (last) {
+      out = T(0);
+    } else if (flags[k] == 2) {
+      uint64_t z = seed + (uint64_t)i * 0x9E3779B97F4A7C15ULL;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      z ^= z >> 31;
+      out = (T)(((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0) / sqrt((double)D));
+    } else {
+      out = (T)((double)w[i] * (1.0 / beta[k]));
+    }
+    v[i] = out;
+  }
+}
+
+// gated k_bmultidot: nothing unless some member has flags >= minflag
+template <typename T>
+__global__ __launch_bounds__(256) void k_bmultidot_g(const T* __restrict__ V, int64_t ldv, int m,
+                                                     const T* __restrict__ w, int64_t D, int K,
+                                                     double* __restrict__ part, const int* __restrict__ flags,
+                                                     int minflag) {
+  if (flags[K] < minflag) return;
+  __shared__ double red[4][64];
+  const int k = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t chunk = (D + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(lo + chunk, D);
+  for (int j = 0; j < m; j++) {
+    double a = 0.0;
+    if (k < K && flags[k] >= minflag)
+      for (int64_t st = lo + wv; st < hi; st += 4) a += (double)V[(int64_t)j * ldv + st * K + k] * (double)w[st * K + k];
+    red[wv][k] = a;
+    __syncthreads();
+    if (wv == 0 && k < K)
+      part[((int64_t)blockIdx.x * m + j) * K + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    __syncthreads();
+  }
+}
+
+__global__ void k_breduce_g(const double* __restrict__ part, int nb, int m, int K, double* __restrict__ out, int mode,
+                            const int* __restrict__ flags, int minflag) {
+  if (flags[K] < minflag) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * K) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(int64_t)b * m * K + idx];
+  out[idx] = mode == 1 ? sqrt(s) : s;
+}
+
+// members with flag 1 keep their vector unless it overlaps a stored one by
+// more than 1e-5 (br_check_orthogonal, src/breorthog.f90:977-1045): their
+// flag drops to 0 when it does not
+__global__ void k_bloss(const double* __restrict__ dots, int m, int K, int* __restrict__ flags) {
+  const int k = threadIdx.x;
+  if (k >= K || flags[k] != 1) return;
+  double dmax = 0.0;
+  for (int j = 0; j < m; j++) dmax = fmax(dmax, fabs(dots[(int64_t)j * K + k]));
+  if (dmax <= 1e-5) flags[k] = 0;
+}
+
+// w -= T(sum_j d_j V_j) and then w /= norm, for members with flags >= minflag
+template <typename T>
+__global__ void k_bmultiaxpy_m(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m,
+                               const double* __restrict__ d, int64_t D, int K, const int* __restrict__ flags,
+                               int minflag) {
+  if (flags[K] < minflag) return;
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % K);
+    if (flags[k] < minflag) continue;
+    double s = 0.0;
+    for (int j = 0; j < m; j++) s += d[j * K + k] * (double)V[(int64_t)j * ldv + i];
+    w[i] = w[i] - (T)s;
+  }
+}
+
+template <typename T>
+__global__ void k_bnormalize_m(T* __restrict__ v, const double* __restrict__ nrm, int64_t D, int K,
+                               const int* __restrict__ flags, int minflag) {
+  if (flags[K] < minflag) return;
+  const int64_t n = D * K;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % K);
+    if (flags[k] >= minflag) v[i] = (T)((double)v[i] * (1.0 / nrm[k]));
+  }
+}
+
 // Lowest nkeep eigenpairs of K symmetric tridiagonal matrices (alpha[i*K+k],
 // beta[i*K+k] = coupling of i and i+1), FP64, one thread per matrix: Sturm
 // bisection for the eigenvalues, inverse iteration (tridiagonal LU with
@@ -1509,6 +1632,8 @@ struct bs_system {
   bool jac_ready = false;
   std::vector<bs::ParamColumn> jac;
   bool uploaded = false;
+  int last_random_restarts = 0;  // breakdown handling of the last Lanczos run
+  int last_second_passes = 0;
   int cpt = 1;
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
   float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
@@ -1949,6 +2074,8 @@ void launch_apply(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st
 }
 
 // Lanczos workspace and loop
+constexpr double kSmallBeta = 1e-4;  // smallbetatol, src/blanczosmain.f90:1460
+
 // Strength-function extras: a pivot to start from (device, D elements of T;
 // NULL = the uniform pivot 1/sqrt(D)), and for every Ritz value of the final
 // projected matrix its value and |Y(0, k)|^2 ||pivot||^2.
@@ -1969,7 +2096,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   int niter = std::min<int64_t>(p->niter, D);
   if (nkeep < 1) return set_err("nkeep must be >= 1");
   if (niter < nkeep) return set_err("niter must be >= nkeep");
-  const int thick = p->thick;
+  // a Krylov space as large as the basis is exact after one cycle: no restart
+  const int thick = p->niter >= D ? 0 : p->thick;
   if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter))) return set_err("thick must be 0 or in [nkeep, niter)");
   const int mtot = niter + 1;
   const int nb = grid_for(D) < 1024 ? grid_for(D) : 1024;
@@ -2000,6 +2128,73 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   HIP_OK(hipGetLastError());
   if (pr.pivot_norm) *pr.pivot_norm = pnorm;
 
+  // v_{i+1} from w and beta_i, with the reference's breakdown handling
+  // (src/blanczosmain.f90:1634-1658): beta_i below smallbetatol = 1e-4 of
+  // the previous beta is a breakdown -- v_{i+1} becomes a seeded random
+  // vector orthonormalized against the whole history
+  // (random_restart_with_history, src/breorthog.f90:1471-1482) while beta_i
+  // keeps its small value; beta_i below 0.1 of the previous one triggers the
+  // orthogonality test (br_check_orthogonal, src/breorthog.f90:977-1045:
+  // any |<v_h, v_{i+1}>| > 1e-5) and then a second Gram-Schmidt pass.  The
+  // first step of a cycle has no previous beta; its scale is |H v_i|, which
+  // is sqrt(alpha^2 + beta^2 + sum of the arrow couplings^2) there.  When the
+  // Krylov space is the whole space (i + 1 == D) v_{i+1} is set to zero.
+  double beta_prev = 0.0;
+  int random_restarts = 0, second_passes = 0;
+  auto orthonormalize = [&](T* v, int m) {
+    k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, m, v, D, part.p);
+    k_reduce<<<m, kBlock, 0, st>>>(part.p, nb, m, dots.p, 0);
+    k_multiaxpy<T><<<g, kBlock, 0, st>>>(v, V.p, ld, m, dots.p, D);
+    k_multidot<T><<<nb, kBlock, 0, st>>>(v, D, 1, v, D, part.p);
+    k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, dots.p + m, 1);
+    k_normalize<T><<<g, kBlock, 0, st>>>(v, v, dots.p + m, D);
+  };
+  auto next_vector = [&](int i, int kst, const std::vector<double>& arr) {
+    T* vn = V.p + (int64_t)(i + 1) * ld;
+    double ab[2];
+    HIP_OK(hipMemcpyAsync(&ab[0], d_alpha + i, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&ab[1], d_beta + i, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const double bi = ab[1];
+    if (!std::isfinite(bi) || !std::isfinite(ab[0]))
+      throw std::runtime_error("Lanczos step " + std::to_string(i) + ": alpha or beta is not finite");
+    double scale = beta_prev;
+    if (i == kst) {
+      double s2 = ab[0] * ab[0] + bi * bi;
+      for (int j = 0; j < kst; j++) s2 += arr[j] * arr[j];
+      scale = std::sqrt(s2);
+    }
+    beta_prev = bi;
+    if (i + 1 >= D) {
+      k_fill<T><<<g, kBlock, 0, st>>>(vn, T(0), D);
+      return;
+    }
+    if (!(bi >= kSmallBeta * scale) || bi == 0.0) {
+      k_pivot_random<T><<<g, kBlock, 0, st>>>(vn, D, 0x7a3f0e1d5c2bULL + 0x9E3779B9ULL * (uint64_t)(i + 1));
+      k_multidot<T><<<nb, kBlock, 0, st>>>(vn, D, 1, vn, D, part.p);
+      k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, dots.p, 1);
+      k_normalize<T><<<g, kBlock, 0, st>>>(vn, vn, dots.p, D);
+      orthonormalize(vn, i + 1);
+      orthonormalize(vn, i + 1);
+      random_restarts++;
+      return;
+    }
+    k_normalize<T><<<g, kBlock, 0, st>>>(vn, w.p, d_beta + i, D);
+    if (bi < 0.1 * scale) {
+      const int m = i + 1;
+      k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, m, vn, D, part.p);
+      k_reduce<<<m, kBlock, 0, st>>>(part.p, nb, m, dots.p, 0);
+      std::vector<double> d(m);
+      HIP_OK(hipMemcpy(d.data(), dots.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+      double dmax = 0.0;
+      for (double v : d) dmax = std::max(dmax, std::fabs(v));
+      if (dmax > 1e-5) {
+        orthonormalize(vn, m);
+        second_passes++;
+      }
+    }
+  };
+
   std::vector<double> alpha_h, beta_h;  // current arrowhead/tridiagonal data
   std::vector<double> e_prev, e_check;
   int total_mv = 0;
@@ -2024,7 +2219,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       }
       k_multidot<T><<<nb, kBlock, 0, st>>>(w.p, D, 1, w.p, D, part.p);
       k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
-      k_normalize<T><<<g, kBlock, 0, st>>>(V.p + (int64_t)(i + 1) * ld, w.p, d_beta + i, D);
+      next_vector(i, kstart, arrow);
       HIP_OK(hipGetLastError());
       // plain runs with a tolerance (options ld, lc): every 10 steps the
       // lowest nkeep Ritz values of the tridiagonal so far; stop when none
@@ -2160,6 +2355,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   }
   HIP_OK(hipStreamSynchronize(st));
   if (iters) *iters = total_mv;
+  s->last_random_restarts = random_restarts;
+  s->last_second_passes = second_passes;
   return 0;
 }
 
@@ -2372,7 +2569,8 @@ struct bs_batch {
   struct Work {
     int dtype = 0, niter = 0, nkeep = 0, polish = 0, thick = 0;
     DevBuf<unsigned char> V, w, X64, HX, tmp;
-    DevBuf<double> part, dots, alpha, beta, evals, Y, tdwork, rq, res, A, arrow;
+    DevBuf<double> part, dots, alpha, beta, evals, Y, tdwork, rq, res, A, arrow, bnorm;
+    DevBuf<int> flags;  // breakdown flags per Hamiltonian, then their maximum
     hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
     hipStream_t stream = nullptr;
     ~Work() {
@@ -2840,7 +3038,34 @@ void enqueue_batch_steps(bs_batch* b, bs_batch::Work& w, int i0, int i1, hipStre
     }
     k_bmultidot<T><<<nb, 256, 0, st>>>(wv, DK, 1, wv, D, K, w.part.p);
     k_breduce<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.beta.p + (int64_t)i * K, 1);
-    k_bnormalize<T><<<g, kBlock, 0, st>>>(V + (int64_t)(i + 1) * DK, wv, w.beta.p + (int64_t)i * K, D, K);
+    // v_{i+1} with the breakdown handling of k_bbreak: random restarts
+    // (flag 2) are orthonormalized against v_0..v_i twice, members whose
+    // beta fell below 0.1 of the previous one (flag 1) once if they lost
+    // orthogonality; the gated kernels return at once when no member broke
+    T* vn = V + (int64_t)(i + 1) * DK;
+    const int m = i + 1;
+    k_bbreak<<<1, 64, 0, st>>>(w.alpha.p, w.beta.p, w.arrow.p, i, i0, K, w.flags.p);
+    k_bnext<T><<<g, kBlock, 0, st>>>(vn, wv, w.beta.p + (int64_t)i * K, D, K, w.flags.p,
+                                     0x7a3f0e1d5c2bULL + 0x9E3779B9ULL * (uint64_t)(i + 1), i + 1 >= D ? 1 : 0);
+    if (i + 1 < D) {
+      for (int pass = 0; pass < 3; pass++) {
+        const int minflag = pass == 2 ? 2 : 1;
+        if (pass == 0) {
+          // normalize the random vectors first
+          k_bmultidot_g<T><<<nb, 256, 0, st>>>(vn, DK, 1, vn, D, K, w.part.p, w.flags.p, 2);
+          k_breduce_g<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.bnorm.p, 1, w.flags.p, 2);
+          k_bnormalize_m<T><<<g, kBlock, 0, st>>>(vn, w.bnorm.p, D, K, w.flags.p, 2);
+          continue;
+        }
+        k_bmultidot_g<T><<<nb, 256, 0, st>>>(V, DK, m, vn, D, K, w.part.p, w.flags.p, minflag);
+        k_breduce_g<<<(m * K + 255) / 256, 256, 0, st>>>(w.part.p, nb, m, K, w.dots.p, 0, w.flags.p, minflag);
+        if (pass == 1) k_bloss<<<1, 64, 0, st>>>(w.dots.p, m, K, w.flags.p);
+        k_bmultiaxpy_m<T><<<g, kBlock, 0, st>>>(vn, V, DK, m, w.dots.p, D, K, w.flags.p, minflag);
+        k_bmultidot_g<T><<<nb, 256, 0, st>>>(vn, DK, 1, vn, D, K, w.part.p, w.flags.p, minflag);
+        k_breduce_g<<<(K + 255) / 256, 256, 0, st>>>(w.part.p, nb, 1, K, w.bnorm.p, 1, w.flags.p, minflag);
+        k_bnormalize_m<T><<<g, kBlock, 0, st>>>(vn, w.bnorm.p, D, K, w.flags.p, minflag);
+      }
+    }
   }
   HIP_OK(hipGetLastError());
 }
@@ -3022,7 +3247,8 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
   const int64_t D = b->base->t.dim, DK = D * K;
   const int niter = (int)std::min<int64_t>(p->niter, D);
   const int nkeep = p->nkeep;
-  const int thick = p->thick;
+  // a Krylov space as large as the basis is exact after one cycle: no restart
+  const int thick = p->niter >= D ? 0 : p->thick;
   if (nkeep < 1 || nkeep > 16) throw std::runtime_error("nkeep must be in 1..16");
   if (niter < nkeep) throw std::runtime_error("niter must be >= nkeep");
   if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter || thick > 16)))
@@ -3055,10 +3281,10 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
     w.tdwork.alloc((size_t)K * std::max(6 * niter, thick ? niter * niter : 0));
     w.rq.alloc((size_t)nkeep * K);
     w.res.alloc((size_t)nkeep * K);
-    if (thick) {
-      w.A.alloc((size_t)K * niter * niter);
-      w.arrow.alloc((size_t)std::max(thick, 1) * K);
-    }
+    if (thick) w.A.alloc((size_t)K * niter * niter);
+    w.arrow.alloc((size_t)std::max(thick, 1) * K);
+    w.bnorm.alloc((size_t)K);
+    w.flags.alloc((size_t)K + 1);
     HIP_OK(hipStreamCreate(&w.stream));
     batch_variant<T>(b->base.get(), b->bv);
     if (w.polish) batch_variant<double>(b->base.get(), b->bv);
@@ -3960,6 +4186,13 @@ int bs_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha, double* 
   } catch (const std::exception& e) {
     return set_err(e.what());
   }
+}
+
+int bs_lanczos_stats(const bs_system* s, int* random_restarts, int* second_passes) {
+  if (!s) return set_err("null system");
+  if (random_restarts) *random_restarts = s->last_random_restarts;
+  if (second_passes) *second_passes = s->last_second_passes;
+  return 0;
 }
 
 }  // extern "C"

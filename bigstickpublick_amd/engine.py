@@ -58,6 +58,7 @@ def load_library():
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
+    L.bs_lanczos_stats.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_block_strength.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, i32, vp, vp, vp, vp, vp,
                                     ctypes.POINTER(i32)]
@@ -355,6 +356,13 @@ class System:
                                          ctypes.c_void_p(vec.data_ptr()) if vec is not None else None,
                                          res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
+
+    def lanczos_stats(self):
+        """(random restarts, second Gram-Schmidt passes) of the last lanczos()
+        or strength() run."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(load_library().bs_lanczos_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def exact(self, nkeep, dtype="f64", want_vectors=False):
         """Exact diagonalization of the dense H (dim <= 65536): the nkeep

@@ -163,6 +163,18 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* Breakdown handling of the last bs_lanczos / bs_strength run on sys, as the
+ * reference does it inside lanczos_p (src/blanczosmain.f90:1634-1658):
+ * random_restarts counts steps whose beta fell below 1e-4 of the previous
+ * beta (|H v| on a cycle's first step) and whose next vector was replaced by
+ * a seeded random vector orthonormalized against the history
+ * (random_restart_with_history, src/breorthog.f90:1471-1482), beta itself
+ * kept; second_passes counts steps whose beta fell below 0.1 of the previous
+ * one and whose new vector then overlapped a stored one by more than 1e-5
+ * (br_check_orthogonal, src/breorthog.f90:977-1045) and was
+ * re-orthogonalized.  Either pointer may be NULL. */
+int bs_lanczos_stats(const bs_system *sys, int *random_restarts, int *second_passes);
+
 /* Exact diagonalization: the dense H (dim <= 65536, dim^2 doubles in HBM)
  * built 64 columns at a time with the multi-vector kernel in FP64, all
  * eigenpairs by rocSOLVER's symmetric divide-and-conquer solver; evals (nkeep

@@ -866,18 +866,31 @@ static int fsign(uint64_t mask, int c) {
  * order, protons to the left of neutrons.  H is real symmetric, so each row
  * y[s] is gathered from the states the operators reach from s:
  * y[s] = sum_t <t|H|s> x[t]; rows are independent (OpenMP over proton SDs). */
-void bso_apply(const bso_t *o, const double *x, double *y) {
+/* The states of the listed proton SDs, one OpenMP work item per state (the
+ * rows' states flattened, so that a few rows still spread over every
+ * thread). */
+static void apply_rows(const bso_t *o, const double *x, double *y, const int64_t *rows, int64_t nrows) {
   int ns = o->nsps;
   sdmap_t mp[2];
   for (int it = 0; it < 2; it++) map_build(&mp[it], o->mask[it], o->nsd[it]);
   int npx = ns * (ns - 1) / 2;
-  int64_t nP = o->nsd[0];
-#pragma omp parallel for schedule(dynamic, 4)
-  for (int64_t ip = 0; ip < nP; ip++) {
+  int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrows + 1));
+  off[0] = 0;
+  for (int64_t r = 0; r < nrows; r++) off[r + 1] = off[r] + o->seccount[1][o->conj[0][o->sdsec[0][rows[r]]]];
+  const int64_t total = off[nrows];
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t q = 0; q < total; q++) {
+    int64_t lo = 0, hi = nrows - 1;  /* the row r with off[r] <= q < off[r + 1] */
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) / 2;
+      if (off[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    const int64_t ip = rows[lo];
     int sp = o->sdsec[0][ip];
     int sn = o->conj[0][sp];
     uint64_t pm = o->mask[0][ip];
-    for (int64_t in = o->secfirst[1][sn]; in < o->secfirst[1][sn] + o->seccount[1][sn]; in++) {
+    {
+      const int64_t in = o->secfirst[1][sn] + (q - off[lo]);
       uint64_t nm = o->mask[1][in];
       int64_t s = o->start[0][ip] + o->start[1][in];
       double acc = 0.0;
@@ -962,8 +975,28 @@ void bso_apply(const bso_t *o, const double *x, double *y) {
       y[s] += acc;
     }
   }
+  free(off);
   for (int it = 0; it < 2; it++) { free(mp[it].keys); free(mp[it].vals); }
   (void)find_sd;
+}
+
+void bso_apply(const bso_t *o, const double *x, double *y) {
+  int64_t n = o->nsd[0];
+  int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) rows[i] = i;
+  apply_rows(o, x, y, rows, n);
+  free(rows);
+}
+
+/* y += H x on the states of the listed proton SDs only (the states
+ * pstart[ip] + nstart[in], in over the conjugate neutron sector); the other
+ * entries of y are left alone.  x must hold the whole vector.  Returns -1
+ * for a row index out of range. */
+int bso_apply_rows(const bso_t *o, const double *x, double *y, const int64_t *rows, int64_t nrows) {
+  for (int64_t r = 0; r < nrows; r++)
+    if (rows[r] < 0 || rows[r] >= o->nsd[0]) return -1;
+  if (nrows > 0) apply_rows(o, x, y, rows, nrows);
+  return 0;
 }
 
 /* ------------------------------------------------ factorized CPU port -- */

@@ -390,6 +390,30 @@ def coefficients_only():
         shutil.rmtree(wd)
 
 
+def config4_coefficients(niter=4):
+    """pf 6p6n M=0 (D = 109,954,620): the reference's option li
+    (src/blanczosmain.f90:787-795) for `niter` Lanczos steps from its uniform
+    pivot, FP32 vectors, 8 OpenMP threads (~64 s per H x here).  Keeps the
+    .lcoef coefficients and the printed operation counts as config4.npz; the
+    basis dump is left out (one line per state, 1.1e8 lines)."""
+    case = config_case(4)
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        script = "\n".join(["n", case.name, case.space, "%d %d" % (case.Z, case.N), str(case.jz2),
+                            case.name, "1 1 1 1", "end", "li", str(niter)]) + "\n"
+        run_ref(wd, script, timeout=7200)
+        a, b = read_lcoef(os.path.join(wd, case.name + ".lcoef"))
+        with open(os.path.join(wd, "stdout.txt")) as f:
+            txt = f.read()
+        out = dict(alpha=a, beta=b, niter=niter)
+        out.update(read_opcounts(txt))
+        np.savez_compressed(os.path.join(HERE, "config4.npz"), **out)
+        print("config4 alpha", a, "beta", b, out.get("nops_total"))
+    finally:
+        shutil.rmtree(wd)
+
+
 # one-body operators for menu o: (name, space, format, J, T, seed)
 OPME = [("sd_e2", "sd", "iso", 2, 0, 31), ("sd_gt", "sd", "iso", 1, 1, 32), ("sd_pns2", "sd", "pns", 2, 0, 33),
         ("pf_xpn1", "pf", "xpn", 1, 0, 34)]
@@ -627,5 +651,7 @@ if __name__ == "__main__":
             hmatrix(*c)
     if "config2" in want:
         ensemble()
+    if "config4" in want:
+        config4_coefficients()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps
         config(3, with_basis=True, niter=20)

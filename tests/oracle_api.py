@@ -37,6 +37,7 @@ def lib():
         L.bso_coupled.restype = ctypes.c_float
         L.bso_coupled.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6
         L.bso_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.bso_apply_rows.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64]
         L.bso_port_create.restype = ctypes.c_void_p
         L.bso_port_create.argtypes = [ctypes.c_void_p]
         L.bso_port_free.argtypes = [ctypes.c_void_p]
@@ -99,6 +100,17 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         lib().bso_apply(self.h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def apply_rows(self, x, rows):
+        """(H x) on the states of the listed proton SDs (FP64 brute force, the
+        same arithmetic as apply); returns the full-length y with only those
+        states set."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        if lib().bso_apply_rows(self.h, x.ctypes.data, y.ctypes.data, r.ctypes.data, len(r)) != 0:
+            raise ValueError("proton row out of range")
         return y
 
     def opme(self, path):

@@ -1,0 +1,119 @@
+"""pf 6p6n M=0 (D = 109,954,620; 440 MB per FP32 vector), the H x workload of
+bench.py, on the GPU:
+
+* every kernel variant (FP32 and FP64) and the autotuned one on a sample of
+  proton rows -- the first, middle and last row of each of the 27 proton
+  sectors plus random ones -- against the FP64 brute-force oracle on the
+  same rows (bso_apply_rows, a different algorithm from the factorized jumps);
+* x^T H y = y^T H x at full size;
+* the first Lanczos coefficients from the uniform pivot against the
+  reference's own option-li run at pf 6p6n (tests/golden/config4.npz, FP32
+  vectors there: src/blanczosmain.f90:1685 writes them).
+
+These are the kernel shapes no smaller system reaches: neutron rows up to
+3,952 columns (8 columns per thread) and up to 64 column chunks per
+(sector, class)."""
+import os
+
+import numpy as np
+import pytest
+
+from bigstickpublick_amd.inputs import config_case, write_case
+from golden_cases import GOLD, paths
+from oracle_api import Oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg4(tmp_path_factory):
+    from bigstickpublick_amd import System
+    d = str(tmp_path_factory.mktemp("cfg4"))
+    case = config_case(4)
+    write_case(case, d)
+    s = System(*paths(case, d), case.Z, case.N, case.jz2, upload=True)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    fx = dict(np.load(os.path.join(GOLD, "config4.npz")))
+    yield s, o, fx
+    s.close()
+
+
+def _sample(o):
+    """Proton rows (first, middle, last of every sector, 16 random) and the
+    basis states they hold."""
+    _, pstart, sec = o.sds(0)
+    rows = set()
+    for k in range(int(sec.max()) + 1):
+        idx = np.nonzero(sec == k)[0]
+        rows.update([int(idx[0]), int(idx[len(idx) // 2]), int(idx[-1])])
+    rows.update(np.random.default_rng(7).integers(0, len(pstart), 16).tolist())
+    rows = np.array(sorted(rows))
+    bounds = np.append(np.sort(pstart), o.dim)
+    states = [np.arange(pstart[r], bounds[np.searchsorted(bounds, pstart[r], side="right")]) for r in rows]
+    return rows, np.concatenate(states)
+
+
+def test_config4_rows_every_variant(cfg4):
+    s, o, fx = cfg4
+    assert s.dim == o.dim == 109954620
+    rows, states = _sample(o)
+    assert len(states) > 100000
+    x = np.random.default_rng(11).standard_normal(s.dim)
+    want = o.apply_rows(x, rows)[states]
+    scale = np.abs(want).max()
+    xd = torch.tensor(x, device="cuda")
+    xf = xd.float()
+    st = torch.tensor(states, device="cuda")
+    del x
+    n64 = n32 = 0
+    worst = {}
+    for v in range(len(s.variants())):
+        if s.set_variant("f64", v):
+            got = s.apply(xd)[st].cpu().numpy()
+            err = np.abs(got - want).max() / scale
+            assert err < 1e-12, ("f64", v, err)
+            n64 += 1
+        if s.set_variant("f32", v):
+            got = s.apply(xf)[st].cpu().numpy()
+            err = np.abs(got - want).max() / scale
+            worst[v] = err
+            assert err < 2e-5, ("f32", v, err)
+            n32 += 1
+    assert n32 >= 14 and n64 >= 7
+    v = s.autotune("f32")
+    got = s.apply(xf)[st].cpu().numpy()
+    assert np.abs(got - want).max() / scale < 2e-5, v
+    print("config 4 rows: %d proton rows, %d states; %d FP32 and %d FP64 variants; worst FP32 %.2e"
+          % (len(rows), len(states), n32, n64, max(worst.values())))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_config4_hermitian(cfg4, dtype):
+    """y . (H x) = x . (H y) over all 1.1e8 states (FP64 sums)."""
+    s, o, fx = cfg4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randn(s.dim, device="cuda", dtype=torch.float64, generator=g).to(dtype)
+    y = torch.randn(s.dim, device="cuda", dtype=torch.float64, generator=g).to(dtype)
+    hx, hy = s.apply(x), s.apply(y)
+    lhs = torch.dot(y.double(), hx.double()).item()
+    rhs = torch.dot(x.double(), hy.double()).item()
+    norm = (torch.linalg.vector_norm(y.double()) * torch.linalg.vector_norm(hx.double())).item()
+    tol = 1e-12 if dtype == torch.float64 else 1e-6
+    assert abs(lhs - rhs) / norm < tol, (lhs, rhs)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_config4_lanczos_matches_reference(cfg4, dtype):
+    """alpha_1..4, beta_1..4 from the uniform pivot equal the reference's own
+    pf 6p6n run (its FP32 vectors bound the agreement)."""
+    s, o, fx = cfg4
+    n = len(fx["alpha"])
+    r = s.lanczos(nkeep=1, niter=n, dtype=dtype, polish=False)
+    da = np.abs(r.alpha[:n] - fx["alpha"])
+    db = np.abs(r.beta[:n] - fx["beta"])
+    print("config 4 %s: |d alpha| %s |d beta| %s" % (dtype, da, db))
+    assert da.max() < 1e-4 and db.max() < 1e-4
+    # and the reference's printed operation count
+    assert s.op_counts()["ref_total"] == int(fx["nops_total"])

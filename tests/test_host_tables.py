@@ -160,3 +160,20 @@ def test_entry_points_refuse_bad_arguments(casedir, tmp_path):
         assert L.bs_block_strength(s.h, ctypes.byref(prm), blk, pv, pdt, None, None, ev.ctypes.data, None, None,
                                    ctypes.byref(it)) != 0
         assert msg in L.bs_last_error(), L.bs_last_error()
+
+
+def test_config4_operation_count(tmp_path):
+    """pf 6p6n M=0: the host tables (no GPU) reproduce the reference's own
+    printed operation counts per type and in total (tests/golden/config4.npz,
+    written from its option-li run; src/bigstick_main.f90:1150-1192)."""
+    from bigstickpublick_amd.inputs import config_case, write_case
+    case = config_case(4)
+    write_case(case, str(tmp_path))
+    fx = dict(np.load(os.path.join(ROOT, "tests", "golden", "config4.npz")))
+    s = System(*paths(case, str(tmp_path)), case.Z, case.N, case.jz2)
+    assert s.dim == 109954620
+    c = s.op_counts()
+    assert c["ref_total"] == int(fx["nops_total"]) == 152771809712
+    for k in ("pp", "nn", "pn"):
+        assert c["ref_" + k] == int(fx["nops_" + k]), k
+    s.close()

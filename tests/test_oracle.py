@@ -252,3 +252,26 @@ def test_autoscaled_int_matches_reference_run(tmp_path):
     conv = np.abs(res[:, 1] - np.round(res[:, 1])) < 1e-3
     assert conv.sum() >= 3
     assert np.abs(e - res[:, 0])[conv].max() < 2e-5, (e, res)
+
+
+def test_oracle_row_apply_equals_full_apply(casedir):
+    """bso_apply_rows (the row-sampled oracle used at sizes where the full
+    apply is too slow) gives, on every listed proton row, the very bits of
+    the full apply, and leaves the other rows zero."""
+    case, d, fx = small_fixture("pf2p2n", casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    x = np.random.default_rng(4).standard_normal(o.dim)
+    full = o.apply(x)
+    _, pstart, _ = o.sds(0)
+    nP = len(pstart)
+    rows = np.array([0, 3, nP // 2, nP - 1])
+    part = o.apply_rows(x, rows)
+    bounds = np.append(np.sort(pstart), o.dim)
+    mask = np.zeros(o.dim, bool)
+    for r in rows:
+        lo = pstart[r]
+        hi = bounds[np.searchsorted(bounds, lo, side="right")]
+        mask[lo:hi] = True
+    assert mask.sum() > 0
+    assert np.array_equal(part[mask], full[mask])
+    assert not np.any(part[~mask])
