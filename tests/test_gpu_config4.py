@@ -114,6 +114,8 @@ def test_config4_lanczos_matches_reference(cfg4, dtype):
     da = np.abs(r.alpha[:n] - fx["alpha"])
     db = np.abs(r.beta[:n] - fx["beta"])
     print("config 4 %s: |d alpha| %s |d beta| %s" % (dtype, da, db))
-    assert da.max() < 1e-4 and db.max() < 1e-4
+    # measured: |d alpha| <= 7.1e-6, |d beta| <= 4.7e-7 (1e-7 relative: the
+    # reference's FP32 vectors)
+    assert da.max() < 2e-5 and db.max() < 2e-6
     # and the reference's printed operation count
     assert s.op_counts()["ref_total"] == int(fx["nops_total"])
