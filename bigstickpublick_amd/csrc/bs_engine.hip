@@ -34,6 +34,15 @@
 #include "../../include/bigstick_amd.h"
 #include "bs_tables.hpp"
 #include <rocsolver/rocsolver.h>
+#include <mutex>
+
+// Device scratch of bs_lanczos_noalloc: one array per GPU, held from the
+// library's load until the process ends.  In host code its name is only a
+// stand-in for the symbol; the device address comes from
+// hipGetSymbolAddress (scratch_ptr below) and is the only thing passed to
+// kernels or copies.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -1497,19 +1506,28 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  bool owned = true;
   void alloc(size_t count) {
     release();
     n = count;
     if (count) HIP_OK(hipMalloc(&p, count * sizeof(T)));
+  }
+  // use memory owned elsewhere (not freed by release)
+  void wrap(void* ptr, size_t count) {
+    release();
+    p = static_cast<T*>(ptr);
+    n = count;
+    owned = false;
   }
   void upload(const std::vector<T>& v) {
     alloc(v.size());
     if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
   }
   ~DevBuf() { release(); }
 };
@@ -2089,7 +2107,8 @@ struct PivotRun {
 
 template <typename T>
 int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, double* beta_out,
-                double* evals, void* evecs, double* resid, int* iters, const PivotRun<T>& pr = PivotRun<T>()) {
+                double* evals, void* evecs, double* resid, int* iters, const PivotRun<T>& pr = PivotRun<T>(),
+                void* ext = nullptr, size_t ext_bytes = 0) {
   const int64_t D = s->t.dim;
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
@@ -2106,11 +2125,33 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   const int64_t ld = (D + 63) / 64 * 64;
   DevBuf<T> V, w;
   DevBuf<double> part, dots, ab;
-  V.alloc((size_t)mtot * ld);
-  w.alloc(D);
-  part.alloc((size_t)nb * mtot);
-  dots.alloc(mtot);
-  ab.alloc(2 * (size_t)mtot + 2);  // alpha[0..], beta[mtot..]
+  if (ext) {
+    // caller's scratch (bs_lanczos_noalloc): the same five buffers carved
+    // from it at 256-byte boundaries; the paths below that allocate (thick
+    // restart, Ritz vectors, polish) are refused by the caller
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bV = up(sizeof(T) * (size_t)mtot * ld), bw = up(sizeof(T) * (size_t)ld);
+    const size_t bp = up(sizeof(double) * (size_t)nb * mtot), bd = up(sizeof(double) * (size_t)mtot);
+    const size_t ba = up(sizeof(double) * (2 * (size_t)mtot + 2));
+    const size_t need = bV + bw + bp + bd + ba;
+    if (need > ext_bytes)
+      return set_err("this run needs " + std::to_string(need) + " bytes of scratch; the no-allocation entry point has " +
+                     std::to_string(ext_bytes) + " (dim " + std::to_string(D) + ", " + std::to_string(niter) +
+                     " iterations, " + std::to_string(sizeof(T)) + "-byte vectors)");
+    if (thick || evecs || resid || p->polish) return set_err("the no-allocation run is a plain run without vectors");
+    unsigned char* b = static_cast<unsigned char*>(ext);
+    V.wrap(b, (size_t)mtot * ld);
+    w.wrap(b + bV, ld);
+    part.wrap(b + bV + bw, (size_t)nb * mtot);
+    dots.wrap(b + bV + bw + bp, mtot);
+    ab.wrap(b + bV + bw + bp + bd, 2 * (size_t)mtot + 2);
+  } else {
+    V.alloc((size_t)mtot * ld);
+    w.alloc(D);
+    part.alloc((size_t)nb * mtot);
+    dots.alloc(mtot);
+    ab.alloc(2 * (size_t)mtot + 2);  // alpha[0..], beta[mtot..]
+  }
   double* d_alpha = ab.p;
   double* d_beta = ab.p + mtot;
   const int g = grid_for(D);
@@ -2363,6 +2404,27 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
 bs_system* checked(bs_system* s) {
   if (!s) throw std::runtime_error("null system");
   return s;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+void* scratch_ptr() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
 }
 
 }  // namespace
@@ -4182,6 +4244,27 @@ int bs_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha, double* 
     if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
     if (p->dtype == BS_F32) return run_lanczos<float>(s, p, alpha, beta, evals, evecs, resid, iters);
     if (p->dtype == BS_F64) return run_lanczos<double>(s, p, alpha, beta, evals, evecs, resid, iters);
+    return set_err("dtype must be BS_F32 or BS_F64");
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
+int bs_lanczos_noalloc(bs_system* s, const bs_lanczos_params* p, double* alpha, double* beta, double* evals,
+                       int* iters) {
+  try {
+    checked(s);
+    if (!p || !evals) return set_err("null argument");
+    if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
+    if (p->thick != 0) return set_err("bs_lanczos_noalloc runs plain Lanczos only (thick = 0)");
+    if (p->polish != 0) return set_err("bs_lanczos_noalloc has no polish (polish = 0)");
+    void* ptr = scratch_ptr();
+    if (p->dtype == BS_F32)
+      return run_lanczos<float>(s, p, alpha, beta, evals, nullptr, nullptr, iters, PivotRun<float>(), ptr,
+                                kScratchBytes);
+    if (p->dtype == BS_F64)
+      return run_lanczos<double>(s, p, alpha, beta, evals, nullptr, nullptr, iters, PivotRun<double>(), ptr,
+                                 kScratchBytes);
     return set_err("dtype must be BS_F32 or BS_F64");
   } catch (const std::exception& e) {
     return set_err(e.what());

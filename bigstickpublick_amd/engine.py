@@ -58,6 +58,7 @@ def load_library():
     L.bs_observable.argtypes = [vp, i32, vp, i32, ctypes.POINTER(ctypes.c_double)]
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
+    L.bs_lanczos_noalloc.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_lanczos_stats.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_block_strength.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, i32, vp, vp, vp, vp, vp,

@@ -163,6 +163,19 @@ typedef struct {
 int bs_lanczos(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
                double *evals, void *evecs, double *resid, int *iters);
 
+/* bs_lanczos with no device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Plain Lanczos only
+ * (p->thick = 0, p->polish = 0; no vectors or residuals); alpha, beta, evals
+ * and iters as in bs_lanczos.  The Lanczos vectors and partial sums live in a
+ * fixed device array of 128 MiB per GPU (a static device symbol, held from
+ * the library's load until the process ends): the run needs about
+ * (niter + 2) * dim * (4 or 8) bytes, and a larger request is refused with an
+ * error that states both sizes.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU race; each call initializes what it reads. */
+int bs_lanczos_noalloc(bs_system *sys, const bs_lanczos_params *p, double *alpha, double *beta,
+                       double *evals, int *iters);
+
 /* Breakdown handling of the last bs_lanczos / bs_strength run on sys, as the
  * reference does it inside lanczos_p (src/blanczosmain.f90:1634-1658):
  * random_restarts counts steps whose beta fell below 1e-4 of the previous
