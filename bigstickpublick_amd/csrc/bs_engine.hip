@@ -1652,6 +1652,7 @@ struct bs_system {
   bool uploaded = false;
   int last_random_restarts = 0;  // breakdown handling of the last Lanczos run
   int last_second_passes = 0;
+  int last_steps = 0;            // Lanczos steps of its last Krylov sequence
   int cpt = 1;
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
   float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
@@ -2182,6 +2183,25 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   // Krylov space is the whole space (i + 1 == D) v_{i+1} is set to zero.
   double beta_prev = 0.0;
   int random_restarts = 0, second_passes = 0;
+  // convergence test parameters (src/blanczosmain.f90:1469-1512)
+  if (p->conv_test != 0 && p->conv_test != 1) return set_err("conv_test must be 0 (mean energy change) or 1 (largest)");
+  if (p->ncheck_ex < 0) return set_err("ncheck_ex must be >= 0");
+  const int itskip = std::max(10, (int)std::sqrt((double)nkeep));
+  const int ncheck = (int)std::min<int64_t>(D, (int64_t)nkeep + p->ncheck_ex);
+  const int ntest = nkeep > 100 && thick == 0 ? (int)(std::sqrt((double)ncheck) + 1) : ncheck;
+  // the test of convergence_test (src/blanczoslib.f90:2726-2770) between
+  // two sets of Ritz values: test 0 sum_{j<nt} |dE_j| / sqrt(nt + 1) times
+  // (1 + restarts), test 1 max_j |dE_j|
+  auto energy_change = [&](const std::vector<double>& e0, const std::vector<double>& e1, int nt, int rst) {
+    nt = std::min<int>(nt, (int)std::min(e0.size(), e1.size()));
+    double sum = 0.0, mx = 0.0;
+    for (int j = 0; j < nt; j++) {
+      const double d = std::fabs(e0[j] - e1[j]);
+      sum += d;
+      mx = std::max(mx, d);
+    }
+    return p->conv_test == 1 ? mx : sum / std::sqrt((double)nt + 1.0) * (1.0 + rst);
+  };
   auto orthonormalize = [&](T* v, int m) {
     k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, m, v, D, part.p);
     k_reduce<<<m, kBlock, 0, st>>>(part.p, nb, m, dots.p, 0);
@@ -2262,25 +2282,27 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
       next_vector(i, kstart, arrow);
       HIP_OK(hipGetLastError());
-      // plain runs with a tolerance (options ld, lc): every 10 steps the
-      // lowest nkeep Ritz values of the tridiagonal so far; stop when none
-      // moved by more than tol since the previous check (the reference tests
-      // its criterion every itskipeig = 10 steps, src/blanczoslib.f90:2660-2800)
-      if (thick == 0 && p->tol > 0.0 && (i + 1) % 10 == 0 && i + 1 >= nkeep + 2 && i + 1 < niter) {
-        std::vector<double> a(i + 1), b(i + 1);
-        HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * (i + 1), hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * (i + 1), hipMemcpyDeviceToHost));
-        std::vector<double> ek = tridiag_lowest(a, b, nkeep);
-        bool stop = false;
-        if (!e_check.empty()) {
-          double dmax = 0.0;
-          for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(ek[k] - e_check[k]));
-          stop = dmax < p->tol;
-        }
-        e_check = ek;
-        if (stop) {
-          niter = i + 1;
-          break;
+      // plain runs with a tolerance (options ld, lc): the reference's
+      // schedule and test (lanczos_p, src/blanczosmain.f90:1489-1512,
+      // :1690-1724; convergence_test, src/blanczoslib.f90:2660-2800): at
+      // every itskipeig-th step (10, or sqrt(nkeep) if larger) beyond ncheck =
+      // nkeep + ncheck_ex, the lowest ncheck Ritz values of the tridiagonal so
+      // far; stop when, against the previous such check, the mean change of
+      // the first ntest of them (sum |dE| / sqrt(ntest + 1), test 0) or their
+      // largest change (test 1) is below tol
+      if (thick == 0 && p->tol > 0.0) {
+        const int iter = i + 1;
+        if (iter % itskip == 0 && iter > ncheck && iter < niter) {
+          std::vector<double> a(iter), b(iter);
+          HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * iter, hipMemcpyDeviceToHost));
+          HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * iter, hipMemcpyDeviceToHost));
+          std::vector<double> ek = tridiag_lowest(a, b, ncheck);
+          const bool stop = !e_check.empty() && energy_change(e_check, ek, ntest, 0) < p->tol;
+          e_check = ek;
+          if (stop) {
+            niter = iter;
+            break;
+          }
         }
       }
     }
@@ -2305,12 +2327,13 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     }
     jacobi_eigen(m, A, e, Y);
     bool done = (thick == 0) || restarts >= p->max_restarts;
-    if (!done && p->tol > 0.0 && !e_prev.empty()) {
-      double dmax = 0.0;
-      for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(e[k] - e_prev[k]));
-      if (dmax < p->tol) done = true;
-    }
-    e_prev.assign(e.begin(), e.begin() + nkeep);
+    // thick restart: the same measure over ncheck values, checked at each
+    // restart; the mean (test 0) carries the reference's factor (1 + number
+    // of restarts) (src/blanczoslib.f90:2735)
+    const int nc_thick = std::min(ncheck, m);
+    if (!done && p->tol > 0.0 && !e_prev.empty() && energy_change(e_prev, e, std::min(ntest, nc_thick), restarts) < p->tol)
+      done = true;
+    e_prev.assign(e.begin(), e.begin() + nc_thick);
     if (done) break;
     // thick restart (src/blanczoslib.f90:1656-1799): keep `thick` Ritz
     // vectors, V[0:thick] = V[0:m] Y[:, 0:thick]; couplings beta_m * Y[m-1, k]
@@ -2398,6 +2421,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (iters) *iters = total_mv;
   s->last_random_restarts = random_restarts;
   s->last_second_passes = second_passes;
+  s->last_steps = niter;
   return 0;
 }
 
@@ -3542,6 +3566,50 @@ int bs_op_counts(const bs_system* s, int64_t out[8]) {
   return 0;
 }
 
+int bs_jumps(const bs_system* s, int it, int order, int64_t* count, int32_t* isd, int32_t* fsd, int32_t* op,
+             int8_t* phase) {
+  try {
+    if (!s || !count) return set_err("null argument");
+    if (it != 0 && it != 1) return set_err("species must be 0 (protons) or 1 (neutrons)");
+    if (order != 1 && order != 2) return set_err("order must be 1 or 2");
+    const bs::Tables& t = s->t;
+    const bool want = isd || fsd || op || phase;
+    if (order == 1) {
+      const bs::OneBody& O = t.ob[it];
+      const int64_t nsd = O.rowptr.empty() ? 0 : (int64_t)O.rowptr.size() - 1;
+      const int64_t n = nsd ? O.rowptr[nsd] : 0;
+      if (want && *count < n) return set_err("count is smaller than the number of jumps");
+      *count = n;
+      if (!want) return 0;
+      for (int64_t f = 0; f < nsd; f++)
+        for (int64_t e = O.rowptr[f]; e < O.rowptr[f + 1]; e++) {
+          if (isd) isd[e] = O.src[e];
+          if (fsd) fsd[e] = (int32_t)f;
+          if (op) op[e] = O.op[e];
+          if (phase) phase[e] = O.phase[e];
+        }
+      return 0;
+    }
+    bs::TwoBodyOps T;
+    bs::build_twobody_ops(t, it, T);
+    const int64_t nsd = (int64_t)T.rowptr.size() - 1;
+    const int64_t n = nsd > 0 ? T.rowptr[nsd] : 0;
+    if (want && *count < n) return set_err("count is smaller than the number of jumps");
+    *count = n;
+    if (!want) return 0;
+    for (int64_t f = 0; f < nsd; f++)
+      for (int64_t e = T.rowptr[f]; e < T.rowptr[f + 1]; e++) {
+        if (isd) isd[e] = T.src[e];
+        if (fsd) fsd[e] = (int32_t)f;
+        if (op) op[e] = T.me[e];
+        if (phase) phase[e] = T.phase[e];
+      }
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
 int bs_apply_h(bs_system* s, const void* x, void* y, int dtype, int accumulate, void* stream) {
   try {
     checked(s);
@@ -4271,10 +4339,11 @@ int bs_lanczos_noalloc(bs_system* s, const bs_lanczos_params* p, double* alpha, 
   }
 }
 
-int bs_lanczos_stats(const bs_system* s, int* random_restarts, int* second_passes) {
+int bs_lanczos_stats(const bs_system* s, int* random_restarts, int* second_passes, int* steps) {
   if (!s) return set_err("null system");
   if (random_restarts) *random_restarts = s->last_random_restarts;
   if (second_passes) *second_passes = s->last_second_passes;
+  if (steps) *steps = s->last_steps;
   return 0;
 }
 

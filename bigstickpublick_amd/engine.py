@@ -18,7 +18,7 @@ _lib = None
 class _Params(ctypes.Structure):
     _fields_ = [("nkeep", ctypes.c_int), ("niter", ctypes.c_int), ("thick", ctypes.c_int),
                 ("max_restarts", ctypes.c_int), ("dtype", ctypes.c_int), ("tol", ctypes.c_double),
-                ("polish", ctypes.c_int)]
+                ("polish", ctypes.c_int), ("conv_test", ctypes.c_int), ("ncheck_ex", ctypes.c_int)]
 
 
 def load_library():
@@ -45,6 +45,7 @@ def load_library():
     L.bs_me_table.argtypes = [vp, i32, ctypes.POINTER(i32), vp]
     L.bs_op_counts.argtypes = [vp, vp]
     L.bs_apply_h.argtypes = [vp, vp, vp, i32, i32, vp]
+    L.bs_jumps.argtypes = [vp, i32, i32, i64p, vp, vp, vp, vp]
     L.bs_num_variants.argtypes = [ctypes.POINTER(i32)]
     L.bs_variant_info.argtypes = [i32] + [ctypes.POINTER(i32)] * 4
     L.bs_set_variant.argtypes = [vp, i32, i32]
@@ -59,7 +60,7 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_lanczos_noalloc.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, ctypes.POINTER(i32)]
-    L.bs_lanczos_stats.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.bs_lanczos_stats.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_block_strength.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, i32, vp, vp, vp, vp, vp,
                                     ctypes.POINTER(i32)]
@@ -229,6 +230,18 @@ class System:
                     ref_spe=int(out[4]), ref_pp=int(out[5]), ref_nn=int(out[6]),
                     ref_pn=int(out[7]), ref_total=int(out[4:].sum()))
 
+    def jumps(self, it, order):
+        """(isd, fsd, op, phase) arrays of the engine's one- (order 1) or
+        two-body (order 2) jumps of species it, every direction."""
+        L = load_library()
+        n = ctypes.c_int64(0)
+        _check(L.bs_jumps(self.h, it, order, ctypes.byref(n), None, None, None, None))
+        isd, fsd, op = (np.zeros(n.value, np.int32) for _ in range(3))
+        ph = np.zeros(n.value, np.int8)
+        _check(L.bs_jumps(self.h, it, order, ctypes.byref(n), isd.ctypes.data, fsd.ctypes.data, op.ctypes.data,
+                          ph.ctypes.data))
+        return isd, fsd, op, ph
+
     # ---- device side: torch tensors are used only as device buffers ----
     def apply(self, x, y=None, accumulate=False, stream=None):
         import torch
@@ -336,11 +349,15 @@ class System:
         return out
 
     def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
-                want_vectors=False):
+                want_vectors=False, conv_test=1, ncheck_ex=0):
+        """Lanczos from the uniform pivot.  With tol > 0 the run stops by the
+        reference's energy test conv_test (0: mean change, 1: largest change)
+        over nkeep + ncheck_ex Ritz values (the reference's ld is conv_test=0,
+        ncheck_ex=5, tol=1e-3)."""
         import torch
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
-        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish))
+        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish), conv_test, ncheck_ex)
         m = min(niter, self.dim)
         alpha = np.zeros(m)
         beta = np.zeros(m)
@@ -359,11 +376,11 @@ class System:
         return LanczosResult(ev, alpha, beta, res, it.value, vec)
 
     def lanczos_stats(self):
-        """(random restarts, second Gram-Schmidt passes) of the last lanczos()
-        or strength() run."""
-        a, b = ctypes.c_int(), ctypes.c_int()
-        _check(load_library().bs_lanczos_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        """(random restarts, second Gram-Schmidt passes, Lanczos steps of the
+        last Krylov sequence) of the last lanczos() or strength() run."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load_library().bs_lanczos_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def exact(self, nkeep, dtype="f64", want_vectors=False):
         """Exact diagonalization of the dense H (dim <= 65536): the nkeep

@@ -11,6 +11,7 @@ module bigstick_amd
     integer(c_int) :: nkeep, niter, thick, max_restarts, dtype
     real(c_double) :: tol
     integer(c_int) :: polish
+    integer(c_int) :: conv_test = 0, ncheck_ex = 0
   end type bs_lanczos_params
 
   interface
@@ -102,6 +103,14 @@ module bigstick_amd
       integer(c_int), intent(out) :: iters
       integer(c_int) :: rc
     end function bs_lanczos
+
+    function bs_lanczos_stats(sys, random_restarts, second_passes, steps) bind(C, name='bs_lanczos_stats') &
+        result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: sys
+      integer(c_int), intent(out) :: random_restarts, second_passes, steps
+      integer(c_int) :: rc
+    end function bs_lanczos_stats
 
     function bs_parity(sys, parity) bind(C, name='bs_parity') result(rc)
       import :: c_ptr, c_int

@@ -60,6 +60,7 @@ contains
     logical, intent(in) :: write_wfn
     character(len=256) :: outname, spsname, intname, line, lopt
     integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter, ncheck, ctest
+    integer(c_int) :: nrr, nsp, steps
     integer(8) :: t0, t1, t2, trate
     real(c_double) :: ctol
     real(c_float) :: spscale, a0, a, x
@@ -155,12 +156,19 @@ contains
       read(5, *, iostat=ios) maxrest
       if (ios /= 0) maxrest = 20
     end if
-    ctol = 1.0d-6
+    ! default convergence (ld, td): the reference's test 0 with tolerance
+    ! converge_tol0_def = 1e-3 over nkeep + ncheck_ex energies, ncheck_ex =
+    ! 5 for ld (src/blanczosmain.f90:800-812, :1000-1020, bmodules_flags.f90:38)
+    ctol = 1.0d-3
+    ctest = 0
+    ncheck = 0
+    if (lopt(1:2) == 'ld') ncheck = 5
+    if (lopt(1:1) == 'b') ctol = 1.0d-6
     if (lopt(2:2) == 'c' .or. lopt(2:2) == 'C') then
       ! lc / bc / tc: [additional states for the test,] test choice, tolerance
-      ! (src/blanczosmain.f90:943-985, :873-921, :1117-1175).  The test here is
-      ! the largest change of a kept energy; choice 0 (the reference's mean
-      ! change) is served by it, the wave-function tests 2 and 3 are not.
+      ! (src/blanczosmain.f90:943-985, :873-921, :1117-1175): the energy tests
+      ! 0 (mean change) and 1 (largest change); the wave-function tests 2 and
+      ! 3 are not supported
       if (lopt(1:1) /= 't') read(5, *) ncheck
       read(5, *) ctest
       if (ctest /= 0 .and. ctest /= 1) then
@@ -194,6 +202,9 @@ contains
     p%dtype = dtype
     p%tol = 0.0d0
     if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'C') p%tol = ctol
+    p%conv_test = ctest
+    p%ncheck_ex = ncheck
+    if (lopt(1:1) == 'b') p%ncheck_ex = 0
     p%polish = 1
     nbytes = int(nkeep, c_int64_t) * dim * esize
     dvec = device_alloc(nbytes)
@@ -226,8 +237,12 @@ contains
       call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
     close(iu)
     if (dimblock == 0 .and. lopt(1:2) /= 'ex') then
+      ! a plain run that converged stopped after `steps` iterations, where
+      ! the reference's .lcoef ends too
+      rc = bs_lanczos_stats(sys, nrr, nsp, steps)
+      call bs_check(rc, 'bs_lanczos_stats')
       open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
-      do k = 1, niter
+      do k = 1, min(niter, int(steps))
         write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
       end do
       close(iu)

@@ -83,6 +83,21 @@ int bs_me_table(const bs_system *sys, int kind, int *npair, float *table);
  * applications, diagonal pair terms twice). */
 int bs_op_counts(const bs_system *sys, int64_t out[8]);
 
+/* One- (order 1) or two-body (order 2) jumps of species it (0 protons, 1
+ * neutrons) as this engine builds them, one entry per operator application
+ * on a Slater determinant: isd -> fsd (0-based SD indices of the species, in
+ * the reference's SD order), op (order 1: a*nsps + c for a+_a a_c; order 2:
+ * the dense pair-pair index cpair*npair + dpair of bs_me_table for
+ * a+_i a+_j a_l a_k) and the fermion phase.  Both directions are listed; the
+ * reference stores the sector jumps with isector >= fsector and applies the
+ * rest as their transposes.  Pass NULL arrays to get the count in *count;
+ * otherwise *count is the arrays' capacity on entry and the count on exit.
+ * Replaces the jump lists of jumpmaster / weld (src/bjumplib_master.f90:
+ * 342-431, src/bjumplib_weld.f90:32-818) as dumped by write_out_jumps
+ * (src/boutputlib.f90:1258-1351). */
+int bs_jumps(const bs_system *sys, int it, int order, int64_t *count, int32_t *isd, int32_t *fsd, int32_t *op,
+             int8_t *phase);
+
 /* ---- device ---- */
 /* y = H x (accumulate = 0) or y += H x (accumulate = 1).  x, y: device
  * arrays of dim elements, dtype BS_F32 (FP32 vectors, FP32 accumulation) or
@@ -147,9 +162,15 @@ typedef struct {
   int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart */
   int max_restarts; /* thick restart: maximum number of restarts */
   int dtype;        /* BS_F32: FP32 vectors/matvec, FP64 coefficients; BS_F64: all FP64 */
-  double tol;       /* stop when max |E_i - E_i(previous)| < tol, checked at each thick restart, or
-                       every 10 steps of a plain run (0: run all) */
+  double tol;       /* convergence tolerance (0: run all iterations); the test is conv_test's */
   int polish;       /* 1: FP64 Rayleigh quotient / residual polish of the Ritz vectors */
+  int conv_test;    /* the reference's energy tests (convergence_test, src/blanczoslib.f90:2660-2800)
+                       over ncheck = nkeep + ncheck_ex Ritz values: 0 = their mean change
+                       sum |dE| / sqrt(ncheck + 1) (times 1 + restarts in a thick run), 1 = their
+                       largest change.  A plain run checks every 10th step (sqrt(nkeep) if larger)
+                       beyond ncheck, as lanczos_p does (src/blanczosmain.f90:1690-1724); a thick
+                       run checks at each restart. */
+  int ncheck_ex;    /* additional Ritz values in the test (the reference's ld uses 5) */
 } bs_lanczos_params;
 
 /* Lanczos from the reference's uniform pivot 1/sqrt(D)
@@ -185,8 +206,10 @@ int bs_lanczos_noalloc(bs_system *sys, const bs_lanczos_params *p, double *alpha
  * kept; second_passes counts steps whose beta fell below 0.1 of the previous
  * one and whose new vector then overlapped a stored one by more than 1e-5
  * (br_check_orthogonal, src/breorthog.f90:977-1045) and was
- * re-orthogonalized.  Either pointer may be NULL. */
-int bs_lanczos_stats(const bs_system *sys, int *random_restarts, int *second_passes);
+ * re-orthogonalized; steps receives the number of Lanczos steps of the last
+ * Krylov sequence (where a converged plain run stopped: the length of the
+ * reference's .lcoef).  Any pointer may be NULL. */
+int bs_lanczos_stats(const bs_system *sys, int *random_restarts, int *second_passes, int *steps);
 
 /* Exact diagonalization: the dense H (dim <= 65536, dim^2 doubles in HBM)
  * built 64 columns at a time with the multi-vector kernel in FP64, all

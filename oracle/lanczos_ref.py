@@ -34,3 +34,33 @@ def lanczos(apply, dim, niter, nkeep):
     e, y = np.linalg.eigh(T)
     vecs = y[:, :nkeep].T @ V[:niter]
     return alpha, beta, e[:nkeep], vecs
+
+
+def convergence_stop(alpha, beta, nkeep, ncheck_ex=5, tol=1e-3, test=0, niter=None):
+    """Iteration at which a plain Lanczos run with the reference's default
+    convergence stops (the rule of lanczos_p, src/blanczosmain.f90:1469-1512,
+    :1690-1724, and convergence_test, src/blanczoslib.f90:2660-2800): every
+    itskipeig-th iteration (10, or int(sqrt(nkeep)) if larger) beyond ncheck =
+    nkeep + ncheck_ex, the lowest ncheck eigenvalues of the tridiagonal so far
+    are compared with those of the previous such check; test 0 stops when
+    sum |dE| / sqrt(ncheck + 1) < tol, test 1 when max |dE| < tol.  Returns the
+    stopping iteration (1-based): niter when the test never passes before it,
+    None when the coefficients end before either."""
+    niter = len(alpha) if niter is None else niter
+    itskip = max(10, int(np.sqrt(nkeep)))
+    ncheck = nkeep + ncheck_ex
+    ntest = ncheck if nkeep <= 100 else int(np.sqrt(ncheck) + 1)
+    eold = None
+    for it in range(1, len(alpha) + 1):
+        if it >= niter:
+            return niter
+        if it % itskip or it <= ncheck:
+            continue
+        T = np.diag(alpha[:it]) + np.diag(beta[:it - 1], 1) + np.diag(beta[:it - 1], -1)
+        e = np.linalg.eigvalsh(T)[:ncheck]
+        if eold is not None:
+            d = np.abs(eold[:ntest] - e[:ntest])
+            if (d.max() if test == 1 else d.sum() / np.sqrt(ntest + 1)) < tol:
+                return it
+        eold = e
+    return None

@@ -390,6 +390,74 @@ def coefficients_only():
         shutil.rmtree(wd)
 
 
+def read_jumps_dump(path):
+    """The reference's jump dump (write_out_jumps, src/boutputlib.f90:1258-1351:
+    a block per species and order, then per sector jump the lines
+    '(i6,a,3i9)' jump, ':', initial SD, final SD, op; the op column is the
+    neutron two-body operator array whatever the block and can overflow, so
+    only the two SD columns are read, 1-based)."""
+    blocks, cur = {}, None
+    for line in open(path):
+        t = line.strip()
+        if t.endswith("jumps") and "body" in t:
+            cur = "%s%d" % ("p" if t.startswith("proton") else "n", 1 if "1-body" in t else 2)
+            blocks[cur] = []
+            continue
+        if ":" in line and "jumps from sector" not in line and cur is not None:
+            a, b = line.rstrip("\n").split(":", 1)
+            if a.strip().isdigit() and len(b) >= 18:
+                blocks[cur].append((int(b[0:9]), int(b[9:18])))
+    return {k: np.array(v, dtype=np.int32).reshape(-1, 2) for k, v in blocks.items()}
+
+
+def jumps(name, space, Z, N, jz2, seed, case=None):
+    """The reference's own jump lists (its print_jumps dump, switched on by
+    oracle/ref_hooks.c flag j) for one system: jumps_<name>.npz with arrays
+    p1, n1, p2, n2 of (initial SD, final SD), 1-based, as stored."""
+    if case is None:
+        case = small_case(space, Z, N, jz2, seed=seed)
+        case.name = name
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, menu_script(case, "n", "lf"), env_extra={"BS_REF_PRINT": "j", "OMP_NUM_THREADS": "1"})
+        out = read_jumps_dump(os.path.join(wd, case.name + ".jumps"))
+        np.savez_compressed(os.path.join(HERE, "jumps_%s.npz" % name), **out)
+        print("jumps", name, {k: len(v) for k, v in out.items()})
+    finally:
+        shutil.rmtree(wd)
+
+
+def default_convergence():
+    """The reference's option ld (default convergence: test 0, the mean
+    change of nkeep + 5 energies over 10 iterations below 1e-3,
+    src/blanczosmain.f90:800-812 and src/blanczoslib.f90:2660-2800) with
+    nkeep 5 on sd 2p2n, sd 1p2n (2Jz = 1), pf 2p1n (2Jz = 3) and config 0:
+    .res and .lcoef kept as <name>_ld.res / .lcoef; the .lcoef length is where
+    the reference's test stopped."""
+    runs = []
+    for name, answer in (("sd2p2n", "5 300"), ("sd1p2n_m1", "5 120"), ("pf2p1n_m3", "5 300")):
+        meta = [c for c in SMALL if c[0] == name][0]
+        c = small_case(*meta[1:5], seed=meta[5])
+        c.name = name
+        runs.append((c, answer))
+    runs.append((config_case(0), "5 150"))
+    for case, answer in runs:
+        wd = tempfile.mkdtemp(prefix="bsgold_")
+        try:
+            write_case(case, wd)
+            out = case.name + "_ld"
+            script = "\n".join(["n", out, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), case.name,
+                                "1 1 1 1", "end", "ld", answer]) + "\n"
+            run_ref(wd, script)
+            for ext in (".res", ".lcoef"):
+                shutil.copy(os.path.join(wd, out + ext), os.path.join(HERE, out + ext))
+            a, _ = read_lcoef(os.path.join(wd, out + ".lcoef"))
+            print(out, "stopped after", len(a), "iterations")
+        finally:
+            shutil.rmtree(wd)
+
+
 def config4_coefficients(niter=4):
     """pf 6p6n M=0 (D = 109,954,620): the reference's option li
     (src/blanczosmain.f90:787-795) for `niter` Lanczos steps from its uniform
@@ -651,6 +719,17 @@ if __name__ == "__main__":
             hmatrix(*c)
     if "config2" in want:
         ensemble()
+    if "jumps" in want:
+        # the dump prints the neutron two-body op array beside every jump
+        # (src/boutputlib.f90:1337); with fewer than two neutrons that array
+        # is never allocated and the reference faults there, so only systems
+        # with N >= 2 have a dump
+        for c in SMALL:
+            if c[3] >= 2:
+                jumps(*c)
+        jumps("config0", None, None, None, None, None, case=config_case(0))
+    if "ld" in want:
+        default_convergence()
     if "config4" in want:
         config4_coefficients()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

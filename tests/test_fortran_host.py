@@ -308,7 +308,8 @@ def _two_float_rows(path):
 @pytest.mark.gpu
 def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
     """The reference's own 25Mg / USDB example (examples/sd, outputs committed
-    by its authors) replayed with this host's menus: n (10 states, ld 400),
+    by its authors) replayed with this host's menus: n (10 states, lc 400
+    with the largest-change test at 1e-6: its 180-iteration run is converged),
     o (the spin operator sdspin.opme on every state), s (30 iterations from
     O|state 1>), x (<H> in the ten states).  Energies, J, T to the printed
     digits; the strength table's total, its first rows and its centroid.
@@ -324,7 +325,7 @@ def test_fortran_host_mg25_usdb_chain_matches_reference_example(casedir):
         shutil.copy(os.path.join(gold, f), wd)
     scal = "1.0000 18.0000 25.0000 0.3000"
     steps = [
-        "n\nmg25ref\nsd\n4 5\n1\nusdb\n%s\nend\nld\n10 400\n" % scal,
+        "n\nmg25ref\nsd\n4 5\n1\nusdb\n%s\nend\nlc\n10 400\n5\n1\n1e-6\n" % scal,
         "o\nmg25ref\nmg25spin\nsdspin\n",
         "s\nmg25spin\nmg25spinstrength1\nusdb\n%s\nend\n30 30\n1\n" % scal,
         "x\nmg25ref\nmg25expectH\nusdb\n%s\nend\n" % scal,
@@ -600,6 +601,57 @@ def test_fortran_host_user_tolerance_matches_reference(name, option, answers, ca
     assert got.shape == ref.shape == (5, 3)
     assert np.abs(got[:, 0] - ref[:, 0]).max() < 1e-4, (got[:, 0], ref[:, 0])
     assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3, (got, ref)
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_fortran_host_default_convergence_matches_reference(dtype, casedir):
+    """Option ld (the reference's default convergence: test 0, tolerance 1e-3,
+    nkeep + 5 energies every 10 iterations).  Where the test's passage does
+    not hinge on rounding -- sd 1p2n at 2Jz = 1 and pf 2p1n at 2Jz = 3 (the
+    uniform pivot overlaps every low state) and config 0 -- the .lcoef ends
+    exactly where the reference's does (after 70 and 100 iterations, at the
+    150 cap), its first coefficients equal the reference's to FP32-vector
+    accuracy and the energies equal its .res.  On sd 2p2n three low states
+    have overlaps 1e-8..1e-10 with the uniform pivot and enter the Krylov
+    space through rounding, so the mean change jumps up and down (an FP64 run
+    passes the test at 90, the reference's FP32 run at 100): there the run
+    must stop exactly where the rule, replayed on its own coefficients, says
+    (lanczos_ref.convergence_stop, pinned to the reference's stopping points
+    in tests/test_oracle.py)."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    from lanczos_ref import convergence_stop
+    runs = []
+    for name, seed, space, Z, N, jz2, answer, want_len in (
+            ("sd1p2n_m1", 78, "sd", 1, 2, 1, "5 120", 70), ("pf2p1n_m3", 83, "pf", 2, 1, 3, "5 300", 100),
+            ("sd2p2n", 77, "sd", 2, 2, 0, "5 300", None)):
+        c = small_case(space, Z, N, jz2, seed=seed)
+        c.name = name
+        runs.append((c, answer, want_len))
+    runs.append((config_fixture(0, casedir)[0], "5 150", 150))
+    for cs, answer, want_len in runs:
+        wd = os.path.join(casedir, "fld_%s_%s" % (cs.name, dtype))
+        write_case(cs, wd)
+        out = cs.name + "_ld"
+        lines = ["n", out, cs.space, "%d %d" % (cs.Z, cs.N), str(cs.jz2), cs.name, "1 1 1 1", "end", "ld", answer]
+        r = _run("\n".join(lines) + "\n", wd, {"BS_DTYPE": dtype})
+        assert r.returncode == 0, r.stdout + r.stderr
+        lc = np.loadtxt(os.path.join(wd, out + ".lcoef"))
+        rlc = np.loadtxt(os.path.join(GOLD, out + ".lcoef"))
+        cap = min(int(answer.split()[1]), 127 if cs.name == "sd1p2n_m1" else 10 ** 9)
+        assert len(lc) == convergence_stop(lc[:, 1], lc[:, 2], 5, niter=cap), cs.name
+        assert np.abs(lc[:20, 1:] - rlc[:20, 1:]).max() < 2e-4, cs.name
+        if want_len is None:
+            continue
+        assert lc.shape == rlc.shape == (want_len, 3), (cs.name, lc.shape)
+        got = read_res(os.path.join(wd, out + ".res"))
+        ref = read_res(os.path.join(GOLD, out + ".res"))
+        assert got.shape == ref.shape == (5, 3)
+        # the lowest states are converged in both runs; the 5th of config 0
+        # only to the reference's own FP32 accuracy at the cap
+        assert np.abs(got[:3, 0] - ref[:3, 0]).max() < 2e-5, (cs.name, got, ref)
+        assert np.abs(got[:, 0] - ref[:, 0]).max() < 2e-3, (cs.name, got, ref)
 
 
 @needs_driver

@@ -275,3 +275,36 @@ def test_oracle_row_apply_equals_full_apply(casedir):
     assert mask.sum() > 0
     assert np.array_equal(part[mask], full[mask])
     assert not np.any(part[~mask])
+
+
+def test_convergence_rule_reproduces_reference_stopping_points():
+    """The restated default convergence test (lanczos_ref.convergence_stop),
+    replayed on the reference's own coefficients of its option-ld runs, stops
+    exactly where the reference's .lcoef ends: sd 2p2n after 100 iterations
+    (mean change 7.4e-4 < 1e-3 there, 0.37 at 90), config 0 never before its
+    150-iteration cap."""
+    from lanczos_ref import convergence_stop
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    a, b = _read_lcoef(os.path.join(gold, "sd2p2n_ld.lcoef"))
+    assert len(a) == 100
+    assert convergence_stop(a, b, 5, niter=300) == 100
+    # planted variants of the rule do not stop there: the largest change
+    # instead of the mean, no extra states (stops at 80), a tighter tolerance
+    assert convergence_stop(a, b, 5, niter=300, test=1) != 100
+    assert convergence_stop(a, b, 5, ncheck_ex=0, niter=300) == 80
+    assert convergence_stop(a, b, 5, tol=3e-4, niter=300) != 100
+    a0, b0 = _read_lcoef(os.path.join(gold, "sd4p4n_ld.lcoef"))
+    assert len(a0) == 150 and convergence_stop(a0, b0, 5, niter=150) == 150
+    for name, stop, cap in (("sd1p2n_m1", 70, 127), ("pf2p1n_m3", 100, 300)):
+        a1, b1 = _read_lcoef(os.path.join(gold, "%s_ld.lcoef" % name))
+        assert len(a1) == stop and convergence_stop(a1, b1, 5, niter=cap) == stop, name
+
+
+def _read_lcoef(path):
+    a, b = [], []
+    for line in open(path):
+        p = line.split()
+        if len(p) == 3:
+            a.append(float(p[1]))
+            b.append(float(p[2]))
+    return np.array(a), np.array(b)
