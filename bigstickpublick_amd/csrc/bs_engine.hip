@@ -75,6 +75,13 @@ struct DevTables {
   const int32_t* tile_f0;
   const int32_t* tile_nr;
   const int32_t* tile_map;
+  // PP per tile: the union of the tile rows' source rows (sorted), each entry
+  // the source SD | (mask of the tile rows it feeds) << 27 and R values
+  // (zero where it does not feed a row)
+  const int64_t* tpp_ptr;     // [ntiles + 1]
+  const uint32_t* tpp_src;
+  const float* tpp_val32;     // [entry * R + r]
+  const double* tpp_val64;
   const int32_t* psec;       // [nP] proton sector of each SD
   const int32_t* pconj;      // [nsecP] conjugate neutron sector
   const int32_t* nlen;       // [nsecN] SDs in neutron sector
@@ -195,59 +202,56 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
       }
     }
   }
-  // ---- PP: row AXPYs with coalesced reads
+  // ---- PP: row AXPYs over the union of the tile rows' source rows (sorted
+  // by source SD): each source row is read once, with coalesced buffer loads
+  // through a scalar descriptor, and applied to every tile row its mask
+  // names (wave-uniform branches); four source rows' loads in flight
+  {
+    const int64_t e0 = tb.tpp_ptr[tile], e1 = tb.tpp_ptr[tile + 1];
+    int64_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+      __amdgpu_buffer_rsrc_t rr[4];
+      uint32_t mk[4];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    if (r < nr) {
-      const int f = f0 + r;
-      // source rows through a scalar buffer descriptor per entry (no vector
-      // address math), four (then two) entries' loads in flight together
-      const int64_t e0 = tb.pp_ptr[f], e1 = tb.pp_ptr[f + 1];
-      int64_t e = e0;
-      for (; e + 4 <= e1; e += 4) {
-        __amdgpu_buffer_rsrc_t rr[4];
-        T vv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          rr[u] = buf_rsrc(x + tb.pstart[tb.pp_src[e + u]]);
-          vv[u] = vload<T>(tb.pp_val32, tb.pp_val64, e + u);
-        }
-        T xx[4][CPT];
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int q = 0; q < CPT; q++) {
-            if constexpr (sizeof(T) == 4) xx[u][q] = __uint_as_float(buf_u32(rr[u], cl[q] * 4u, 0u));
-            else xx[u][q] = buf_f64(rr[u], cl[q] * 8u, 0u);
-          }
-#pragma unroll
-        for (int q = 0; q < CPT; q++)
-          acc[r][q] = fma(vv[3], xx[3][q], fma(vv[2], xx[2][q], fma(vv[1], xx[1][q], fma(vv[0], xx[0][q], acc[r][q]))));
+      for (int u = 0; u < 4; u++) {
+        const uint32_t sm = tb.tpp_src[e + u];
+        mk[u] = sm >> 27;
+        rr[u] = buf_rsrc(x + tb.pstart[sm & 0x7ffffffu]);
       }
-      for (; e + 2 <= e1; e += 2) {
-        const __amdgpu_buffer_rsrc_t ra = buf_rsrc(x + tb.pstart[tb.pp_src[e]]);
-        const __amdgpu_buffer_rsrc_t rb = buf_rsrc(x + tb.pstart[tb.pp_src[e + 1]]);
-        const T va = vload<T>(tb.pp_val32, tb.pp_val64, e), vb = vload<T>(tb.pp_val32, tb.pp_val64, e + 1);
-        T xa[CPT], xb[CPT];
+      T xx[4][CPT];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
-          if constexpr (sizeof(T) == 4) {
-            xa[q] = __uint_as_float(buf_u32(ra, cl[q] * 4u, 0u));
-            xb[q] = __uint_as_float(buf_u32(rb, cl[q] * 4u, 0u));
-          } else {
-            xa[q] = buf_f64(ra, cl[q] * 8u, 0u);
-            xb[q] = buf_f64(rb, cl[q] * 8u, 0u);
-          }
+          if constexpr (sizeof(T) == 4) xx[u][q] = __uint_as_float(buf_u32(rr[u], cl[q] * 4u, 0u));
+          else xx[u][q] = buf_f64(rr[u], cl[q] * 8u, 0u);
         }
 #pragma unroll
-        for (int q = 0; q < CPT; q++) acc[r][q] = fma(vb, xb[q], fma(va, xa[q], acc[r][q]));
-      }
-      if (e < e1) {
-        const T* xi = x + tb.pstart[tb.pp_src[e]];
-        const T v = vload<T>(tb.pp_val32, tb.pp_val64, e);
+      for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int q = 0; q < CPT; q++) acc[r][q] = fma(v, xi[cl[q]], acc[r][q]);
+        for (int r = 0; r < R; r++)
+          if ((mk[u] >> r) & 1u) {
+            const T v = vload<T>(tb.tpp_val32, tb.tpp_val64, (e + u) * R + r);
+#pragma unroll
+            for (int q = 0; q < CPT; q++) acc[r][q] = fma(v, xx[u][q], acc[r][q]);
+          }
+    }
+    for (; e < e1; e++) {
+      const uint32_t sm = tb.tpp_src[e];
+      const __amdgpu_buffer_rsrc_t ra = buf_rsrc(x + tb.pstart[sm & 0x7ffffffu]);
+      T xa[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        if constexpr (sizeof(T) == 4) xa[q] = __uint_as_float(buf_u32(ra, cl[q] * 4u, 0u));
+        else xa[q] = buf_f64(ra, cl[q] * 8u, 0u);
       }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (((sm >> 27) >> r) & 1u) {
+          const T v = vload<T>(tb.tpp_val32, tb.tpp_val64, e * R + r);
+#pragma unroll
+          for (int q = 0; q < CPT; q++) acc[r][q] = fma(v, xa[q], acc[r][q]);
+        }
     }
   }
   // ---- NN: the tile's rows staged transposed ([col][R]); one wide LDS read
@@ -295,6 +299,134 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
   const uint32_t* pkt = sizeof(T) == 4 ? tb.n1_pkv[gsel][0] : tb.n1_pkv[gsel][1];
   const int nd = tb.ndelta, nd1 = nd + 1;
   float* wb = reinterpret_cast<float*>(smem);
+  constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
+  if constexpr (R == G) {
+    // one staged row per tile row: batch b of class k holds jump b of every
+    // tile row (slot g = row g; rows with fewer jumps leave their slot empty,
+    // zero coefficients), so one walk of the neutron entries serves all R
+    // rows and each entry's G products go to G independent accumulators
+    for (int k = 0; k < nd; k++) {
+      int p0[R], nj[R];
+      int nbmax = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        p0[r] = nj[r] = 0;
+        if (r < nr) {
+          p0[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k];
+          nj[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k + 1] - p0[r];
+          nbmax = max(nbmax, nj[r]);
+        }
+      }
+      if (nbmax == 0) continue;
+      const int kn = tb.negk[k];
+      const int si = tb.srcsec[sp * nd + k];
+      const int Li = tb.nlen[tb.pconj[si]];
+      const int wsd = tb.wstride[k];
+      const float* Wk = tb.W + tb.woff[k];
+      const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1_base[(int64_t)cn * nd + kn]);
+      const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
+      int Eq[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) Eq[q] = ch[q * (BLOCK / 64)];
+      VecG<T, G> pre[PF ? CPT : 1];
+      if constexpr (PF) {
+        int64_t bi[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) bi[g] = nj[g] > 0 ? tb.pstart[tb.p1_src[p0[g]]] : -1;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int c = tid + q * BLOCK;
+#pragma unroll
+          for (int g = 0; g < G; g++) pre[q].v[g] = (c < Li && bi[g] >= 0) ? x[bi[g] + c] : T(0);
+        }
+      }
+      for (int b = 0; b < nbmax; b++) {
+        float wreg[kWbPer];
+#pragma unroll
+        for (int u = 0; u < kWbPer; u++) {
+          const int i = tid + u * BLOCK;
+          const int row = i / G, g = i - row * G, nl = row >> 1;
+          float w = 0.0f;
+          int njg = 0, p0g = 0;
+#pragma unroll
+          for (int gg = 0; gg < G; gg++)
+            if (gg == g) {
+              njg = nj[gg];
+              p0g = p0[gg];
+            }
+          if (i < 2 * (wsd + 1) * G && b < njg && nl < wsd) {
+            const int meta = tb.p1_meta[p0g + b];
+            w = Wk[(meta & 0xff) * wsd + nl];
+            if ((meta >> 8) ^ (row & 1)) w = -w;
+          }
+          wreg[u] = w;
+        }
+        __syncthreads();
+        if constexpr (PF) {
+#pragma unroll
+          for (int q = 0; q < CPT; q++) {
+            const int c = tid + q * BLOCK;
+            if (c < Li) *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = pre[q];
+          }
+        } else {
+          int64_t bi[G];
+#pragma unroll
+          for (int g = 0; g < G; g++) bi[g] = b < nj[g] ? tb.pstart[tb.p1_src[p0[g] + b]] : -1;
+          for (int c = tid; c < Li; c += BLOCK) {
+            VecG<T, G> v;
+#pragma unroll
+            for (int g = 0; g < G; g++) v.v[g] = bi[g] >= 0 ? x[bi[g] + c] : T(0);
+            *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = v;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kWbPer; u++) {
+          const int i = tid + u * BLOCK;
+          if (i < 2 * (wsd + 1) * G) wb[i] = wreg[u];
+        }
+        __syncthreads();
+        if constexpr (PF) {
+          if (b + 1 < nbmax) {
+            int64_t bi[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) bi[g] = b + 1 < nj[g] ? tb.pstart[tb.p1_src[p0[g] + b + 1]] : -1;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+              const int c = tid + q * BLOCK;
+#pragma unroll
+              for (int g = 0; g < G; g++) pre[q].v[g] = (c < Li && bi[g] >= 0) ? x[bi[g] + c] : T(0);
+            }
+          }
+        }
+        const uint32_t step = (uint32_t)L * 4u;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const int Ee = Eq[q];
+          const uint32_t vo = cl[q] * 4u;
+          int e = 0;
+          for (; e + 4 <= Ee; e += 4) {
+            uint32_t pk[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) pk[v] = buf_u32(rpk, vo, (uint32_t)(e + v) * step);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[v] & 0x1ffffu));
+              const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[v] >> 17));
+#pragma unroll
+              for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
+            }
+          }
+          for (; e < Ee; e++) {
+            const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
+            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
+            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
+#pragma unroll
+            for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
+          }
+        }
+      }
+    }
+  } else
   for (int k = 0; k < nd; k++) {
     int p0[R], p1[R];
     int any = 0;
@@ -347,7 +479,6 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
         const int nb = min(G, p1[r] - b0);
         // this batch's signed coefficients, loaded first so that their
         // latency overlaps the source-row staging (one or two per thread)
-        constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
         float wreg[kWbPer];
 #pragma unroll
         for (int u = 0; u < kWbPer; u++) {
@@ -1659,6 +1790,10 @@ struct bs_system {
   DevTables dt{};
   DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base;
   DevBuf<int32_t> tile_f0[3], tile_nr[3], tile_map[3][2];
+  DevBuf<int64_t> tpp_ptr[3];
+  DevBuf<uint32_t> tpp_src[3];
+  DevBuf<float> tpp_val32[3];
+  DevBuf<double> tpp_val64[3];
   int ntiles[3] = {0, 0, 0};
   int nslots[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   DevBuf<int32_t> psec, pconj, nlen, nfirst, srcsec, pp_src, nn_emax, nn_src, pcls, p1_src, n1_emax;
@@ -1860,6 +1995,52 @@ void upload_tables(bs_system* s) {
     const int nt = (int)f0.size();
     s->tile_f0[v].upload(f0);
     s->tile_nr[v].upload(nr);
+    {
+      // PP source union per tile (sources of all R rows, each listed once)
+      const int R = kTileRows[v];
+      const bs::TwoBody& T = t.tb[0];
+      std::vector<int64_t> ptr(nt + 1, 0);
+      std::vector<uint32_t> src;
+      std::vector<float> v32;
+      std::vector<double> v64;
+      std::vector<std::pair<int32_t, int64_t>> ent;  // (source SD, entry e of row r: e * 4 + r)
+      for (int ti = 0; ti < nt; ti++) {
+        ent.clear();
+        for (int r = 0; r < nr[ti]; r++)
+          for (int64_t e = T.rowptr[f0[ti] + r]; e < T.rowptr[f0[ti] + r + 1]; e++) ent.push_back({T.src[e], e * 4 + r});
+        std::sort(ent.begin(), ent.end());
+        for (size_t q = 0; q < ent.size();) {
+          size_t u = q;
+          const int32_t sd = ent[q].first;
+          if ((uint32_t)sd >= (1u << 27)) throw std::runtime_error("too many proton SDs for the PP tile lists");
+          uint32_t mask = 0;
+          const size_t b = v32.size();
+          v32.resize(b + R, 0.0f);
+          v64.resize(b + R, 0.0);
+          while (u < ent.size() && ent[u].first == sd) {
+            // the row's merged value for this source (one entry per pair)
+            const int r = (int)(ent[u].second & 3);
+            const int64_t e = ent[u].second >> 2;
+            v64[b + r] = T.val[e];
+            v32[b + r] = (float)T.val[e];
+            mask |= 1u << r;
+            u++;
+          }
+          src.push_back((uint32_t)sd | (mask << 27));
+          q = u;
+        }
+        ptr[ti + 1] = (int64_t)src.size();
+      }
+      if (src.empty()) {
+        src.push_back(0);
+        v32.assign(R, 0.0f);
+        v64.assign(R, 0.0);
+      }
+      s->tpp_ptr[v].upload(ptr);
+      s->tpp_src[v].upload(src);
+      s->tpp_val32[v].upload(v32);
+      s->tpp_val64[v].upload(v64);
+    }
     s->ntiles[v] = nt;
     std::vector<int32_t> ident(nt);
     for (int i = 0; i < nt; i++) ident[i] = i;
@@ -2007,6 +2188,10 @@ void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumu
   d.tile_f0 = s->tile_f0[ri].p;
   d.tile_nr = s->tile_nr[ri].p;
   d.tile_map = s->tile_map[ri][map].p;
+  d.tpp_ptr = s->tpp_ptr[ri].p;
+  d.tpp_src = s->tpp_src[ri].p;
+  d.tpp_val32 = s->tpp_val32[ri].p;
+  d.tpp_val64 = s->tpp_val64[ri].p;
   // static LDS (lds_bytes_rg)
   auto fn = pf ? k_apply_tiles<T, kApplyBlock, CPT, R, G, true> : k_apply_tiles<T, kApplyBlock, CPT, R, G, false>;
   hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), 0, st, d, x, y, accumulate);

@@ -29,6 +29,8 @@ from bigstickpublick_amd.inputs import config_case, small_case, write_case, menu
 from oracle_api import read_basis_dump  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "bigstick.x")
+# the reference's FP64 build (make -C oracle ref64: lanc_prec = 8)
+REF64 = os.path.join(ROOT, "oracle", "_ref", "fp64", "bigstick.x")
 
 # (name, space, Z, N, 2Jz, seed): small systems covering the edge cases of the
 # path: no PP (Z=1), no protons (Z=0), odd A, M != 0, one particle (SPE not
@@ -45,7 +47,7 @@ SMALL = [
 ]
 
 
-def run_ref(workdir, script, env_extra=None, timeout=3600):
+def run_ref(workdir, script, env_extra=None, timeout=3600, exe=None):
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = "8"
     if env_extra:
@@ -54,7 +56,7 @@ def run_ref(workdir, script, env_extra=None, timeout=3600):
         f.write(script)
     with open(os.path.join(workdir, "stdin.txt")) as fin, \
             open(os.path.join(workdir, "stdout.txt"), "w") as fout:
-        r = subprocess.run([REF], stdin=fin, stdout=fout, stderr=subprocess.STDOUT,
+        r = subprocess.run([exe or REF], stdin=fin, stdout=fout, stderr=subprocess.STDOUT,
                            cwd=workdir, env=env, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError("reference run failed in %s" % workdir)
@@ -428,6 +430,43 @@ def jumps(name, space, Z, N, jz2, seed, case=None):
         shutil.rmtree(wd)
 
 
+def config_fp64(cfg):
+    """The reference's FP64 build (lanc_prec = 8) on config 0 / 1 with the
+    configuration's own fixed run (lf: nkeep 5 / 10, 150 iterations):
+    .res energies and the FP64 .lcoef coefficients as config<cfg>_fp64.npz."""
+    case = config_case(cfg)
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, menu_script(case, "n", "lf"), exe=REF64, timeout=7200)
+        res = read_res(os.path.join(wd, case.name + ".res"))
+        a, b = read_lcoef(os.path.join(wd, case.name + ".lcoef"))
+        np.savez_compressed(os.path.join(HERE, "config%d_fp64.npz" % cfg), res=res, alpha=a, beta=b,
+                            niter=case.niter, nkeep=case.nkeep)
+        print("config%d fp64" % cfg, res[:, 0], len(a))
+    finally:
+        shutil.rmtree(wd)
+
+
+def config3_thick():
+    """Config 3 (pf 4p4n, D = 1,963,461) solved as the configuration states it
+    -- thick restart keeping 15, restarting at 60, lowest 10 -- by the
+    reference's FP64 build (option tc, largest-change test at 1e-6 MeV, at
+    most 40 restarts): .res energies, J, T as config3_tc.npz."""
+    case = config_case(3)
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        script = "\n".join(["n", case.name, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), case.name,
+                            "1 1 1 1", "end", "tc", "10 60", "15", "40", "1", "1e-6"]) + "\n"
+        run_ref(wd, script, exe=REF64, timeout=7200)
+        res = read_res(os.path.join(wd, case.name + ".res"))
+        np.savez_compressed(os.path.join(HERE, "config3_tc.npz"), res=res)
+        print("config3 tc", res[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 def default_convergence():
     """The reference's option ld (default convergence: test 0, the mean
     change of nkeep + 5 energies over 10 iterations below 1e-3,
@@ -730,6 +769,11 @@ if __name__ == "__main__":
         jumps("config0", None, None, None, None, None, case=config_case(0))
     if "ld" in want:
         default_convergence()
+    if "config3_tc" in want:
+        config3_thick()
+    for cfg in (0, 1):
+        if "fp64_%d" % cfg in want:
+            config_fp64(cfg)
     if "config4" in want:
         config4_coefficients()
     if "config3" in want:  # D = 1,963,461: basis + the first 20 Lanczos steps

@@ -1,0 +1,90 @@
+"""Config 3 (pf 4p4n M=0, D = 1,963,461) solved as the configuration states it
+-- thick-restart Lanczos keeping 15 vectors, restarting at 60, lowest 10 --
+on the GPU in FP64 and in mixed precision, against the reference's FP64
+build solving the same (option tc to 1e-6 MeV, tests/golden/config3_tc.npz);
+and the adjoint gradient over all 199 parameters (4 SPEs, 195 TBMEs):
+E = sum_p theta_p dE/dtheta_p (H is linear in its parameters) and central
+differences at h = 1e-2 MeV on a sample of parameters."""
+import os
+
+import numpy as np
+import pytest
+
+from bigstickpublick_amd.inputs import config_case, f32, write_case
+from golden_cases import GOLD, paths
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg3(tmp_path_factory):
+    from bigstickpublick_amd import System
+    d = str(tmp_path_factory.mktemp("cfg3"))
+    case = config_case(3)
+    write_case(case, d)
+    s = System(*paths(case, d), case.Z, case.N, case.jz2, upload=True)
+    yield case, d, s
+    s.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_config3_thick_restart_matches_reference(cfg3, dtype):
+    case, d, s = cfg3
+    ref = np.load(os.path.join(GOLD, "config3_tc.npz"))["res"]
+    assert ref.shape[0] == 10
+    r = s.lanczos(nkeep=10, niter=60, thick=15, max_restarts=60, dtype=dtype, tol=1e-8, polish=True)
+    print("config 3 %s: %d matvecs, E %s, E_ref(FP64) %s, residuals %s" % (
+        dtype, r.matvecs, " ".join("%.5f" % x for x in r.evals), " ".join("%.5f" % x for x in ref[:, 0]),
+        " ".join("%.0e" % x for x in r.resid)))
+    # every value here is an eigenvalue (residual below 1e-5 / 1e-3), so by
+    # min-max the reference's k-th Ritz value cannot lie below it
+    assert r.resid.max() < (1e-5 if dtype == "f64" else 1e-3)
+    assert np.all(r.evals <= ref[:, 0] + 1.5e-5), (r.evals, ref[:, 0])
+    # the eight lowest agree to the reference's printed digits (f10.5); the
+    # reference's converged run has no state at -93.910 MeV (the uniform
+    # pivot has no component along it in exact arithmetic, and its FP64
+    # rounding had not brought it in), so its 9th value is this run's 10th
+    assert np.abs(r.evals[:8] - ref[:8, 0]).max() < 1.5e-5, (r.evals, ref[:, 0])
+    assert abs(r.evals[9] - ref[8, 0]) < 1.5e-5 and r.evals[8] < ref[8, 0] - 1e-3, (r.evals, ref[:, 0])
+
+
+def _solve_e0(case, d, theta, tag):
+    from bigstickpublick_amd import System
+    norb = len(case.orbits)
+    it = case.interaction
+    c = type(case)(**{**case.__dict__})
+    c.interaction = type(it)(it.orbits, tuple(theta[:norb]), [l[:6] + (v,) for l, v in zip(it.tbmes, theta[norb:])])
+    c.name = tag
+    write_case(c, d)
+    s = System(*paths(c, d), case.Z, case.N, case.jz2, upload=True)
+    r = s.lanczos(nkeep=1, niter=60, thick=15, max_restarts=60, dtype="f64", tol=1e-11, polish=True)
+    s.close()
+    return r.evals[0]
+
+
+def test_config3_gradient_all_parameters(cfg3):
+    case, d, s = cfg3
+    r = s.lanczos(nkeep=1, niter=60, thick=15, max_restarts=60, dtype="f64", tol=1e-11, polish=True,
+                  want_vectors=True)
+    g = s.gradient(r.evecs[0])
+    theta = list(case.interaction.spe) + [t[6] for t in case.interaction.tbmes]
+    nspe, ntb = s.num_params()
+    assert (nspe, ntb) == (4, 195) and len(g) == len(theta) == 199
+    assert np.all(np.isfinite(g))
+    # H(theta) = sum_p theta_p H_p (no mass scaling), so E = sum_p theta_p g_p
+    # up to the FP32 rounding of the matrix elements
+    euler = float(np.dot(theta, g))
+    assert abs(euler - r.evals[0]) < 1e-5 * abs(r.evals[0]), (euler, r.evals[0])
+    # central differences on the four SPEs and four TBMEs (largest |g| and
+    # three seeded ones)
+    rng = np.random.default_rng(3)
+    tb = nspe + np.argsort(-np.abs(g[nspe:]))[:1]
+    pick = list(range(nspe)) + list(tb) + sorted(rng.choice(np.arange(nspe, len(theta)), 3, replace=False))
+    for p in pick:
+        h = 1e-2
+        tp, tm = list(theta), list(theta)
+        tp[p] = f32(theta[p] + h)
+        tm[p] = f32(theta[p] - h)
+        fd = (_solve_e0(case, d, tp, "gp") - _solve_e0(case, d, tm, "gm")) / (tp[p] - tm[p])
+        assert abs(fd - g[p]) < 2e-3, (p, fd, g[p])
