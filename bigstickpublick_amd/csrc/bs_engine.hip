@@ -109,7 +109,9 @@ struct DevTables {
   const int64_t* n1_base;
   const int32_t* n1_emax;
   const uint32_t* n1_pk;      // src | nl << 16 | sign << 24
-  const uint32_t* n1_pkv[3][2];  // per G (1, 2, 4) and T (f32, f64): LDS byte offsets, see k_apply_tiles
+  const uint32_t* n1_pkv[3][2];  // per G (1, 2, 4) and T (f32, f64): LDS byte offsets, see k_apply_tiles;
+                                 // quad layout: slot e of column n at n1q_base[c*ndelta+k] + ((e/4)*L_c + n)*4 + e%4
+  const int64_t* n1q_base;
   const uint8_t* n1_chunk;       // [(c*ndelta + k)*64 + j]: entries used by columns 64j..64j+63
   const uint32_t* n1_pkr;        // src | (2*op + sign) << 16 (multi-vector kernels)
   // pn element blocks: class k block is [size_k][size_-k] at woff[k]
@@ -134,6 +136,46 @@ __device__ __forceinline__ uint32_t buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t v
 }
 __device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 buf_u32x4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+}
+
+// The neutron one-body entries of one PN batch for the thread's CPT columns
+// (column q: entry count Eq[q], wave-uniform), four entries to a 16-byte
+// load ([e/4][column][4], qstep bytes per quad row).  The next quad (the
+// rest of this column, or the next column's first) is in flight while the
+// current one is applied; body(q, pk) applies one entry.
+template <int CPT, typename Body>
+__device__ __forceinline__ void walk_quads(__amdgpu_buffer_rsrc_t rpk, const uint32_t (&cl)[CPT], const int (&Eq)[CPT],
+                                           uint32_t qstep, Body&& body) {
+  u32x4 pq = {0u, 0u, 0u, 0u};
+  if (Eq[0] > 0) pq = buf_u32x4(rpk, cl[0] * 16u, 0u);
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int Ee = Eq[q];
+    u32x4 pn = {0u, 0u, 0u, 0u};
+    if (q + 1 < CPT) {
+      if (Eq[q + 1] > 0) pn = buf_u32x4(rpk, cl[q + 1] * 16u, 0u);
+    }
+    for (int e = 0; e < Ee; e += 4) {
+      const u32x4 cur = pq;
+      if (e + 4 < Ee) pq = buf_u32x4(rpk, cl[q] * 16u, (uint32_t)((e >> 2) + 1) * qstep);
+      if (Ee - e >= 4) {
+        body(q, cur.x);
+        body(q, cur.y);
+        body(q, cur.z);
+        body(q, cur.w);
+      } else {
+        body(q, cur.x);
+        if (Ee - e > 1) body(q, cur.y);
+        if (Ee - e > 2) body(q, cur.z);
+      }
+    }
+    pq = pn;
+  }
 }
 
 // 16-byte vector of T for streaming passes
@@ -323,7 +365,7 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
       const int Li = tb.nlen[tb.pconj[si]];
       const int wsd = tb.wstride[k];
       const float* Wk = tb.W + tb.woff[k];
-      const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1_base[(int64_t)cn * nd + kn]);
+      const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1q_base[(int64_t)cn * nd + kn]);
       const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
       int Eq[CPT];
 #pragma unroll
@@ -398,32 +440,12 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
             }
           }
         }
-        const uint32_t step = (uint32_t)L * 4u;
+        walk_quads<CPT>(rpk, cl, Eq, (uint32_t)L * 16u, [&](int q, uint32_t pk) {
+          const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
+          const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int Ee = Eq[q];
-          const uint32_t vo = cl[q] * 4u;
-          int e = 0;
-          for (; e + 4 <= Ee; e += 4) {
-            uint32_t pk[4];
-#pragma unroll
-            for (int v = 0; v < 4; v++) pk[v] = buf_u32(rpk, vo, (uint32_t)(e + v) * step);
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-              const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[v] & 0x1ffffu));
-              const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[v] >> 17));
-#pragma unroll
-              for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
-            }
-          }
-          for (; e < Ee; e++) {
-            const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
-            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
-            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
-#pragma unroll
-            for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
-          }
-        }
+          for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
+        });
       }
     }
   } else
@@ -445,7 +467,7 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
     const int Li = tb.nlen[tb.pconj[si]];
     const int wsd = tb.wstride[k];
     const float* Wk = tb.W + tb.woff[k];
-    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1_base[(int64_t)cn * nd + kn]);
+    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(pkt + tb.n1q_base[(int64_t)cn * nd + kn]);
     // per 64-column chunk entry counts: the chunk of column tid + q*BLOCK is
     // q*(BLOCK/64) + wave, uniform across the wave
     const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -547,39 +569,18 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
           }
         }
         // column-major over the thread's columns: each (wave, q) runs to its
-        // own chunk's entry count (wave-uniform), four entries per step with
-        // their loads in flight together
-        const uint32_t step = (uint32_t)L * 4u;
+        // own chunk's entry count (wave-uniform), four entries to a load
+        T pa[CPT];
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-          const int Ee = Eq[q];
-          const uint32_t vo = cl[q] * 4u;
-          T a0 = T(0), a1 = T(0);
-          int e = 0;
-          for (; e + 4 <= Ee; e += 4) {
-            uint32_t pk[4];
+        for (int q = 0; q < CPT; q++) pa[q] = T(0);
+        walk_quads<CPT>(rpk, cl, Eq, (uint32_t)L * 16u, [&](int q, uint32_t pk) {
+          const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
+          const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
 #pragma unroll
-            for (int v = 0; v < 4; v++) pk[v] = buf_u32(rpk, vo, (uint32_t)(e + v) * step);
+          for (int g = 0; g < G; g++) pa[q] = fma((T)wv.v[g], xv.v[g], pa[q]);
+        });
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-              const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk[v] & 0x1ffffu));
-              const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk[v] >> 17));
-#pragma unroll
-              for (int g = 0; g < G; g++) {
-                if (v & 1) a1 = fma((T)wv.v[g], xv.v[g], a1);
-                else a0 = fma((T)wv.v[g], xv.v[g], a0);
-              }
-            }
-          }
-          for (; e < Ee; e++) {
-            const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
-            const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
-            const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
-#pragma unroll
-            for (int g = 0; g < G; g++) a0 = fma((T)wv.v[g], xv.v[g], a0);
-          }
-          acc[r][q] += a0 + a1;
-        }
+        for (int q = 0; q < CPT; q++) acc[r][q] += pa[q];
       }
     }
   }
@@ -1788,7 +1789,7 @@ struct bs_system {
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
   float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
   DevTables dt{};
-  DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base;
+  DevBuf<int64_t> pstart, pp_ptr, nn_base, n1_base, n1q_base;
   DevBuf<int32_t> tile_f0[3], tile_nr[3], tile_map[3][2];
   DevBuf<int64_t> tpp_ptr[3];
   DevBuf<uint32_t> tpp_src[3];
@@ -2099,21 +2100,43 @@ void upload_tables(bs_system* s) {
   }
   // the same entries as LDS byte offsets for each staged-row count G and
   // element type: x offset src*G*sizeof(T) (< 2^17) | Wb offset
-  // (2*nl + sign)*G*4 << 17; padding (nl = class size) lands on a zero row
+  // (2*nl + sign)*G*4 << 17; padding (nl = class size) lands on a zero row.
+  // Quad layout for 16-byte loads: per (sector, class) the slot count is
+  // rounded up to a multiple of 4 and slot e of column n sits at
+  // ((e/4)*L + n)*4 + e%4
   if (2 * (t.max_class + 1) > kWbRows) throw std::runtime_error("operator class too large for the Wb table");
+  std::vector<int64_t> n1q_base((size_t)N.nsec * nd);
+  int64_t qtot = 0;
+  for (int c = 0; c < N.nsec; c++)
+    for (int k = 0; k < nd; k++) {
+      n1q_base[(size_t)c * nd + k] = qtot;
+      qtot += (int64_t)((n1_emax[(size_t)c * nd + k] + 3) / 4 * 4) * N.seccount[c];
+    }
+  std::vector<uint32_t> quad(std::max<int64_t>(qtot, 1), 0);
+  for (int c = 0; c < N.nsec; c++) {
+    const int L = nlen[c];
+    for (int k = 0; k < nd; k++) {
+      const int em = n1_emax[(size_t)c * nd + k], em4 = (em + 3) / 4 * 4;
+      const uint32_t pad = (uint32_t)t.class_size[k] << 16;
+      for (int e = 0; e < em4; e++)
+        for (int col = 0; col < L; col++)
+          quad[n1q_base[(size_t)c * nd + k] + ((int64_t)(e / 4) * L + col) * 4 + e % 4] =
+              e < em ? n1_pk[n1_base[(size_t)c * nd + k] + (int64_t)e * L + col] : pad;
+    }
+  }
   for (int gi = 0; gi < 3; gi++)
     for (int ti = 0; ti < 2; ti++) {
       const uint32_t G = 1u << gi, ts = ti == 0 ? 4u : 8u;
-      const std::vector<uint32_t>& ord = n1_pk;
-      std::vector<uint32_t> v(ord.size());
-      for (size_t i = 0; i < ord.size(); i++) {
-        const uint32_t pk = ord[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
+      std::vector<uint32_t> v(quad.size());
+      for (size_t i = 0; i < quad.size(); i++) {
+        const uint32_t pk = quad[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
         const uint32_t xo = sl * G * ts, wo = (2u * nl + sg) * G * 4u;
         if (xo >= (1u << 17) || wo >= (1u << 15)) throw std::runtime_error("neutron jump offsets do not pack");
         v[i] = xo | (wo << 17);
       }
       s->n1_pkv[gi][ti].upload(v);
     }
+  s->n1q_base.upload(n1q_base);
   s->W.upload(t.wpn);
   s->woff.upload(woff);
   s->wrows.upload(wrows);
@@ -2155,6 +2178,7 @@ void upload_tables(bs_system* s) {
   d.n1_pkr = s->n1_pkr.p;
   for (int gi = 0; gi < 3; gi++)
     for (int ti = 0; ti < 2; ti++) d.n1_pkv[gi][ti] = s->n1_pkv[gi][ti].p;
+  d.n1q_base = s->n1q_base.p;
   d.W = s->W.p;
   d.woff = s->woff.p;
   d.wrows = s->wrows.p;

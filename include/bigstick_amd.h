@@ -89,8 +89,10 @@ int bs_op_counts(const bs_system *sys, int64_t out[8]);
  * the reference's SD order), op (order 1: a*nsps + c for a+_a a_c; order 2:
  * the dense pair-pair index cpair*npair + dpair of bs_me_table for
  * a+_i a+_j a_l a_k) and the fermion phase.  Both directions are listed; the
- * reference stores the sector jumps with isector >= fsector and applies the
- * rest as their transposes.  Pass NULL arrays to get the count in *count;
+ * reference stores one triangle and applies the rest as transposes: proton
+ * one-body jumps with Jz(isd) >= Jz(fsd), neutron one-body jumps with
+ * Jz(isd) <= Jz(fsd), two-body jumps with isd >= fsd.  Pass NULL arrays to get
+ * the count in *count;
  * otherwise *count is the arrays' capacity on entry and the count on exit.
  * Replaces the jump lists of jumpmaster / weld (src/bjumplib_master.f90:
  * 342-431, src/bjumplib_weld.f90:32-818) as dumped by write_out_jumps
@@ -353,9 +355,14 @@ int bs_batch_set_variant(bs_batch *b, int dtype, int v);
 /* y_k = H_k x_k, interleaved device vectors. */
 int bs_batch_apply(bs_batch *b, const void *x, void *y, int dtype);
 /* Lanczos on all K Hamiltonians in lockstep (the recurrence of bs_lanczos,
- * thick must be 0), captured once into a HIP graph and replayed on later
- * calls with the same parameters; Ritz values and vectors by FP64 Sturm
- * bisection and inverse iteration on the device.  alpha/beta (may be NULL):
+ * plain or thick restart with thick <= 16 and niter <= 256; p->tol stops the
+ * thick run when no kept Ritz value of any Hamiltonian moves by more than tol
+ * over a restart), captured once into HIP graphs (first cycle, restarted
+ * cycle, restart, Ritz vectors + polish) and replayed on later calls with the
+ * same parameters; breakdown handled on the device per Hamiltonian as in
+ * bs_lanczos; Ritz values and vectors by FP64 Sturm bisection and inverse
+ * iteration (plain) or Jacobi on the arrowhead matrix (thick) on the
+ * device.  alpha/beta (may be NULL):
  * [k*min(niter,dim) + i]; evals[k*nkeep + n];
  * evecs (device, may be NULL): nkeep interleaved sets [n][i*K + k] in dtype;
  * resid[k*nkeep + n] (may be NULL; -1 without polish). */
