@@ -515,6 +515,19 @@ def main():
         torch.cuda.empty_cache()
     if a.solve:
         out["solve"] = solve_leg(s, case, a.max_restarts)
+        # the FP64 H x of the plain FP64 path (autotuned inside the solve leg)
+        xd = torch.randn(D, device=dev, dtype=torch.float64, generator=g)
+        yd = torch.empty_like(xd)
+        s.apply(xd, yd)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(3):
+            s.apply(xd, yd)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        out["solve"]["fp64_kernel_ms"] = ev0.elapsed_time(ev1) / 3
+        out["solve"]["fp32_kernel_ms"] = kern_ms
+        del xd, yd
     if a.batch:
         s.close()
         torch.cuda.empty_cache()
