@@ -1715,6 +1715,34 @@ void jacobi_eigen(int m, std::vector<double> A, std::vector<double>& e, std::vec
   Y.swap(Ys);
 }
 
+// Ritz pairs ordered as the reference's option tx orders them
+// (find_lanczos_eigenvalues, src/blanczoslib.f90:277-327, txchoice 1): by
+// distance to the target energy (selection sort), then the nkeep nearest by
+// energy.  e ascending on entry; Y column-major m x m, column k = pair k.
+void target_order(int m, int nkeep, double et, std::vector<double>& e, std::vector<double>& Y) {
+  auto swap_pair = [&](int a, int b) {
+    std::swap(e[a], e[b]);
+    for (int j = 0; j < m; j++) std::swap(Y[(size_t)j * m + a], Y[(size_t)j * m + b]);
+  };
+  std::vector<double> d(m);
+  for (int i = 0; i < m; i++) d[i] = std::fabs(e[i] - et);
+  for (int i = 0; i < m - 1; i++) {
+    int k = i;
+    for (int j = i + 1; j < m; j++)
+      if (d[j] < d[k]) k = j;
+    if (k != i) {
+      std::swap(d[k], d[i]);
+      swap_pair(k, i);
+    }
+  }
+  for (int i = 0; i < std::min(nkeep, m) - 1; i++) {
+    int k = i;
+    for (int j = i + 1; j < std::min(nkeep, m); j++)
+      if (e[j] < e[k]) k = j;
+    if (k != i) swap_pair(k, i);
+  }
+}
+
 // The nk lowest eigenvalues of the symmetric tridiagonal matrix with diagonal
 // a[0..n) and off-diagonal b[0..n-1) (b[j] couples j and j+1), by Sturm
 // sequence bisection (the count of negative pivots of T - xI is the number of
@@ -1785,6 +1813,7 @@ struct bs_system {
   int last_random_restarts = 0;  // breakdown handling of the last Lanczos run
   int last_second_passes = 0;
   int last_steps = 0;            // Lanczos steps of its last Krylov sequence
+  int last_thick_restarts = 0;   // thick restarts it took
   int cpt = 1;
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
   float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
@@ -2327,7 +2356,10 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (niter < nkeep) return set_err("niter must be >= nkeep");
   // a Krylov space as large as the basis is exact after one cycle: no restart
   const int thick = p->niter >= D ? 0 : p->thick;
-  if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter))) return set_err("thick must be 0 or in [nkeep, niter)");
+  if (thick < 0 || (thick > 0 && (thick < nkeep || thick > niter - 2)))
+    return set_err("thick must be 0 or in [nkeep, niter - 2]");
+  if (p->target < 0 || p->target > 2) return set_err("target must be 0 (lowest states), 1 or 2 (states nearest etarget)");
+  if (p->target && p->thick <= 0) return set_err("targeting an energy needs a thick-restart run (thick > 0)");
   const int mtot = niter + 1;
   const int nb = grid_for(D) < 1024 ? grid_for(D) : 1024;
   // Lanczos vectors at a stride of whole 64-element blocks: every v_i starts
@@ -2396,7 +2428,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (p->conv_test != 0 && p->conv_test != 1) return set_err("conv_test must be 0 (mean energy change) or 1 (largest)");
   if (p->ncheck_ex < 0) return set_err("ncheck_ex must be >= 0");
   const int itskip = std::max(10, (int)std::sqrt((double)nkeep));
-  const int ncheck = (int)std::min<int64_t>(D, (int64_t)nkeep + p->ncheck_ex);
+  // a targeted run tests only the nkeep kept energies (src/blanczosmain.f90:1497-1499)
+  const int ncheck = (int)std::min<int64_t>(D, (int64_t)nkeep + (p->target ? 0 : p->ncheck_ex));
   const int ntest = nkeep > 100 && thick == 0 ? (int)(std::sqrt((double)ncheck) + 1) : ncheck;
   // the test of convergence_test (src/blanczoslib.f90:2726-2770) between
   // two sets of Ritz values: test 0 sum_{j<nt} |dE_j| / sqrt(nt + 1) times
@@ -2465,41 +2498,67 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     }
   };
 
-  std::vector<double> alpha_h, beta_h;  // current arrowhead/tridiagonal data
-  std::vector<double> e_prev, e_check;
+  std::vector<double> alpha_h, beta_h;  // alpha/beta of the final projected matrix
   int total_mv = 0;
-  int kstart = 0;           // number of locked (thick-restart) vectors at the front
+  int kstart = 0;             // number of locked (thick-restart) vectors at the front
   std::vector<double> arrow;  // coupling of the locked vectors to v_kstart
   int restarts = 0;
   std::vector<double> e, Y;
-  while (true) {
-    for (int i = kstart; i < niter; i++) {
-      T* vi = V.p + (int64_t)i * ld;
-      launch_apply<T>(s, vi, w.p, 0, st);
-      total_mv++;
-      // alpha = <v_i, w>; w -= alpha v_i
-      k_multidot<T><<<nb, kBlock, 0, st>>>(vi, D, 1, w.p, D, part.p);
-      k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_alpha + i, 0);
-      k_sub_scaled<T><<<g, kBlock, 0, st>>>(w.p, vi, d_alpha + i, D);
-      // classical Gram-Schmidt against v_0 .. v_{i-1}
-      if (i > 0) {
-        k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, i, w.p, D, part.p);
-        k_reduce<<<i, kBlock, 0, st>>>(part.p, nb, i, dots.p, 0);
-        k_multiaxpy<T><<<g, kBlock, 0, st>>>(w.p, V.p, ld, i, dots.p, D);
-      }
-      k_multidot<T><<<nb, kBlock, 0, st>>>(w.p, D, 1, w.p, D, part.p);
-      k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
-      next_vector(i, kstart, arrow);
-      HIP_OK(hipGetLastError());
-      // plain runs with a tolerance (options ld, lc): the reference's
-      // schedule and test (lanczos_p, src/blanczosmain.f90:1489-1512,
-      // :1690-1724; convergence_test, src/blanczoslib.f90:2660-2800): at
-      // every itskipeig-th step (10, or sqrt(nkeep) if larger) beyond ncheck =
-      // nkeep + ncheck_ex, the lowest ncheck Ritz values of the tridiagonal so
-      // far; stop when, against the previous such check, the mean change of
-      // the first ntest of them (sum |dE| / sqrt(ntest + 1), test 0) or their
-      // largest change (test 1) is below tol
-      if (thick == 0 && p->tol > 0.0) {
+  // one Lanczos step on v_i: w = H v_i, alpha_i, classical Gram-Schmidt
+  // against v_0 .. v_{i-1}, beta_i, v_{i+1}
+  auto step = [&](int i) {
+    T* vi = V.p + (int64_t)i * ld;
+    launch_apply<T>(s, vi, w.p, 0, st);
+    total_mv++;
+    k_multidot<T><<<nb, kBlock, 0, st>>>(vi, D, 1, w.p, D, part.p);
+    k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_alpha + i, 0);
+    k_sub_scaled<T><<<g, kBlock, 0, st>>>(w.p, vi, d_alpha + i, D);
+    if (i > 0) {
+      k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, i, w.p, D, part.p);
+      k_reduce<<<i, kBlock, 0, st>>>(part.p, nb, i, dots.p, 0);
+      k_multiaxpy<T><<<g, kBlock, 0, st>>>(w.p, V.p, ld, i, dots.p, D);
+    }
+    k_multidot<T><<<nb, kBlock, 0, st>>>(w.p, D, 1, w.p, D, part.p);
+    k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
+    next_vector(i, kstart, arrow);
+    HIP_OK(hipGetLastError());
+  };
+  // Ritz pairs of the projected matrix on v_0 .. v_{m-1}: the locked Ritz
+  // values on the diagonal with the arrow couplings in column kstart
+  // (src/blanczoslib.f90:143-222), then the tridiagonal part; ordered by
+  // distance to the target when one is set (find_lanczos_eigenvalues)
+  auto ritz = [&](int m) {
+    alpha_h.assign(m, 0.0);
+    beta_h.assign(m, 0.0);
+    HIP_OK(hipMemcpy(alpha_h.data(), d_alpha, sizeof(double) * m, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(beta_h.data(), d_beta, sizeof(double) * m, hipMemcpyDeviceToHost));
+    std::vector<double> A((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) A[(size_t)i * m + i] = alpha_h[i];
+    for (int i = kstart; i + 1 < m; i++) {
+      A[(size_t)i * m + i + 1] = beta_h[i];
+      A[(size_t)(i + 1) * m + i] = beta_h[i];
+    }
+    for (int i = 0; i < kstart && kstart < m; i++) {
+      A[(size_t)i * m + kstart] = arrow[i];
+      A[(size_t)kstart * m + i] = arrow[i];
+    }
+    jacobi_eigen(m, A, e, Y);
+    if (p->target) target_order(m, nkeep, p->etarget, e, Y);
+  };
+  int m = niter;  // size of the final projected matrix
+  if (thick == 0) {
+    // plain runs with a tolerance (options ld, lc): the reference's schedule
+    // and test (lanczos_p, src/blanczosmain.f90:1489-1512, :1690-1724;
+    // convergence_test, src/blanczoslib.f90:2660-2800): at every
+    // itskipeig-th step (10, or sqrt(nkeep) if larger) beyond ncheck = nkeep
+    // + ncheck_ex, the lowest ncheck Ritz values of the tridiagonal so far;
+    // stop when, against the previous such check, the mean change of the
+    // first ntest of them (sum |dE| / sqrt(ntest + 1), test 0) or their
+    // largest change (test 1) is below tol
+    std::vector<double> e_check;
+    for (int i = 0; i < niter; i++) {
+      step(i);
+      if (p->tol > 0.0) {
         const int iter = i + 1;
         if (iter % itskip == 0 && iter > ncheck && iter < niter) {
           std::vector<double> a(iter), b(iter);
@@ -2515,77 +2574,92 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
         }
       }
     }
-    alpha_h.assign(mtot, 0.0);
-    beta_h.assign(mtot, 0.0);
-    HIP_OK(hipMemcpy(alpha_h.data(), d_alpha, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(beta_h.data(), d_beta, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    // projected matrix: locked Ritz values on the diagonal with the arrow
-    // couplings in column kstart (src/blanczoslib.f90:143-222), then the
-    // tridiagonal part
-    const int m = niter;
-    std::vector<double> A((size_t)m * m, 0.0);
-    for (int i = 0; i < m; i++) A[(size_t)i * m + i] = alpha_h[i];
-    for (int i = kstart; i + 1 < m; i++) {
-      A[(size_t)i * m + i + 1] = beta_h[i];
-      A[(size_t)(i + 1) * m + i] = beta_h[i];
-    }
-    for (int i = 0; i < kstart; i++) {
-      A[(size_t)i * m + i] = alpha_h[i];
-      A[(size_t)i * m + kstart] = arrow[i];
-      A[(size_t)kstart * m + i] = arrow[i];
-    }
-    jacobi_eigen(m, A, e, Y);
-    bool done = (thick == 0) || restarts >= p->max_restarts;
-    // thick restart: the same measure over ncheck values, checked at each
-    // restart; the mean (test 0) carries the reference's factor (1 + number
-    // of restarts) (src/blanczoslib.f90:2735)
-    const int nc_thick = std::min(ncheck, m);
-    if (!done && p->tol > 0.0 && !e_prev.empty() && energy_change(e_prev, e, std::min(ntest, nc_thick), restarts) < p->tol)
-      done = true;
-    e_prev.assign(e.begin(), e.begin() + nc_thick);
-    if (done) break;
-    // thick restart (src/blanczoslib.f90:1656-1799): keep `thick` Ritz
-    // vectors, V[0:thick] = V[0:m] Y[:, 0:thick]; couplings beta_m * Y[m-1, k]
-    {
+    m = niter;
+    ritz(m);
+  } else {
+    // thick restart (options td, tf, tc, tx), the reference's cycle
+    // (lanczos_p, src/blanczosmain.f90:1690-1752; thick_restart_sub_p,
+    // src/blanczoslib.f90:1656-1799): steps run to iter = niter - 1; the
+    // restart keeps the `thick` leading Ritz vectors of the projected matrix
+    // on the first niter - 2 vectors, couples them to v_{niter-1} (the next
+    // start vector) by beta_{niter-2} Y[niter-2, k], and iter resumes at
+    // thick.  The energies are taken at every itskipeig-th iter beyond
+    // max(ncheck, nkeep) and at iter = niter - 1; the test (convergence_test)
+    // compares them at every itskipeig-th iter beyond ncheck with the
+    // previous iteration's, the mean change carrying the factor (1 + number
+    // of restarts).  The run ends converged, at iter = niter (restart length
+    // thick = niter - 1), or when a restart is due after max_restarts + 1
+    // restarts.  A targeted run of the reference (tx) sets no count of tested
+    // energies (ntest_converge keeps its initial 0, src/blanczosmain.f90:
+    // 1497-1499), so its test passes at the first check; target = 2 tests
+    // the nkeep kept energies instead.
+    const int ntest_t = p->target == 1 ? 0 : ncheck;
+    std::vector<double> ecur(ncheck, 0.0), eold(ncheck, 0.0);
+    int iter = 0;
+    bool finished = false;
+    while (!finished) {
+      iter++;
+      step(iter - 1);
+      const bool fly = iter % itskip == 0;
+      if ((fly && iter > std::max(ncheck, nkeep)) || iter + 1 == niter) {
+        ritz(iter);
+        for (int j = 0; j < ncheck; j++) ecur[j] = j < iter ? e[j] : 0.0;
+      }
+      if (iter == niter) {
+        finished = true;
+      } else if (p->tol > 0.0) {
+        if (iter > ncheck && fly && energy_change(eold, ecur, ntest_t, restarts) < p->tol) finished = true;
+        eold = ecur;
+      }
+      if (finished || iter + 1 != niter) continue;
+      if (restarts > p->max_restarts) break;
+      const int mr = iter - 1;
+      ritz(mr);
+      for (int j = 0; j < ncheck; j++) ecur[j] = j < mr ? e[j] : 0.0;
       DevBuf<double> dY;
-      std::vector<double> Yk((size_t)m * thick);
-      for (int j = 0; j < m; j++)
-        for (int k = 0; k < thick; k++) Yk[(size_t)j * thick + k] = Y[(size_t)j * m + k];
+      std::vector<double> Yk((size_t)mr * thick);
+      for (int j = 0; j < mr; j++)
+        for (int k = 0; k < thick; k++) Yk[(size_t)j * thick + k] = Y[(size_t)j * mr + k];
       dY.upload(Yk);
       DevBuf<T> tmp;
       tmp.alloc((size_t)thick * ld);
       for (int k0 = 0; k0 < thick; k0 += 16) {
         int nk = std::min(16, thick - k0);
-        k_combine_mfma<T, T><<<mfma_grid(D), 256, 0, st>>>(tmp.p + (int64_t)k0 * ld, ld, V.p, ld, m, dY.p + k0,
+        k_combine_mfma<T, T><<<mfma_grid(D), 256, 0, st>>>(tmp.p + (int64_t)k0 * ld, ld, V.p, ld, mr, dY.p + k0,
                                                             thick, nk, D);
       }
       HIP_OK(hipMemcpyAsync(V.p, tmp.p, sizeof(T) * (size_t)thick * ld, hipMemcpyDeviceToDevice, st));
-      // the last Krylov vector becomes v_thick
-      HIP_OK(hipMemcpyAsync(V.p + (int64_t)thick * ld, V.p + (int64_t)m * ld, sizeof(T) * D,
-                            hipMemcpyDeviceToDevice, st));
+      if (mr != thick)
+        HIP_OK(hipMemcpyAsync(V.p + (int64_t)thick * ld, V.p + (int64_t)mr * ld, sizeof(T) * D,
+                              hipMemcpyDeviceToDevice, st));
       arrow.assign(thick, 0.0);
       std::vector<double> anew(thick);
       for (int k = 0; k < thick; k++) {
-        arrow[k] = beta_h[m - 1] * Y[(size_t)(m - 1) * m + k];
+        arrow[k] = beta_h[mr - 1] * Y[(size_t)(mr - 1) * mr + k];
         anew[k] = e[k];
       }
+      // the kept Ritz values and their couplings take the places of the
+      // first thick alphas and betas, as in the reference's coefficients
       HIP_OK(hipMemcpy(d_alpha, anew.data(), sizeof(double) * thick, hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(d_beta, arrow.data(), sizeof(double) * thick, hipMemcpyHostToDevice));
       HIP_OK(hipStreamSynchronize(st));
+      kstart = thick;
+      restarts++;
+      iter = thick;
     }
-    kstart = thick;
-    restarts++;
+    m = iter;
+    ritz(m);
   }
   for (int k = 0; k < nkeep; k++) evals[k] = e[k];
   // (src/blanczoslib.f90:1122-1136) E_k and eiglvec(1,k)^2 dnorm0^2
-  for (int k = 0; k < niter; k++) {
+  for (int k = 0; k < m; k++) {
     if (pr.ritz) pr.ritz[k] = e[k];
     if (pr.strength) pr.strength[k] = Y[k] * Y[k] * pnorm * pnorm;
   }
-  if (alpha_out) for (int i = 0; i < niter; i++) alpha_out[i] = alpha_h[i];
-  if (beta_out) for (int i = 0; i < niter; i++) beta_out[i] = beta_h[i];
+  if (alpha_out) for (int i = 0; i < m; i++) alpha_out[i] = alpha_h[i];
+  if (beta_out) for (int i = 0; i < m; i++) beta_out[i] = beta_h[i];
   if (evecs || resid || p->polish) {
     // Ritz vectors in FP64: x_k = sum_j Y[j,k] v_j
-    const int m = niter;
     DevBuf<double> dY, X64, HX, tmpd;
     std::vector<double> Yk((size_t)m * nkeep);
     for (int j = 0; j < m; j++)
@@ -2630,7 +2704,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   if (iters) *iters = total_mv;
   s->last_random_restarts = random_restarts;
   s->last_second_passes = second_passes;
-  s->last_steps = niter;
+  s->last_steps = m;
+  s->last_thick_restarts = restarts;
   return 0;
 }
 
@@ -3546,6 +3621,7 @@ void run_batch_lanczos(bs_batch* b, const bs_lanczos_params* p, double* alpha, d
   const int thick = p->niter >= D ? 0 : p->thick;
   if (nkeep < 1 || nkeep > 16) throw std::runtime_error("nkeep must be in 1..16");
   if (niter < nkeep) throw std::runtime_error("niter must be >= nkeep");
+  if (p->target) throw std::runtime_error("the batched run keeps the lowest states (target must be 0)");
   if (thick < 0 || (thick > 0 && (thick < nkeep || thick >= niter || thick > 16)))
     throw std::runtime_error("thick must be 0 or in [nkeep, min(niter, 17))");
   if (thick > 0 && niter > 256) throw std::runtime_error("thick restart takes niter <= 256");
@@ -4548,8 +4624,10 @@ int bs_lanczos_noalloc(bs_system* s, const bs_lanczos_params* p, double* alpha, 
   }
 }
 
-int bs_lanczos_stats(const bs_system* s, int* random_restarts, int* second_passes, int* steps) {
+int bs_lanczos_stats(const bs_system* s, int* random_restarts, int* second_passes, int* steps,
+                     int* thick_restarts) {
   if (!s) return set_err("null system");
+  if (thick_restarts) *thick_restarts = s->last_thick_restarts;
   if (random_restarts) *random_restarts = s->last_random_restarts;
   if (second_passes) *second_passes = s->last_second_passes;
   if (steps) *steps = s->last_steps;

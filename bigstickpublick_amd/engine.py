@@ -18,7 +18,8 @@ _lib = None
 class _Params(ctypes.Structure):
     _fields_ = [("nkeep", ctypes.c_int), ("niter", ctypes.c_int), ("thick", ctypes.c_int),
                 ("max_restarts", ctypes.c_int), ("dtype", ctypes.c_int), ("tol", ctypes.c_double),
-                ("polish", ctypes.c_int), ("conv_test", ctypes.c_int), ("ncheck_ex", ctypes.c_int)]
+                ("polish", ctypes.c_int), ("conv_test", ctypes.c_int), ("ncheck_ex", ctypes.c_int),
+                ("target", ctypes.c_int), ("etarget", ctypes.c_double)]
 
 
 def load_library():
@@ -60,7 +61,7 @@ def load_library():
     L.bs_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp,
                              ctypes.POINTER(i32)]
     L.bs_lanczos_noalloc.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, ctypes.POINTER(i32)]
-    L.bs_lanczos_stats.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.bs_lanczos_stats.argtypes = [vp] + [ctypes.POINTER(i32)] * 4
     L.bs_block_lanczos.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_block_strength.argtypes = [vp, ctypes.POINTER(_Params), i32, vp, i32, vp, vp, vp, vp, vp,
                                     ctypes.POINTER(i32)]
@@ -349,15 +350,23 @@ class System:
         return out
 
     def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
-                want_vectors=False, conv_test=1, ncheck_ex=0):
+                want_vectors=False, conv_test=1, ncheck_ex=0, target=None, target_test="reference"):
         """Lanczos from the uniform pivot.  With tol > 0 the run stops by the
         reference's energy test conv_test (0: mean change, 1: largest change)
         over nkeep + ncheck_ex Ritz values (the reference's ld is conv_test=0,
-        ncheck_ex=5, tol=1e-3)."""
+        ncheck_ex=5, tol=1e-3).  target (MeV, thick > 0): keep the Ritz
+        pairs nearest that energy instead of the lowest (option tx);
+        target_test "reference" tests convergence as the reference's tx does
+        (over no energies: the first check stops the run), "kept" over the
+        nkeep kept energies."""
         import torch
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
-        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish), conv_test, ncheck_ex)
+        if target_test not in ("reference", "kept"):
+            raise ValueError("target_test must be 'reference' or 'kept'")
+        mode = 0 if target is None else (1 if target_test == "reference" else 2)
+        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish), conv_test, ncheck_ex,
+                    mode, 0.0 if target is None else float(target))
         m = min(niter, self.dim)
         alpha = np.zeros(m)
         beta = np.zeros(m)
@@ -377,10 +386,12 @@ class System:
 
     def lanczos_stats(self):
         """(random restarts, second Gram-Schmidt passes, Lanczos steps of the
-        last Krylov sequence) of the last lanczos() or strength() run."""
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(load_library().bs_lanczos_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        last Krylov sequence, thick restarts) of the last lanczos() or
+        strength() run."""
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load_library().bs_lanczos_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                               ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def exact(self, nkeep, dtype="f64", want_vectors=False):
         """Exact diagonalization of the dense H (dim <= 65536): the nkeep

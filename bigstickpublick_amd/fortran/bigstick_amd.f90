@@ -12,6 +12,8 @@ module bigstick_amd
     real(c_double) :: tol
     integer(c_int) :: polish
     integer(c_int) :: conv_test = 0, ncheck_ex = 0
+    integer(c_int) :: target = 0
+    real(c_double) :: etarget = 0.0d0
   end type bs_lanczos_params
 
   interface
@@ -104,11 +106,11 @@ module bigstick_amd
       integer(c_int) :: rc
     end function bs_lanczos
 
-    function bs_lanczos_stats(sys, random_restarts, second_passes, steps) bind(C, name='bs_lanczos_stats') &
-        result(rc)
+    function bs_lanczos_stats(sys, random_restarts, second_passes, steps, thick_restarts) &
+        bind(C, name='bs_lanczos_stats') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: sys
-      integer(c_int), intent(out) :: random_restarts, second_passes, steps
+      integer(c_int), intent(out) :: random_restarts, second_passes, steps, thick_restarts
       integer(c_int) :: rc
     end function bs_lanczos_stats
 

@@ -59,10 +59,10 @@ contains
   subroutine diagonalize_run(write_wfn)
     logical, intent(in) :: write_wfn
     character(len=256) :: outname, spsname, intname, line, lopt
-    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, ios, k, iu, dimblock, nbiter, ncheck, ctest
-    integer(c_int) :: nrr, nsp, steps
+    integer :: z, n, jz2, nkeep, niter, nthick, maxrest, k, iu, dimblock, nbiter, ncheck, ctest, tgt
+    integer(c_int) :: nrr, nsp, steps, nthr
     integer(8) :: t0, t1, t2, trate
-    real(c_double) :: ctol
+    real(c_double) :: ctol, etarget
     real(c_float) :: spscale, a0, a, x
     type(c_ptr) :: sys, dvec
     integer(c_int64_t) :: dim, nbytes, esize
@@ -101,10 +101,10 @@ contains
     read(5, '(a)') lopt
     lopt = lower(adjustl(lopt))
     select case (lopt(1:2))
-    case ('ex', 'ld', 'lf', 'lc', 'td', 'tf', 'tc', 'bd', 'bf', 'bc', 'sk', 'li')
+    case ('ex', 'ld', 'lf', 'lc', 'td', 'tf', 'tc', 'tx', 'bd', 'bf', 'bc', 'sk', 'li')
     case default
       write(6, *) ' bs_driver: diagonalization option ', trim(lopt), ' is not supported', &
-                  ' (supported: ex, ld, lf, lc, td, tf, tc, bd, bf, bc, sk, li)'
+                  ' (supported: ex, ld, lf, lc, td, tf, tc, tx, bd, bf, bc, sk, li)'
       stop 1
     end select
     dimblock = 0
@@ -151,10 +151,32 @@ contains
     end if
     nthick = 0
     maxrest = 0
-    if (lopt(1:1) == 't') then
+    tgt = 0
+    etarget = 0.0d0
+    if (lopt(1:2) == 'tx') then
+      ! tx: thick restart keeping the states nearest a target energy
+      ! (src/blanczosmain.f90:1022-1053): niter raised to at least
+      ! max((nkeep + 5) * 2, 10), kept vectors at least nkeep, restarts,
+      ! target energy; test 0 with tolerance 1e-3 over the nkeep energies
+      niter = max(niter, max((nkeep + 5) * 2, 10))
       read(5, *) nthick
-      read(5, *, iostat=ios) maxrest
-      if (ios /= 0) maxrest = 20
+      if (nthick < nkeep) nthick = nkeep
+      read(5, *) maxrest
+      read(5, *) etarget
+      tgt = 1
+    else if (lopt(1:2) == 'td') then
+      ! td: no count of kept vectors is asked (src/blanczosmain.f90:993-1019):
+      ! niter raised to at least max((nkeep + 5) * 2, 10), max(nkeep + 5,
+      ! 3 nkeep) vectors kept but at most niter - 5; then max restarts
+      niter = max(niter, max((nkeep + 5) * 2, 10))
+      nthick = min(max(nkeep + 5, 3 * nkeep), niter - 5)
+      read(5, *) maxrest
+    else if (lopt(1:1) == 't') then
+      ! tf, tc: kept vectors (tf: at least nkeep), restarts
+      ! (src/blanczosmain.f90:1091-1145)
+      read(5, *) nthick
+      if (lopt(1:2) == 'tf' .and. nthick < nkeep) nthick = nkeep
+      read(5, *) maxrest
     end if
     ! default convergence (ld, td): the reference's test 0 with tolerance
     ! converge_tol0_def = 1e-3 over nkeep + ncheck_ex energies, ncheck_ex =
@@ -201,7 +223,9 @@ contains
     p%max_restarts = maxrest
     p%dtype = dtype
     p%tol = 0.0d0
-    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'C') p%tol = ctol
+    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'x') p%tol = ctol
+    p%target = tgt
+    p%etarget = etarget
     p%conv_test = ctest
     p%ncheck_ex = ncheck
     if (lopt(1:1) == 'b') p%ncheck_ex = 0
@@ -225,25 +249,43 @@ contains
       rc = bs_lanczos(sys, p, c_loc(alpha), c_loc(beta), c_loc(evals), dvec, c_loc(resid), iters)
       call bs_check(rc, 'bs_lanczos')
     end if
+    steps = 0
+    nthr = 0
+    if (dimblock == 0 .and. lopt(1:2) /= 'ex') then
+      rc = bs_lanczos_stats(sys, nrr, nsp, steps, nthr)
+      call bs_check(rc, 'bs_lanczos_stats')
+    end if
 
     open(newunit=iu, file=trim(first_word(outname)) // '.res', status='replace')
     write(iu, '(a)') '  bigstick_amd MI355X engine (BIGSTICK v8.0.0 input surface)'
     write(iu, '(a,a)') '  single-particle file = ', trim(first_word(spsname))
     write(iu, '(1x,i0,1x,i0)') z, n
     write(iu, '(1x,i0,a)') jz2, ' +'
-    write(iu, '(1x,i0,a)') iters, '  iterations '
-    write(iu, '(a)') '  '
+    if (steps > 0) then
+      ! the size of the final projected matrix and, after thick restarts, the
+      ! reference's total iter + niter * restarts (src/blanczosmain.f90:1762-1770)
+      write(iu, '(1x,i0,a)') steps, '  iterations '
+      write(iu, '(a)') '  '
+      if (nthr > 0) write(iu, '(a,i0,a)') '  ( total =  ', steps + niter * nthr, '  ) '
+    else
+      write(iu, '(1x,i0,a)') iters, '  iterations '
+      write(iu, '(a)') '  '
+    end if
     if (lopt(1:2) /= 'li') &
       call state_table(iu, sys, dvec, nkeep, evals, dtype, dim, write_wfn, trim(first_word(outname)) // '.wfn')
     close(iu)
     if (dimblock == 0 .and. lopt(1:2) /= 'ex') then
       ! a plain run that converged stopped after `steps` iterations, where
-      ! the reference's .lcoef ends too
-      rc = bs_lanczos_stats(sys, nrr, nsp, steps)
-      call bs_check(rc, 'bs_lanczos_stats')
+      ! the reference's .lcoef ends too; after a thick restart the file holds
+      ! the kept Ritz values and their couplings as lines -1 .. -nthick, then
+      ! the steps of the last cycle (src/blanczoslib.f90:1750-1757)
       open(newunit=iu, file=trim(first_word(outname)) // '.lcoef', status='replace')
       do k = 1, min(niter, int(steps))
-        write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
+        if (nthr > 0 .and. k <= nthick) then
+          write(iu, '(i10,2(2x,f20.15))') -k, alpha(k), beta(k)
+        else
+          write(iu, '(i10,2(2x,f20.15))') k, alpha(k), beta(k)
+        end if
       end do
       close(iu)
     end if

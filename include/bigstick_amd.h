@@ -173,6 +173,14 @@ typedef struct {
                        beyond ncheck, as lanczos_p does (src/blanczosmain.f90:1690-1724); a thick
                        run checks at each restart. */
   int ncheck_ex;    /* additional Ritz values in the test (the reference's ld uses 5) */
+  int target;       /* 0: the lowest states.  1, 2: the Ritz pairs nearest etarget (option tx,
+                       txchoice 1, src/blanczoslib.f90:277-327; thick > 0): every Ritz solve orders
+                       them by |E - etarget|, the thick restart keeps the thick nearest, evals
+                       receives the nkeep nearest in ascending energy.  1 tests convergence as the
+                       reference's tx does, over no energies (its count is never set on that path,
+                       src/blanczosmain.f90:1497-1499), so the run stops at the first check;
+                       2 tests the nkeep kept energies */
+  double etarget;   /* target energy (MeV) when target != 0 */
 } bs_lanczos_params;
 
 /* Lanczos from the reference's uniform pivot 1/sqrt(D)
@@ -210,8 +218,12 @@ int bs_lanczos_noalloc(bs_system *sys, const bs_lanczos_params *p, double *alpha
  * (br_check_orthogonal, src/breorthog.f90:977-1045) and was
  * re-orthogonalized; steps receives the number of Lanczos steps of the last
  * Krylov sequence (where a converged plain run stopped: the length of the
- * reference's .lcoef).  Any pointer may be NULL. */
-int bs_lanczos_stats(const bs_system *sys, int *random_restarts, int *second_passes, int *steps);
+ * reference's .lcoef; for a thick-restart run the size of the final projected
+ * matrix, the reference's "iterations"), thick_restarts the number of thick
+ * restarts taken (the reference's total = steps + niter * thick_restarts).
+ * Any pointer may be NULL. */
+int bs_lanczos_stats(const bs_system *sys, int *random_restarts, int *second_passes, int *steps,
+                     int *thick_restarts);
 
 /* Exact diagonalization: the dense H (dim <= 65536, dim^2 doubles in HBM)
  * built 64 columns at a time with the multi-vector kernel in FP64, all

@@ -64,3 +64,132 @@ def convergence_stop(alpha, beta, nkeep, ncheck_ex=5, tol=1e-3, test=0, niter=No
                 return it
         eold = e
     return None
+
+
+def target_order(e, Y, nkeep, etarget):
+    """Ritz pairs as the reference's option tx orders them (txchoice 1 of
+    find_lanczos_eigenvalues, src/blanczoslib.f90:277-327): a selection sort
+    by |E - etarget| (first minimum kept on ties), then a selection sort of
+    the first nkeep by energy.  e ascending; Y columns are the pairs."""
+    e = e.copy()
+    Y = Y.copy()
+    d = np.abs(e - etarget)
+    n = len(e)
+    for i in range(n - 1):
+        k = i + int(np.argmin(d[i:]))
+        if k != i:
+            d[[i, k]] = d[[k, i]]
+            e[[i, k]] = e[[k, i]]
+            Y[:, [i, k]] = Y[:, [k, i]]
+    m = min(nkeep, n)
+    for i in range(m - 1):
+        k = i + int(np.argmin(e[i:m]))
+        if k != i:
+            e[[i, k]] = e[[k, i]]
+            Y[:, [i, k]] = Y[:, [k, i]]
+    return e, Y
+
+
+def thick_lanczos(apply, dim, nkeep, niter, nthick, max_restarts, tol=0.0, test=0, ncheck_ex=0,
+                  etarget=None, target_test="reference"):
+    """Thick-restart Lanczos as the reference runs it (options td, tf, tc,
+    tx: lanczos_p, src/blanczosmain.f90:1582-1752; thick_restart_sub_p,
+    src/blanczoslib.f90:1656-1799; convergence_test, :2660-2800), in FP64,
+    from the uniform pivot, without the breakdown restarts.
+
+    iter counts the vectors of the current projected matrix.  Each step
+    applies H to v_iter, takes alpha, orthogonalizes against every earlier
+    vector and normalizes.  The Ritz values are taken when iter is a
+    multiple of itskipeig = max(10, int(sqrt(nkeep))) beyond max(ncheck,
+    nkeep), and at iter = niter - 1; at every multiple beyond ncheck (ncheck
+    = nkeep + ncheck_ex, nkeep for a targeted run) the test compares them
+    with the previous iteration's: test 0 sum |dE| / sqrt(ntest + 1) times
+    (1 + restarts), test 1 max |dE|, over the first ntest values.  At iter =
+    niter - 1 a run that has not converged restarts -- after max_restarts + 1
+    restarts it stops instead: the matrix on the first niter - 2 vectors is
+    diagonalized, its nthick leading Ritz vectors replace v_1 .. v_nthick,
+    couple to v_{niter-1} (the next v_{nthick+1}) by beta_{niter-2}
+    Y[niter-2, k], and iter resumes at nthick.  The projected matrix holds the
+    kept Ritz values on its diagonal with those couplings in column nthick+1,
+    then the tridiagonal part.  With etarget every Ritz solve is ordered by
+    target_order; the reference's tx tests no energies (its count of tested
+    energies is never set on that path, src/blanczosmain.f90:1497-1499),
+    target_test="kept" tests the nkeep kept ones.
+
+    Returns a dict: evals (nkeep), alpha and beta of the final matrix (the
+    reference's .lcoef: the kept values and couplings, then the last
+    cycle's steps), steps (its size), restarts, matvecs."""
+    itskip = max(10, int(np.sqrt(nkeep)))
+    ncheck = min(dim, nkeep + (0 if etarget is not None else ncheck_ex))
+    ntest = 0 if (etarget is not None and target_test == "reference") else ncheck
+    V = np.zeros((niter + 1, dim))
+    V[0] = 1.0 / np.sqrt(dim)
+    alpha = np.zeros(niter + 1)
+    beta = np.zeros(niter + 1)
+    kstart = 0
+    restarts = 0
+    matvecs = 0
+
+    def ritz(m):
+        A = np.diag(alpha[:m])
+        for i in range(kstart, m - 1):
+            A[i, i + 1] = A[i + 1, i] = beta[i]
+        if kstart < m:
+            A[:kstart, kstart] = beta[:kstart]
+            A[kstart, :kstart] = beta[:kstart]
+        e, Y = np.linalg.eigh(A)
+        if etarget is not None:
+            e, Y = target_order(e, Y, nkeep, etarget)
+        return e, Y
+
+    def change(e0, e1):
+        d = np.abs(e0[:ntest] - e1[:ntest])
+        if ntest == 0:
+            return 0.0
+        return d.max() if test == 1 else d.sum() / np.sqrt(ntest + 1) * (1 + restarts)
+
+    ecur = np.zeros(ncheck)
+    eold = np.zeros(ncheck)
+    it = 0
+    finished = False
+    while not finished:
+        it += 1
+        i = it - 1
+        w = apply(V[i])
+        matvecs += 1
+        a = float(V[i] @ w)
+        w = w - a * V[i]
+        d = V[:i] @ w
+        w = w - d @ V[:i]
+        b = float(np.linalg.norm(w))
+        alpha[i] = a
+        beta[i] = b
+        V[i + 1] = w / b
+        fly = it % itskip == 0
+        if (fly and it > max(ncheck, nkeep)) or it + 1 == niter:
+            e, _ = ritz(it)
+            ecur = np.array([e[j] if j < it else 0.0 for j in range(ncheck)])
+        if it == niter:
+            finished = True
+        elif tol > 0.0:
+            if it > ncheck and fly and change(eold, ecur) < tol:
+                finished = True
+            eold = ecur.copy()
+        if finished or it + 1 != niter:
+            continue
+        if restarts > max_restarts:
+            break
+        m = it - 1
+        e, Y = ritz(m)
+        ecur = np.array([e[j] if j < m else 0.0 for j in range(ncheck)])
+        keep = Y[:, :nthick].T @ V[:m]
+        V[nthick] = V[m].copy()
+        V[:nthick] = keep
+        beta[:nthick] = beta[m - 1] * Y[m - 1, :nthick]
+        alpha[:nthick] = e[:nthick]
+        kstart = nthick
+        restarts += 1
+        it = nthick
+    e, Y = ritz(it)
+    return {"evals": e[:nkeep], "alpha": alpha[:it].copy(), "beta": beta[:it].copy(), "steps": it,
+            "restarts": restarts, "matvecs": matvecs}

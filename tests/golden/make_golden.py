@@ -497,6 +497,39 @@ def default_convergence():
             shutil.rmtree(wd)
 
 
+THICK = [("pf2p1n_m3", "tf", ["4 24", "10", "3"]),
+         ("pf2p1n_m3", "tc", ["4 30", "12", "40", "0", "1e-4"]),
+         ("sd1p2n_m1", "td", ["4 20", "40"]),
+         ("sd1p2n_m1", "tx", ["4 30", "12", "200", "-16.2"]),
+         ("pf2p1n_m3", "tx", ["4 30", "12", "200", "-26.5"])]
+
+
+def thick_cycles():
+    """The reference's FP64 build on its thick-restart options
+    (src/blanczosmain.f90:993-1053, :1091-1145): tf (fixed restarts), tc
+    (test 0 at 1e-4), td (defaults: nthick max(nkeep + 5, 3 nkeep) capped at
+    niter - 5, test 0 at 1e-3) and tx (states nearest a target energy) on
+    pf 2p1n (2Jz = 3) and sd 1p2n (2Jz = 1), whose low states all overlap the
+    uniform pivot: .res and .lcoef (the last cycle's coefficients after the
+    kept Ritz values and couplings) kept as <name>_<option>_fp64.*."""
+    for name, option, answers in THICK:
+        meta = [c for c in SMALL if c[0] == name][0]
+        case = small_case(*meta[1:5], seed=meta[5])
+        case.name = name
+        wd = tempfile.mkdtemp(prefix="bsgold_")
+        try:
+            write_case(case, wd)
+            out = "%s_%s_fp64" % (name, option)
+            script = "\n".join(["n", out, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), case.name,
+                                "1 1 1 1", "end", option] + answers) + "\n"
+            run_ref(wd, script, exe=REF64)
+            for ext in (".res", ".lcoef"):
+                shutil.copy(os.path.join(wd, out + ext), os.path.join(HERE, out + ext))
+            print(out, read_res(os.path.join(wd, out + ".res"))[:, 0])
+        finally:
+            shutil.rmtree(wd)
+
+
 def config4_coefficients(niter=4):
     """pf 6p6n M=0 (D = 109,954,620): the reference's option li
     (src/blanczosmain.f90:787-795) for `niter` Lanczos steps from its uniform
@@ -738,6 +771,8 @@ if __name__ == "__main__":
     for c in CONVERGE:
         if "converge" in want:
             converge(*c)
+    if "thick" in want:
+        thick_cycles()
     if "np" in want:
         pivot_spectrum()
     if "li" in want:

@@ -732,8 +732,8 @@ def test_fortran_host_pivot_spectrum_matches_reference(casedir):
 
 @needs_driver
 def test_fortran_host_refuses_unsupported_option(casedir):
-    """A diagonalization option the driver does not implement (tx: thick
-    restart near a target energy) is refused with a message, not misread;
+    """A diagonalization option the driver does not implement (tb: thick-
+    restart block Lanczos) is refused with a message, not misread;
     upper-case spellings of supported ones pass that check."""
     from bigstickpublick_amd.inputs import small_case, write_case
     case = small_case("sd", 2, 2, 0, seed=77)
@@ -741,7 +741,7 @@ def test_fortran_host_refuses_unsupported_option(casedir):
     wd = os.path.join(casedir, "fopt_refuse")
     write_case(case, wd)
     head = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end"]
-    r = _run("\n".join(head + ["tx", "5 30"]) + "\n", wd)
+    r = _run("\n".join(head + ["tb", "5 30"]) + "\n", wd)
     assert r.returncode != 0 and "is not supported" in r.stdout, r.stdout + r.stderr
     r = _run("\n".join(head + ["LC", "5 300", "2", "3", "1e-6"]) + "\n", wd)
     assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
@@ -763,3 +763,46 @@ def test_fortran_host_block_strength_fp64(casedir):
         gt, gr = got[i]
         assert abs(gt - pn2[n]) < 1e-9 * max(1.0, pn2[n])
         assert np.abs(gr[:, 0] - e).max() < 2e-6 and np.abs(gr[:, 1] - st[n]).max() < 2e-6, (gr, e, st[n])
+
+
+THICK_MENUS = [("pf2p1n_m3", "tf", ["4 24", "10", "3"]),
+               ("pf2p1n_m3", "tc", ["4 30", "12", "40", "0", "1e-4"]),
+               ("sd1p2n_m1", "td", ["4 20", "40"]),
+               ("sd1p2n_m1", "tx", ["4 30", "12", "200", "-16.2"]),
+               ("pf2p1n_m3", "tx", ["4 30", "12", "200", "-26.5"])]
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,option,answers", THICK_MENUS, ids=["%s-%s" % (m[0], m[1]) for m in THICK_MENUS])
+def test_fortran_host_thick_restart_menus_match_reference(name, option, answers, casedir):
+    """Options tf, tc, td (no kept-vector count asked: the reference derives
+    it) and tx, all FP64 (BS_DTYPE=f64), against the reference's FP64 build
+    on the same answers: the .res header (final matrix size, total =
+    iterations + niter * restarts), energies to 2e-5, J and T to 2e-3 (tx
+    stops unconverged at iteration 10 with non-integer J, as the reference
+    does), and the .lcoef (kept Ritz values, couplings up to sign, the last
+    cycle) to 1e-4."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    from golden_cases import read_lcoef, res_iterations
+    meta = {"pf2p1n_m3": ("pf", 2, 1, 3, 83), "sd1p2n_m1": ("sd", 1, 2, 1, 78)}[name]
+    case = small_case(*meta[:4], seed=meta[4])
+    case.name = name
+    wd = os.path.join(casedir, "fthick_%s_%s" % (name, option))
+    write_case(case, wd)
+    out = "%s_%s_fp64" % (name, option)
+    lines = ["n", out, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), name, "1 1 1 1", "end",
+             option] + answers
+    r = _run("\n".join(lines) + "\n", wd, {"BS_DTYPE": "f64"})
+    assert r.returncode == 0, r.stdout + r.stderr
+    base = os.path.join(GOLD, out)
+    assert res_iterations(os.path.join(wd, out + ".res")) == res_iterations(base + ".res")
+    got = read_res(os.path.join(wd, out + ".res"))
+    ref = read_res(base + ".res")
+    assert got.shape == ref.shape == (4, 3)
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 2e-5, (got[:, 0], ref[:, 0])
+    assert np.abs(got[:, 1:] - ref[:, 1:]).max() < 2e-3, (got, ref)
+    k, a, b = read_lcoef(os.path.join(wd, out + ".lcoef"))
+    rk, ra, rb = read_lcoef(base + ".lcoef")
+    assert np.array_equal(k, rk)
+    assert np.abs(a - ra).max() < 1e-4 and np.abs(np.abs(b) - np.abs(rb)).max() < 1e-4

@@ -44,7 +44,7 @@ def test_strength_from_exact_eigenvector(dtype, casedir):
     k = 2
     piv = ex.evecs[k].clone() if dtype == "f64" else ex.evecs[k].float()
     r = s.strength(piv, nkeep=5, niter=200, polish=True)
-    rr, sp, _ = s.lanczos_stats()
+    rr, sp, _, _ = s.lanczos_stats()
     assert rr >= 1
     for a in (r.ritz, r.strength, r.evals, r.alpha, r.beta, r.resid):
         assert np.all(np.isfinite(a))

@@ -30,23 +30,45 @@ def cfg3(tmp_path_factory):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_config3_thick_restart_matches_reference(cfg3, dtype):
+    """The configuration's own run -- option tc, keep 10, restart at 60 with
+    15 kept vectors, at most 40 restarts, largest-change test at 1e-6 --
+    against the reference's FP64 build on it.  The FP64 run follows the
+    reference's cycle step for step and prints its ten energies (f10.5),
+    the -93.910 MeV state absent from both: the uniform pivot has no
+    component along it in exact arithmetic and FP64 rounding has not brought
+    it in by the time the test passes."""
     case, d, s = cfg3
     ref = np.load(os.path.join(GOLD, "config3_tc.npz"))["res"]
     assert ref.shape[0] == 10
-    r = s.lanczos(nkeep=10, niter=60, thick=15, max_restarts=60, dtype=dtype, tol=1e-8, polish=True)
-    print("config 3 %s: %d matvecs, E %s, E_ref(FP64) %s, residuals %s" % (
-        dtype, r.matvecs, " ".join("%.5f" % x for x in r.evals), " ".join("%.5f" % x for x in ref[:, 0]),
-        " ".join("%.0e" % x for x in r.resid)))
-    # every value here is an eigenvalue (residual below 1e-5 / 1e-3), so by
-    # min-max the reference's k-th Ritz value cannot lie below it
-    assert r.resid.max() < (1e-5 if dtype == "f64" else 1e-3)
-    assert np.all(r.evals <= ref[:, 0] + 1.5e-5), (r.evals, ref[:, 0])
-    # the eight lowest agree to the reference's printed digits (f10.5); the
-    # reference's converged run has no state at -93.910 MeV (the uniform
-    # pivot has no component along it in exact arithmetic, and its FP64
-    # rounding had not brought it in), so its 9th value is this run's 10th
-    assert np.abs(r.evals[:8] - ref[:8, 0]).max() < 1.5e-5, (r.evals, ref[:, 0])
-    assert abs(r.evals[9] - ref[8, 0]) < 1.5e-5 and r.evals[8] < ref[8, 0] - 1e-3, (r.evals, ref[:, 0])
+    r = s.lanczos(nkeep=10, niter=60, thick=15, max_restarts=40, dtype=dtype, tol=1e-6, conv_test=1,
+                  polish=True)
+    _, _, steps, restarts = s.lanczos_stats()
+    print("config 3 %s: %d matvecs, %d restarts, final matrix %d, E %s, E_ref(FP64) %s, residuals %s" % (
+        dtype, r.matvecs, restarts, steps, " ".join("%.5f" % x for x in r.evals),
+        " ".join("%.5f" % x for x in ref[:, 0]), " ".join("%.0e" % x for x in r.resid)))
+    if dtype == "f64":
+        assert np.abs(r.evals - ref[:, 0]).max() < 1.5e-5, (r.evals, ref[:, 0])
+    else:
+        # FP32 rounding can bring the hidden state in; the eight lowest agree
+        # and, every value being a Ritz value, none lies above the reference's
+        assert np.abs(r.evals[:8] - ref[:8, 0]).max() < 1e-4, (r.evals, ref[:, 0])
+        assert np.all(r.evals <= ref[:, 0] + 1e-4), (r.evals, ref[:, 0])
+
+
+def test_config3_hidden_state_found_by_longer_run(cfg3):
+    """Run on past the reference's stopping point (test at 1e-10, 60
+    restarts), the FP64 run's lowest ten are eigenpairs (residual below
+    1e-5) and, by min-max, none lies above the reference's; the eight lowest
+    are the reference's."""
+    case, d, s = cfg3
+    ref = np.load(os.path.join(GOLD, "config3_tc.npz"))["res"]
+    r = s.lanczos(nkeep=10, niter=60, thick=15, max_restarts=60, dtype="f64", tol=1e-10, conv_test=1,
+                  polish=True)
+    print("config 3 long: E %s residuals %s" % (" ".join("%.5f" % x for x in r.evals),
+                                                 " ".join("%.0e" % x for x in r.resid)))
+    assert r.resid.max() < 1e-5
+    assert np.all(r.evals <= ref[:, 0] + 1.5e-5)
+    assert np.abs(r.evals[:8] - ref[:8, 0]).max() < 1.5e-5
 
 
 def _solve_e0(case, d, theta, tag):

@@ -7,8 +7,8 @@ import os
 import numpy as np
 import pytest
 
-from golden_cases import (SMALL, config_fixture, ensemble_fixture, dense_from_fixture, paths, probe_vectors,
-                          small_fixture)
+from golden_cases import (GOLD, SMALL, THICK_RUNS, config_fixture, ensemble_fixture, dense_from_fixture, paths,
+                          probe_vectors, read_lcoef, res_iterations, small_fixture)
 from lanczos_ref import lanczos
 from oracle_api import Oracle, lib
 
@@ -327,3 +327,64 @@ def test_oracle_lanczos_equals_reference_fp64_build(casedir):
     assert np.abs(e - ref).max() < 1e-9
     # the printed energies (f10.5) are these Ritz values
     assert np.abs(f64["res"][:5, 0] - ref).max() < 6e-6
+
+
+def _res_energies(path):
+    rows = []
+    lines = open(path, errors="replace").read().splitlines()
+    for i, line in enumerate(lines):
+        if line.strip().startswith("State") and "Ex" in line:
+            for l2 in lines[i + 1:]:
+                p = l2.split()
+                if len(p) < 5 or not p[0].isdigit():
+                    break
+                rows.append(float(p[1]))
+            break
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("name,option,kw", THICK_RUNS, ids=["%s-%s" % (r[0], r[1]) for r in THICK_RUNS])
+def test_oracle_thick_restart_equals_reference_fp64_build(name, option, kw, casedir):
+    """The thick-restart restatement (lanczos_ref.thick_lanczos) against the
+    reference's FP64 build on the same menu answers (tf, tc, td, tx): the same
+    final matrix size and number of restarts (.res "iterations" and "total"),
+    the .lcoef -- kept Ritz values, their couplings up to sign (an
+    eigenvector's sign is free), then the last cycle's steps -- to 1e-4, the
+    .res energies to 2e-5.  The coefficients agree to 1e-8 .. 2e-5, not to
+    1e-12: on these cases the Hamiltonians already differ at the reference's
+    FP32 input rounding (the first alpha by 4e-8) and unconverged Ritz values
+    amplify it; a cycle one vector longer moves them by about 1.  The tx runs
+    stop at the first check (iteration 10): the reference's tx tests no
+    energies."""
+    from lanczos_ref import thick_lanczos
+    case, d, _ = small_fixture(name, casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    r = thick_lanczos(o.apply, o.dim, **kw)
+    base = os.path.join(GOLD, "%s_%s_fp64" % (name, option))
+    it, total = res_iterations(base + ".res")
+    assert r["steps"] == it
+    assert (total if total is not None else it) == it + kw["niter"] * r["restarts"]
+    k, a, b = read_lcoef(base + ".lcoef")
+    assert len(a) == it
+    assert (k < 0).sum() == (kw["nthick"] if r["restarts"] else 0)
+    assert np.abs(a - r["alpha"]).max() < 1e-4
+    assert np.abs(np.abs(b) - np.abs(r["beta"])).max() < 1e-4
+    assert np.abs(_res_energies(base + ".res") - r["evals"]).max() < 2e-5
+    if option == "tx":
+        assert it == 10 and r["restarts"] == 0
+
+
+def test_oracle_targeted_run_converges_to_states_nearest_target(casedir):
+    """tx with the test over the nkeep kept energies (no reference run: the
+    reference's tx never tests them): the four converged energies are the
+    four eigenvalues of the dense H nearest the target, sorted."""
+    from lanczos_ref import thick_lanczos
+    for name, et in (("sd1p2n_m1", -16.2), ("pf2p1n_m3", -26.5)):
+        case, d, _ = small_fixture(name, casedir)
+        o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+        ev = np.linalg.eigvalsh(o.dense())
+        want = np.sort(ev[np.argsort(np.abs(ev - et), kind="stable")[:4]])
+        r = thick_lanczos(o.apply, o.dim, nkeep=4, niter=30, nthick=12, max_restarts=400, tol=1e-9, test=1,
+                          etarget=et, target_test="kept")
+        assert r["restarts"] <= 400
+        assert np.abs(r["evals"] - want).max() < 1e-6, (name, r["evals"], want)
