@@ -167,14 +167,19 @@ def solve_leg(s, case, max_restarts):
     polish) for the plain FP64 path and the mixed path (FP32 vectors and
     elements, FP64 alpha/beta, reorthogonalization, Ritz solve and polish),
     both under the same stopping rule (SOLVE_TOL, the reference's thick-restart
-    cycle and test schedule).  Each counts the eigenpairs within 1e-4 MeV of a
-    tightly converged FP64 run (test at 1e-7 MeV, untimed for the ratio)."""
+    cycle and test schedule).  Each counts the eigenpairs within 1e-4 MeV of
+    the scoring energies: per state the lower of two tightly converged runs
+    (test at 1e-7 MeV, FP64 polish), one FP64 and one mixed, untimed for the
+    ratio.  Both are Ritz values, upper bounds of the true eigenvalues; the
+    lower one is the run that has the state -- from the uniform pivot some
+    states enter only through rounding, sooner in FP32."""
     import numpy as np
     import torch
     thick, niter, nkeep = case.thick, case.niter, case.nkeep
     out = {"nkeep": nkeep, "niter": niter, "thick": thick, "stop_tol_mev": SOLVE_TOL}
     runs = {}
-    for label, dt, tol in (("fp64", "f64", SOLVE_TOL), ("mixed", "f32", SOLVE_TOL), ("exact", "f64", 1e-7)):
+    for label, dt, tol in (("fp64", "f64", SOLVE_TOL), ("mixed", "f32", SOLVE_TOL), ("exact", "f64", 1e-7),
+                           ("exact_mixed", "f32", 1e-7)):
         s.autotune(dt)
         torch.cuda.synchronize()
         t0 = time.time()
@@ -183,14 +188,18 @@ def solve_leg(s, case, max_restarts):
         torch.cuda.synchronize()
         runs[label] = (r, time.time() - t0)
         print("solve %s: %.2f s, %d matvecs" % (label, runs[label][1], r.matvecs), file=sys.stderr, flush=True)
-    rex, tex = runs["exact"]
-    out["exact_fp64"] = {"seconds": tex, "matvecs": rex.matvecs, "evals": rex.evals.tolist(), "stop_tol_mev": 1e-7}
+    for label in ("exact", "exact_mixed"):
+        r, t = runs[label]
+        out[label + "_tight"] = {"seconds": t, "matvecs": r.matvecs, "evals": r.evals.tolist(),
+                                 "resid": r.resid.tolist(), "stop_tol_mev": 1e-7}
+    ref = np.minimum(runs["exact"][0].evals, runs["exact_mixed"][0].evals)
+    out["scoring_evals"] = ref.tolist()
     for label in ("fp64", "mixed"):
         r, t = runs[label]
-        err = np.abs(r.evals - rex.evals)
+        err = np.abs(r.evals - ref)
         ok = int((err <= 1e-4).sum())
         out[label] = {"seconds": t, "matvecs": r.matvecs, "evals": r.evals.tolist(), "resid": r.resid.tolist(),
-                      "abs_err_vs_exact": err.tolist(), "within_1e-4": ok, "eigenpairs_per_s": ok / t}
+                      "abs_err_vs_scoring": err.tolist(), "within_1e-4": ok, "eigenpairs_per_s": ok / t}
     out["speedup_mixed_vs_fp64"] = out["mixed"]["eigenpairs_per_s"] / max(out["fp64"]["eigenpairs_per_s"], 1e-30)
     return out
 

@@ -161,8 +161,12 @@ int bs_gradient(bs_system *sys, const void *psi, int dtype, double *grad);
 typedef struct {
   int nkeep;        /* eigenpairs wanted (<= niter; <= 16 for the batched solve) */
   int niter;        /* Lanczos iterations (plain) or restart length (thick) */
-  int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart */
-  int max_restarts; /* thick restart: maximum number of restarts */
+  int thick;        /* 0: plain Lanczos; >0: vectors kept at each thick restart, <= niter - 2.
+                       The cycle is the reference's (lanczos_p, src/blanczosmain.f90:1690-1752;
+                       thick_restart_sub_p, src/blanczoslib.f90:1656-1799): steps to iter =
+                       niter - 1, restart from the projection on the first niter - 2 vectors
+                       with v_{niter-1} as the next start vector, iter resuming at thick */
+  int max_restarts; /* thick restart: a restart due after max_restarts + 1 of them ends the run */
   int dtype;        /* BS_F32: FP32 vectors/matvec, FP64 coefficients; BS_F64: all FP64 */
   double tol;       /* convergence tolerance (0: run all iterations); the test is conv_test's */
   int polish;       /* 1: FP64 Rayleigh quotient / residual polish of the Ritz vectors */
@@ -171,7 +175,10 @@ typedef struct {
                        sum |dE| / sqrt(ncheck + 1) (times 1 + restarts in a thick run), 1 = their
                        largest change.  A plain run checks every 10th step (sqrt(nkeep) if larger)
                        beyond ncheck, as lanczos_p does (src/blanczosmain.f90:1690-1724); a thick
-                       run checks at each restart. */
+                       run takes the energies at every such step of a cycle beyond max(ncheck,
+                       nkeep) and at iter = niter - 1, and compares at every such step beyond
+                       ncheck with the previous iteration's energies (a restart's own Ritz
+                       values after a restart). */
   int ncheck_ex;    /* additional Ritz values in the test (the reference's ld uses 5) */
   int target;       /* 0: the lowest states.  1, 2: the Ritz pairs nearest etarget (option tx,
                        txchoice 1, src/blanczoslib.f90:277-327; thick > 0): every Ritz solve orders
@@ -186,7 +193,9 @@ typedef struct {
 /* Lanczos from the reference's uniform pivot 1/sqrt(D)
  * (src/blanczoslib.f90:825-831) with full FP64-accumulated classical
  * Gram-Schmidt.  alpha/beta receive the coefficients of the last Krylov
- * sequence (niter entries each, may be NULL); evals receives nkeep Ritz
+ * sequence (up to niter entries each, may be NULL; after a thick restart the
+ * kept Ritz values and their couplings come first, then the last cycle's
+ * steps, as in the reference's .lcoef); evals receives nkeep Ritz
  * values; evecs (device, may be NULL) receives nkeep Ritz vectors of dim
  * elements in dtype, row-major; resid (host, may be NULL) receives the FP64
  * residual norms ||H v - E v||.  iters (may be NULL) receives the total number
