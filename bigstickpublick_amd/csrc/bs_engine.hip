@@ -1145,6 +1145,65 @@ __global__ __launch_bounds__(256) void k_apply_batched(DevTables tb, BatchVals b
   if (on) y[(bf + c) * K + k] = acc;
 }
 
+// Row kernel: y = H x with every gather from global memory and no LDS, one
+// thread per output state, a workgroup per (proton row, run of 256 columns)
+// from the rows_f / rows_c map.  It carries the spaces the tile kernel's LDS
+// cannot: neutron sectors longer than its staged row (4,096 SDs) and Delta
+// classes larger than its signed-coefficient table (63 operators); the
+// neutron rows it gathers from stay in L2.  It is also one more variant on
+// every system.  Same sums as k_apply_tiles: SPE, PP row AXPYs, NN ELL
+// gathers within the row, PN per proton jump over the neutron entries of the
+// conjugate class (padding slots carry the class size as operator index).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_rows(DevTables tb, const int32_t* __restrict__ rows_f,
+                                                    const int32_t* __restrict__ rows_c, const T* __restrict__ x,
+                                                    T* __restrict__ y, int accumulate) {
+  const int f = rows_f[blockIdx.x];
+  const int c = rows_c[blockIdx.x] + (int)threadIdx.x;
+  const int sp = tb.psec[f];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  if (c >= L) return;  // no barriers below
+  const int64_t bf = tb.pstart[f];
+  T acc = T(0);
+  if (tb.call_spe) acc = ((T)tb.pspe_sd[f] + (T)tb.nspe_sd[tb.nfirst[cn] + c]) * x[bf + c];
+  for (int64_t e = tb.pp_ptr[f]; e < tb.pp_ptr[f + 1]; e++)
+    acc = fma(vload<T>(tb.pp_val32, tb.pp_val64, e), x[tb.pstart[tb.pp_src[e]] + c], acc);
+  {
+    const int64_t base = tb.nn_base[cn];
+    const int E = tb.nn_emax[cn];
+    for (int e = 0; e < E; e++) {
+      const int64_t o = base + (int64_t)e * L + c;
+      acc = fma(vload<T>(tb.nn_val32, tb.nn_val64, o), x[bf + tb.nn_src[o]], acc);
+    }
+  }
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int k = 0; k < nd; k++) {
+    const int p0 = tb.pcls[(int64_t)f * nd1 + k], p1 = tb.pcls[(int64_t)f * nd1 + k + 1];
+    if (p0 == p1) continue;
+    const int kn = tb.negk[k];
+    const int ws = tb.wstride[k];
+    const int64_t ebase = tb.n1_base[(int64_t)cn * nd + kn];
+    const int E = tb.n1_emax[(int64_t)cn * nd + kn];
+    const float* Wk = tb.W + tb.woff[k];
+    for (int pj = p0; pj < p1; pj++) {
+      const int64_t bi = tb.pstart[tb.p1_src[pj]];
+      const int meta = tb.p1_meta[pj];
+      const float* wr = Wk + (int64_t)(meta & 0xff) * ws;
+      T a = T(0);
+      for (int e = 0; e < E; e++) {
+        const uint32_t pk = tb.n1_pk[ebase + (int64_t)e * L + c];
+        const int nl = (pk >> 16) & 0xff;
+        if (nl >= ws) continue;  // padding slot
+        const T t = (T)wr[nl] * x[bi + (pk & 0xffffu)];
+        a += ((pk >> 24) & 1u) ? -t : t;
+      }
+      acc += (meta >> 8) ? -a : a;
+    }
+  }
+  y[bf + c] = accumulate ? y[bf + c] + acc : acc;
+}
+
 // K interleaved vectors per basis state on large bases: the tile kernel's
 // design (one workgroup per proton row, CPT columns per thread, per-chunk
 // entry bounds, signed coefficient tables, buffer-load table walks) with every
@@ -1815,6 +1874,9 @@ struct bs_system {
   int last_steps = 0;            // Lanczos steps of its last Krylov sequence
   int last_thick_restarts = 0;   // thick restarts it took
   int cpt = 1;
+  bool tiles_ok = true;        // neutron rows and Delta classes fit the tile kernels' LDS
+  DevBuf<int32_t> rows_f, rows_c;  // k_apply_rows: (proton row, first column) per workgroup
+  int nrows_wg = 0;
   int variant[2] = {-1, -1};   // chosen kernel variant per dtype (0: f32, 1: f64)
   float tune_ms[2][32] = {};    // autotuner timings (ms; 0 = not run / does not fit)
   DevTables dt{};
@@ -1863,8 +1925,10 @@ void upload_tables(bs_system* s) {
     nfirst[c] = (int32_t)N.secfirst[c];
     lmax = std::max(lmax, nlen[c]);
   }
-  if (lmax > kMaxCols) throw std::runtime_error("neutron sector longer than 4096 SDs is not supported");
-  if (lmax > 0xffff) throw std::runtime_error("neutron sector too long for packed jumps");
+  // rows longer than the tile kernels' staged row, or Delta classes larger
+  // than their coefficient table, leave only the row kernel (k_apply_rows)
+  s->tiles_ok = lmax <= kMaxCols && 2 * (t.max_class + 1) <= kWbRows;
+  if (lmax > 0xffff) throw std::runtime_error("neutron sector longer than 65535 SDs does not pack");
   std::vector<int32_t> srcsec((size_t)P.nsec * nd, -1);
   for (int sp = 0; sp < P.nsec; sp++)
     for (int k = 0; k < nd; k++)
@@ -1985,9 +2049,11 @@ void upload_tables(bs_system* s) {
           n1_pk[b + (int64_t)e * L + col] = (uint32_t)sl | ((uint32_t)nl << 16) | (O1.phase[q] < 0 ? (1u << 24) : 0u);
           e++;
         }
-        uint8_t& cm = n1_chunk[((size_t)c * nd + k) * 64 + col / 64];
-        if (e > 255) throw std::runtime_error("too many neutron jumps into one column");
-        cm = (uint8_t)std::max<int>(cm, e);
+        if (s->tiles_ok) {
+          uint8_t& cm = n1_chunk[((size_t)c * nd + k) * 64 + col / 64];
+          if (e > 255) throw std::runtime_error("too many neutron jumps into one column");
+          cm = (uint8_t)std::max<int>(cm, e);
+        }
         for (; e < em; e++) n1_pk[b + (int64_t)e * L + col] = pad;
       }
     }
@@ -2094,6 +2160,24 @@ void upload_tables(bs_system* s) {
   int cpt = 1;
   while (cpt * kApplyBlock < lmax) cpt *= 2;
   s->cpt = cpt;
+  {
+    std::vector<int32_t> rf, rc;
+    for (int f = 0; f < nP; f++) {
+      const int L = nlen[P.conj[P.sdsec[f]]];
+      for (int c0 = 0; c0 < L; c0 += 256) {
+        rf.push_back(f);
+        rc.push_back(c0);
+      }
+    }
+    if ((int64_t)rf.size() >= INT32_MAX) throw std::runtime_error("too many row-kernel workgroups");
+    s->nrows_wg = (int)rf.size();
+    if (rf.empty()) {
+      rf.push_back(0);
+      rc.push_back(0);
+    }
+    s->rows_f.upload(rf);
+    s->rows_c.upload(rc);
+  }
 
   s->pstart.upload(P.start);
   s->psec.upload(psec);
@@ -2133,16 +2217,15 @@ void upload_tables(bs_system* s) {
   // Quad layout for 16-byte loads: per (sector, class) the slot count is
   // rounded up to a multiple of 4 and slot e of column n sits at
   // ((e/4)*L + n)*4 + e%4
-  if (2 * (t.max_class + 1) > kWbRows) throw std::runtime_error("operator class too large for the Wb table");
   std::vector<int64_t> n1q_base((size_t)N.nsec * nd);
   int64_t qtot = 0;
-  for (int c = 0; c < N.nsec; c++)
+  for (int c = 0; s->tiles_ok && c < N.nsec; c++)
     for (int k = 0; k < nd; k++) {
       n1q_base[(size_t)c * nd + k] = qtot;
       qtot += (int64_t)((n1_emax[(size_t)c * nd + k] + 3) / 4 * 4) * N.seccount[c];
     }
   std::vector<uint32_t> quad(std::max<int64_t>(qtot, 1), 0);
-  for (int c = 0; c < N.nsec; c++) {
+  for (int c = 0; s->tiles_ok && c < N.nsec; c++) {
     const int L = nlen[c];
     for (int k = 0; k < nd; k++) {
       const int em = n1_emax[(size_t)c * nd + k], em4 = (em + 3) / 4 * 4;
@@ -2160,7 +2243,8 @@ void upload_tables(bs_system* s) {
       for (size_t i = 0; i < quad.size(); i++) {
         const uint32_t pk = quad[i], sl = pk & 0xffffu, nl = (pk >> 16) & 0xffu, sg = (pk >> 24) & 1u;
         const uint32_t xo = sl * G * ts, wo = (2u * nl + sg) * G * 4u;
-        if (xo >= (1u << 17) || wo >= (1u << 15)) throw std::runtime_error("neutron jump offsets do not pack");
+        if (s->tiles_ok && (xo >= (1u << 17) || wo >= (1u << 15)))
+          throw std::runtime_error("neutron jump offsets do not pack");
         v[i] = xo | (wo << 17);
       }
       s->n1_pkv[gi][ti].upload(v);
@@ -2251,8 +2335,14 @@ void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumu
   HIP_OK(hipGetLastError());
 }
 
+// the row kernel, after the tile variants
+constexpr int kRowsVariant = kNumVariants;
+constexpr int kAllVariants = kNumVariants + 1;
+
 template <typename T>
 size_t lds_bytes(const bs_system* s, int v) {
+  if (v == kRowsVariant) return 0;
+  if (!s->tiles_ok) return SIZE_MAX;
   return lds_bytes_rg<T>(s, kVarR[v % kNumShapes], kVarG[v % kNumShapes]);
 }
 
@@ -2273,8 +2363,15 @@ void launch_cpt(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream
 
 template <typename T>
 void launch_apply_v(bs_system* s, int v, const T* x, T* y, int accumulate, hipStream_t st) {
-  if (v < 0 || v >= kNumVariants) throw std::runtime_error("bad kernel variant");
+  if (v < 0 || v >= kAllVariants) throw std::runtime_error("bad kernel variant");
   if (lds_bytes<T>(s, v) > kLdsLimit) throw std::runtime_error("kernel variant does not fit in LDS");
+  if (v == kRowsVariant) {
+    if (s->nrows_wg > 0)
+      hipLaunchKernelGGL(k_apply_rows<T>, dim3(s->nrows_wg), dim3(256), 0, st, s->dt, s->rows_f.p, s->rows_c.p, x, y,
+                         accumulate);
+    HIP_OK(hipGetLastError());
+    return;
+  }
   switch (s->cpt) {
     case 1: launch_cpt<T, 1>(s, v, x, y, accumulate, st); break;
     case 2: launch_cpt<T, 2>(s, v, x, y, accumulate, st); break;
@@ -2299,7 +2396,7 @@ int autotune(bs_system* s, hipStream_t st) {
   HIP_OK(hipEventCreate(&e1));
   float best = 1e30f;
   int bestv = -1;
-  for (int v = 0; v < kNumVariants; v++) {
+  for (int v = 0; v < kAllVariants; v++) {
     if (lds_bytes<T>(s, v) > kLdsLimit) continue;
     launch_apply_v<T>(s, v, x.p, y.p, 0, st);
     HIP_OK(hipEventRecord(e0, st));
@@ -2324,8 +2421,8 @@ void launch_apply(bs_system* s, const T* x, T* y, int accumulate, hipStream_t st
   if (s->variant[di] < 0) {
     // default: the largest tile that fits, until autotuned
     int v = 2 * kNumShapes - 1;
-    while (v > 0 && lds_bytes<T>(s, v) > kLdsLimit) v--;
-    s->variant[di] = v;
+    while (v >= 0 && lds_bytes<T>(s, v) > kLdsLimit) v--;
+    s->variant[di] = v >= 0 ? v : kRowsVariant;
   }
   launch_apply_v<T>(s, s->variant[di], x, y, accumulate, st);
 }
@@ -2766,6 +2863,9 @@ void build_density_tables(bs_system* s) {
   const bs::Species& P = t.sp[0];
   const bs::Species& N = t.sp[1];
   const int nd = t.ndelta;
+  // the density kernels pack neutron columns in 12 bits and stage rows in LDS
+  if (s->dt.lmax > 4096)
+    throw std::runtime_error("the density and gradient pass takes neutron sectors of at most 4096 SDs");
   // PP: unmerged proton operators
   bs::TwoBodyOps op;
   bs::build_twobody_ops(t, 0, op);
@@ -3002,6 +3102,7 @@ void launch_multi_cpt(const bs_system* s, int v, const BatchVals& bv, const T* x
 
 bool multi_fits(const bs_system* s, int v, int K, size_t tsize) {
   if (v == 0) return true;
+  if (!s->tiles_ok) return false;
   if (K != 1 && K != 2 && K != 4 && K != 8) return false;
   const int G = (v - 1) / 2 == 0 ? 1 : 2;
   return (size_t)G * K * s->dt.lmax * tsize + (size_t)kWbRows * G * K * sizeof(float) <= kLdsLimit;
@@ -3913,12 +4014,20 @@ int bs_apply_h(bs_system* s, const void* x, void* y, int dtype, int accumulate, 
 
 int bs_num_variants(int* n) {
   if (!n) return set_err("null argument");
-  *n = kNumVariants;
+  *n = kAllVariants;
   return 0;
 }
 
 int bs_variant_info(int v, int* rows, int* staged, int* xcd_map, int* prefetch) {
-  if (v < 0 || v >= kNumVariants) return set_err("bad variant");
+  if (v < 0 || v >= kAllVariants) return set_err("bad variant");
+  if (v == kRowsVariant) {
+    // the row kernel: one proton row, nothing staged
+    if (rows) *rows = 1;
+    if (staged) *staged = 0;
+    if (xcd_map) *xcd_map = 0;
+    if (prefetch) *prefetch = 0;
+    return 0;
+  }
   if (rows) *rows = kVarR[v % kNumShapes];
   if (staged) *staged = kVarG[v % kNumShapes];
   if (xcd_map) *xcd_map = (v / kNumShapes) % 2;
@@ -3931,7 +4040,7 @@ int bs_set_variant(bs_system* s, int dtype, int v) {
     checked(s);
     if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
     if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
-    if (v < -1 || v >= kNumVariants) return set_err("bad variant");
+    if (v < -1 || v >= kAllVariants) return set_err("bad variant");
     if (v >= 0) {
       size_t need = dtype == BS_F32 ? lds_bytes<float>(s, v) : lds_bytes<double>(s, v);
       if (need > kLdsLimit) return set_err("variant does not fit in LDS for this system");
@@ -4449,7 +4558,7 @@ int bs_read_wfn(const bs_system* s, const char* path, int first, int count, floa
 
 int bs_autotune_times(const bs_system* s, int dtype, float* ms) {
   if (!s || !ms || (dtype != BS_F32 && dtype != BS_F64)) return set_err("bad argument");
-  for (int v = 0; v < kNumVariants; v++) ms[v] = s->tune_ms[dtype == BS_F32 ? 0 : 1][v];
+  for (int v = 0; v < kAllVariants; v++) ms[v] = s->tune_ms[dtype == BS_F32 ? 0 : 1][v];
   return 0;
 }
 

@@ -66,6 +66,11 @@ SD_ORBITS = (Orbit(0, 2, 5, 2), Orbit(1, 0, 1, 2), Orbit(0, 2, 3, 2))  # 0d5/2 1
 PF_ORBITS = (Orbit(0, 3, 7, 3), Orbit(1, 1, 3, 3), Orbit(0, 3, 5, 3), Orbit(1, 1, 1, 3))
 SD_SPE = (-3.95, -3.16, 1.65)
 PF_SPE = (-8.6, -6.6, -2.1, -4.5)
+# sd plus the 0g orbits: one parity, one W (every orbit written with w = 2,
+# so no truncation applies); its Delta-m = 0 operator class has 110 members
+SDG_ORBITS = SD_ORBITS + (Orbit(0, 4, 9, 2), Orbit(0, 4, 7, 2))
+SDG_SPE = SD_SPE + (4.0, 6.5)
+SPACES = {"sd": (SD_ORBITS, SD_SPE), "pf": (PF_ORBITS, PF_SPE), "sdg": (SDG_ORBITS, SDG_SPE)}
 
 
 def sps_text(orbits: Sequence[Orbit]) -> str:
@@ -136,7 +141,7 @@ def gaussian_interaction(orbits, spe, seed: int, spe_sigma: float = 0.0) -> Inte
 class Case:
     """One solve: model space, nucleons, interaction and Lanczos settings."""
     name: str
-    space: str          # "sd" or "pf"
+    space: str          # "sd", "pf" or "sdg"
     Z: int
     N: int
     jz2: int
@@ -172,7 +177,7 @@ def config_case(cfg: int, member: int = 0) -> Case:
 
 def small_case(space: str, Z: int, N: int, jz2: int = 0, seed: int = 77,
                nkeep: int = 5, niter: int = 60) -> Case:
-    orbits, spe = (SD_ORBITS, SD_SPE) if space == "sd" else (PF_ORBITS, PF_SPE)
+    orbits, spe = SPACES[space]
     it = gaussian_interaction(orbits, spe, seed=seed)
     return Case("%s%dp%dn" % (space, Z, N), space, Z, N, jz2, it, nkeep=nkeep, niter=niter)
 

@@ -113,6 +113,10 @@ int bs_apply_h(bs_system *sys, const void *x, void *y, int dtype, int accumulate
  * tiles on the workgroups that the dispatcher places on one XCD, so each
  * XCD's L2 holds their shared source rows), and prefetch (1: the next batch
  * of source rows is loaded into registers while the current one is applied).
+ * The last variant (staged = 0) is the row kernel: one thread per output
+ * state, every gather from global memory, no LDS; it is the only one on
+ * spaces whose neutron sectors exceed 4096 SDs or whose Delta-m operator
+ * classes exceed 63 operators (up to 65535 SDs and 254 operators).
  * bs_autotune times every variant that fits in LDS on this system and keeps
  * the fastest for that dtype; bs_set_variant forces one (-1 = default
  * choice). */

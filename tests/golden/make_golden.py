@@ -44,6 +44,8 @@ SMALL = [
     ("sd1p1n", "sd", 1, 1, 0, 82),
     ("pf2p1n_m3", "pf", 2, 1, 3, 83),
     ("pf2p2n", "pf", 2, 2, 0, 84),
+    ("sdg1p2n_m1", "sdg", 1, 2, 1, 85),
+    ("pf0p8n", "pf", 0, 8, 0, 86),
 ]
 
 
@@ -132,7 +134,7 @@ def small(name, space, Z, N, jz2, seed):
         case.niter = lanc
         case.nkeep = min(case.nkeep, lanc)
         run_ref(wd, menu_script(case, "n", "lf"), env_extra={"BS_REF_PRINT": "b"})
-        nsps = 12 if space == "sd" else 20
+        nsps = sum(o.j2 + 1 for o in case.orbits)
         pm, nm, tr = read_basis_dump(os.path.join(wd, name + ".basis"), nsps)
         res = read_res(os.path.join(wd, name + ".res"))
         a, b = read_lcoef(os.path.join(wd, name + ".lcoef"))
@@ -497,6 +499,28 @@ def default_convergence():
             shutil.rmtree(wd)
 
 
+def long_rows(name="pf1p8n_m1", space="pf", Z=1, N=8, jz2=1, seed=87, nkeep=5, niter=60):
+    """A system whose neutron sectors (up to 12,022 SDs) exceed the tile
+    kernels' staged row, with proton-neutron terms: the reference's lf run
+    (.res energies, J, T and the .lcoef) as <name>.npz, its parameters as
+    <name>.json; D = 219,781, so no Hamiltonian dump."""
+    case = small_case(space, Z, N, jz2, seed=seed, nkeep=nkeep, niter=niter)
+    case.name = name
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, menu_script(case, "n", "lf"))
+        res = read_res(os.path.join(wd, name + ".res"))
+        a, b = read_lcoef(os.path.join(wd, name + ".lcoef"))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), res=res, alpha=a, beta=b)
+        meta = dict(space=space, Z=Z, N=N, jz2=jz2, seed=seed, nkeep=nkeep, niter=niter)
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(meta, f, indent=1)
+        print(name, res[:, 0])
+    finally:
+        shutil.rmtree(wd)
+
+
 THICK = [("pf2p1n_m3", "tf", ["4 24", "10", "3"]),
          ("pf2p1n_m3", "tc", ["4 30", "12", "40", "0", "1e-4"]),
          ("sd1p2n_m1", "td", ["4 20", "40"]),
@@ -773,6 +797,8 @@ if __name__ == "__main__":
             converge(*c)
     if "thick" in want:
         thick_cycles()
+    if "long" in want:
+        long_rows()
     if "np" in want:
         pivot_spectrum()
     if "li" in want:

@@ -8,7 +8,7 @@ from bigstickpublick_amd.inputs import config_case, small_case, write_case
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SMALL = ["sd2p2n", "sd1p2n_m1", "sd3p1n_mm2", "sd0p3n_m3", "sd1p0n_m1", "sd1p1n",
-         "pf2p1n_m3", "pf2p2n"]
+         "pf2p1n_m3", "pf2p2n", "sdg1p2n_m1", "pf0p8n"]
 
 
 def small_fixture(name, workdir):

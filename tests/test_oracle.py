@@ -388,3 +388,15 @@ def test_oracle_targeted_run_converges_to_states_nearest_target(casedir):
                           etarget=et, target_test="kept")
         assert r["restarts"] <= 400
         assert np.abs(r["evals"] - want).max() < 1e-6, (name, r["evals"], want)
+
+
+def test_oracle_long_rows_equals_reference_run(casedir):
+    """pf 1p8n (2Jz = 1, D = 219,781; neutron sectors up to 12,022 SDs): the
+    FP64 Lanczos restatement on the oracle's H against the reference's lf
+    run: 20 coefficients agree to its FP32-vector rounding (2e-5)."""
+    case, d, fx = small_fixture("pf1p8n_m1", casedir)
+    o = Oracle(*paths(case, d), case.Z, case.N, case.jz2)
+    assert o.dim == 219781
+    a, b, _, _ = lanczos(o.apply, o.dim, 20, 5)
+    assert np.abs(a - fx["alpha"][:20]).max() < 2e-5
+    assert np.abs(b - fx["beta"][:20]).max() < 2e-5
