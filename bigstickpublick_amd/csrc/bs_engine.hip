@@ -3332,10 +3332,25 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   const hipStream_t st = nullptr;
   const int nkeep = p->nkeep;
   if (nkeep < 1) return set_err("nkeep must be >= 1");
-  if (p->thick != 0) return set_err("block Lanczos runs fixed block iterations (thick must be 0)");
   if (D < K) return set_err("block larger than the basis");
-  int nblk = (int)(std::min<int64_t>(p->niter, D) / K);
-  if (nblk * K < nkeep) return set_err("niter must give at least nkeep basis vectors (niter >= nkeep, multiple of block)");
+  if (p->thick < 0) return set_err("thick must be >= 0");
+  // thick restart (option tb): kb whole blocks of Ritz vectors kept; a
+  // Krylov space as large as the basis needs none
+  const int kb = (p->thick > 0 && p->niter < D) ? p->thick / K : 0;
+  // restart when (iter + 1) * K >= niter, iter = blocks in the projected
+  // matrix (lanczos_block, src/blanczosmain.f90:2075-2086)
+  const int iter_r = kb ? (p->niter + K - 1) / K - 1 : 0;
+  int nblk = kb ? iter_r + 1 : (int)(std::min<int64_t>(p->niter, D) / K);
+  if (kb) {
+    if (kb * K < nkeep || kb + 1 >= iter_r)
+      return set_err("thick restart needs nkeep <= (thick / block) * block and room for two more blocks before "
+                     "the restart at niter");
+    if (bp && bp->strength) return set_err("strengths are not defined after a thick restart");
+    if (p->conv_test != 0 && p->conv_test != 1) return set_err("conv_test must be 0 or 1");
+    if (p->ncheck_ex < 0) return set_err("ncheck_ex must be >= 0");
+  } else if (nblk * K < nkeep) {
+    return set_err("niter must give at least nkeep basis vectors (niter >= nkeep, multiple of block)");
+  }
   const int64_t bsz = D * K;
   const int g = grid_for(D);
   const int nbx = std::min(g, 512);
@@ -3397,22 +3412,46 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   std::vector<std::vector<double>> A, B;  // diagonal and sub-diagonal blocks
   std::vector<double> e, Y, e_prev;
   int mv = 0, nb = 0;
+  int ks = 0;               // kept blocks at the front after a thick restart
+  std::vector<double> cpl;  // coupling of block ks to the kept vectors, [a][i], K x ks*K
   auto ritz = [&](int nbl) {
     const int n = nbl * K;
     std::vector<double> T_((size_t)n * n, 0.0);
     for (int j = 0; j < nbl; j++) {
       for (int a = 0; a < K; a++)
         for (int c = 0; c < K; c++) T_[(size_t)(j * K + a) * n + j * K + c] = A[j][a * K + c];
-      if (j + 1 < nbl)
+      if (j + 1 < nbl && j >= ks)
         for (int a = 0; a < K; a++)
           for (int c = 0; c < K; c++) {
             T_[(size_t)((j + 1) * K + a) * n + j * K + c] = B[j][a * K + c];
             T_[(size_t)(j * K + c) * n + (j + 1) * K + a] = B[j][a * K + c];
           }
     }
+    if (ks > 0 && ks < nbl)
+      for (int a = 0; a < K; a++)
+        for (int i = 0; i < ks * K; i++) {
+          T_[(size_t)(ks * K + a) * n + i] = cpl[(size_t)a * ks * K + i];
+          T_[(size_t)i * n + ks * K + a] = cpl[(size_t)a * ks * K + i];
+        }
     jacobi_eigen(n, T_, e, Y);
   };
-  for (int j = 0; j < nblk; j++) {
+  // thick restart: the reference's test (convergence_test with block_flag,
+  // src/blanczoslib.f90:2726-2770) over ncheck = nkeep + ncheck_ex values at
+  // every block iteration, the mean change carrying (1 + restarts)
+  const int ncheck = kb ? (int)std::min<int64_t>(D, (int64_t)nkeep + p->ncheck_ex) : 0;
+  std::vector<double> ecur(ncheck, 0.0), eold(ncheck, 0.0);
+  int restarts = 0;
+  auto energy_change = [&]() {
+    double sum = 0.0, mx = 0.0;
+    for (int i = 0; i < ncheck; i++) {
+      const double d = std::fabs(eold[i] - ecur[i]);
+      sum += d;
+      mx = std::max(mx, d);
+    }
+    return p->conv_test == 1 ? mx : sum / std::sqrt((double)ncheck + 1.0) * (1.0 + restarts);
+  };
+  int j = 0;
+  while (true) {
     T* Vj = V.p + (int64_t)j * bsz;
     launch_batched<T>(s, bv, Vj, W.p, st);
     mv += K;
@@ -3436,20 +3475,85 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
     k_block_update<T, K><<<g, kBlock, 0, st>>>(W.p, V.p, D, m, C.p);
     HIP_OK(hipGetLastError());
     for (int a = 0; a < K; a++) hnorm = std::max(hnorm, std::fabs(Aj[a * K + a]));
-    if (p->tol > 0.0 && nb * K >= nkeep) {
-      // (src/blanczoslib.f90:2660-2800) stop when no kept Ritz value moves by tol
-      ritz(nb);
-      if (!e_prev.empty()) {
-        double dmax = 0.0;
-        for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(e[k] - e_prev[k]));
-        if (dmax < p->tol) break;
+    if (!kb) {
+      if (p->tol > 0.0 && nb * K >= nkeep) {
+        // (src/blanczoslib.f90:2660-2800) stop when no kept Ritz value moves by tol
+        ritz(nb);
+        if (!e_prev.empty()) {
+          double dmax = 0.0;
+          for (int k = 0; k < nkeep; k++) dmax = std::max(dmax, std::fabs(e[k] - e_prev[k]));
+          if (dmax < p->tol) break;
+        }
+        e_prev.assign(e.begin(), e.begin() + nkeep);
       }
-      e_prev.assign(e.begin(), e.begin() + nkeep);
+      if (j + 1 == nblk) break;
+      std::vector<double> Bj;
+      if (!orthonormalize(V.p + (int64_t)(j + 1) * bsz, W.p, Bj, 100.0 * eps * hnorm)) break;  // invariant span
+      B.push_back(Bj);
+      j++;
+      continue;
     }
-    if (j + 1 == nblk) break;
+    // thick-restart block Lanczos: next block, energies once nb * K >= nkeep,
+    // the test at every block iteration beyond ncheck (and the first), then
+    // the restart when (nb + 1) * K >= niter
     std::vector<double> Bj;
-    if (!orthonormalize(V.p + (int64_t)(j + 1) * bsz, W.p, Bj, 100.0 * eps * hnorm)) break;  // invariant span
+    const bool next_ok = orthonormalize(V.p + (int64_t)nb * bsz, W.p, Bj, 100.0 * eps * hnorm);
+    if (nb * K >= nkeep) {
+      ritz(nb);
+      for (int i = 0; i < ncheck; i++) ecur[i] = i < nb * K ? e[i] : 0.0;
+    }
+    bool finished = false;
+    if (p->tol > 0.0) {
+      if (nb * K > ncheck && nb > 1 && energy_change() < p->tol) finished = true;
+      eold = ecur;
+    }
+    if (finished || !next_ok) break;
     B.push_back(Bj);
+    if (nb < iter_r) {
+      j = nb;
+      continue;
+    }
+    if (restarts > p->max_restarts) break;
+    // restart (block_thick_restart_sub_p, src/blanczoslib.f90:1929-2070):
+    // kept Ritz vectors X = V Y[:, 0:kb*K] into blocks 0..kb-1, the new block
+    // V_nb into block kb, coupled to them by B_{nb-1} Y[last block rows, 0:kb*K]
+    ritz(nb);
+    const int n = nb * K, nk = kb * K;
+    {
+      DevBuf<double> dY, X64;
+      std::vector<double> Yk((size_t)n * nk);
+      for (int c = 0; c < n; c++)
+        for (int i = 0; i < nk; i++) Yk[(size_t)c * nk + i] = Y[(size_t)c * n + i];
+      dY.upload(Yk);
+      X64.alloc((size_t)nk * D);
+      for (int k0 = 0; k0 < nk; k0 += 16)
+        k_block_combine<T, K><<<g, kBlock, 0, st>>>(X64.p + (int64_t)k0 * D, V.p, D, n, dY.p + k0, nk,
+                                                    std::min(16, nk - k0));
+      for (int b = 0; b < kb; b++)
+        k_load_block<T, double, K><<<g, kBlock, 0, st>>>(V.p + (int64_t)b * bsz, X64.p + (int64_t)b * K * D, D);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(V.p + (int64_t)kb * bsz, V.p + (int64_t)nb * bsz, sizeof(T) * bsz,
+                            hipMemcpyDeviceToDevice, st));
+      HIP_OK(hipStreamSynchronize(st));
+    }
+    const std::vector<double>& Bl = B[nb - 1];
+    cpl.assign((size_t)K * nk, 0.0);
+    for (int a = 0; a < K; a++)
+      for (int i = 0; i < nk; i++) {
+        double v = 0.0;
+        for (int c = 0; c < K; c++) v += Bl[a * K + c] * Y[(size_t)((nb - 1) * K + c) * n + i];
+        cpl[(size_t)a * nk + i] = v;
+      }
+    A.resize(kb);
+    B.assign(kb, std::vector<double>(K * K, 0.0));
+    for (int b = 0; b < kb; b++) {
+      A[b].assign(K * K, 0.0);
+      for (int a = 0; a < K; a++) A[b][a * K + a] = e[b * K + a];
+    }
+    ks = kb;
+    nb = kb;
+    restarts++;
+    j = kb;
   }
   if (nb * K < nkeep) return set_err("Krylov space smaller than nkeep");
   ritz(nb);
@@ -3507,6 +3611,10 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   }
   HIP_OK(hipStreamSynchronize(st));
   if (iters) *iters = mv;
+  s->last_random_restarts = 0;
+  s->last_second_passes = 0;
+  s->last_steps = nb * K;
+  s->last_thick_restarts = restarts;
   return 0;
 }
 

@@ -412,13 +412,18 @@ class System:
                                        res.ctypes.data, ctypes.byref(it)))
         return LanczosResult(ev, None, None, res, it.value, vec)
 
-    def block_lanczos(self, nkeep, niter, block, dtype="f32", tol=0.0, polish=True, want_vectors=False):
+    def block_lanczos(self, nkeep, niter, block, dtype="f32", tol=0.0, polish=True, want_vectors=False,
+                      thick=0, max_restarts=0, conv_test=0, ncheck_ex=0):
         """Block Lanczos with `block` vectors per block and a basis of niter
-        vectors (fixed, or stopped at tol); alpha/beta are not returned."""
+        vectors (fixed, or stopped at tol); alpha/beta are not returned.
+        thick > 0 (option tb): restart when the basis would reach niter,
+        keeping (thick // block) whole blocks of Ritz vectors, at most
+        max_restarts + 1 restarts, the reference's energy test conv_test over
+        nkeep + ncheck_ex values at every block iteration (tb: 0, 2, 1e-3)."""
         import torch
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
-        p = _Params(nkeep, niter, 0, 0, dt, tol, int(polish))
+        p = _Params(nkeep, niter, thick, max_restarts, dt, tol, int(polish), conv_test, ncheck_ex)
         ev = np.zeros(nkeep)
         res = np.zeros(nkeep)
         it = ctypes.c_int()

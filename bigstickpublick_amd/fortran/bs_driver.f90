@@ -101,10 +101,10 @@ contains
     read(5, '(a)') lopt
     lopt = lower(adjustl(lopt))
     select case (lopt(1:2))
-    case ('ex', 'ld', 'lf', 'lc', 'td', 'tf', 'tc', 'tx', 'bd', 'bf', 'bc', 'sk', 'li')
+    case ('ex', 'ld', 'lf', 'lc', 'td', 'tf', 'tc', 'tx', 'tb', 'bd', 'bf', 'bc', 'sk', 'li')
     case default
       write(6, *) ' bs_driver: diagonalization option ', trim(lopt), ' is not supported', &
-                  ' (supported: ex, ld, lf, lc, td, tf, tc, tx, bd, bf, bc, sk, li)'
+                  ' (supported: ex, ld, lf, lc, td, tf, tc, tx, tb, bd, bf, bc, sk, li)'
       stop 1
     end select
     dimblock = 0
@@ -141,9 +141,10 @@ contains
       ! the .lcoef and the .res header are written, no eigenstates
       read(5, *) niter
       nkeep = 1
-    else if (lopt(1:1) == 'b') then
+    else if (lopt(1:1) == 'b' .or. lopt(1:2) == 'tb') then
       ! block Lanczos bf / bd: nkeep, block size, block iterations; the basis
-      ! holds (iterations + 1) blocks (src/blanczosmain.f90:849-869)
+      ! holds (iterations + 1) blocks (src/blanczosmain.f90:849-869); tb
+      ! restarts there (src/blanczosmain.f90:1179-1212)
       read(5, *) nkeep, dimblock, nbiter
       niter = nbiter * dimblock + dimblock
     else
@@ -153,7 +154,15 @@ contains
     maxrest = 0
     tgt = 0
     etarget = 0.0d0
-    if (lopt(1:2) == 'tx') then
+    if (lopt(1:2) == 'tb') then
+      ! tb: niter raised to at least max((nkeep + 5) * 2, 10), then the kept
+      ! vectors (at least dimblock + 2) and max restarts; test 0 at 1e-3 over
+      ! nkeep + 2 energies (src/blanczosmain.f90:1197-1212)
+      niter = max(niter, max((nkeep + 5) * 2, 10))
+      read(5, *) nthick
+      if (nthick < dimblock + 2) nthick = dimblock + 2
+      read(5, *) maxrest
+    else if (lopt(1:2) == 'tx') then
       ! tx: thick restart keeping the states nearest a target energy
       ! (src/blanczosmain.f90:1022-1053): niter raised to at least
       ! max((nkeep + 5) * 2, 10), kept vectors at least nkeep, restarts,
@@ -185,6 +194,7 @@ contains
     ctest = 0
     ncheck = 0
     if (lopt(1:2) == 'ld') ncheck = 5
+    if (lopt(1:2) == 'tb') ncheck = 2
     if (lopt(1:1) == 'b') ctol = 1.0d-6
     if (lopt(2:2) == 'c' .or. lopt(2:2) == 'C') then
       ! lc / bc / tc: [additional states for the test,] test choice, tolerance
@@ -223,7 +233,7 @@ contains
     p%max_restarts = maxrest
     p%dtype = dtype
     p%tol = 0.0d0
-    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'x') p%tol = ctol
+    if (lopt(2:2) == 'd' .or. lopt(2:2) == 'c' .or. lopt(2:2) == 'x' .or. lopt(1:2) == 'tb') p%tol = ctol
     p%target = tgt
     p%etarget = etarget
     p%conv_test = ctest

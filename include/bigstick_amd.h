@@ -252,11 +252,20 @@ int bs_exact(bs_system *sys, int nkeep, int dtype, double *evals, void *evecs, d
  * block (FP64 sums), each new block orthonormalized by the spectral
  * decomposition of its Gram matrix, Ritz pairs of the block tridiagonal
  * matrix.  p->niter is the basis size (rounded down to a multiple of block),
- * p->thick must be 0, p->tol > 0 stops when no kept Ritz value moves by more
- * than tol between block iterations; evals, evecs, resid, iters and the FP64
- * polish as in bs_lanczos.  The pivot block is seeded pseudo-random.
- * Replaces lanczos_block with option bf / bd (src/blanczosmain.f90:1804-2153,
- * :822-872). */
+ * p->tol > 0 stops when no kept Ritz value moves by more than tol between
+ * block iterations; evals, evecs, resid, iters and the FP64 polish as in
+ * bs_lanczos.  The pivot block is seeded pseudo-random.  p->thick > 0 is
+ * option tb (src/blanczosmain.f90:1179-1212, block_thick_restart_sub_p,
+ * src/blanczoslib.f90:1929-2070): when the next block would bring the basis
+ * to niter vectors, (thick / block) whole blocks of Ritz vectors are kept
+ * (nkeep <= that count), the new block is coupled to them by its beta block
+ * times the last block rows of their eigenvectors, and the run goes on; at
+ * most max_restarts + 1 restarts; the test is then the reference's energy
+ * test p->conv_test over nkeep + p->ncheck_ex values at every block iteration
+ * (mean change times 1 + restarts; tb uses 0, 2 and 1e-3).
+ * bs_lanczos_stats reports the final basis size and the restarts.
+ * Replaces lanczos_block with option bf / bd / tb
+ * (src/blanczosmain.f90:1804-2153, :822-872). */
 int bs_block_lanczos(bs_system *sys, const bs_lanczos_params *p, int block, double *evals, void *evecs,
                      double *resid, int *iters);
 

@@ -278,6 +278,27 @@ def pivot_wfn(name, seed):
             f.write(v.astype("<f4").tobytes())
 
 
+def pivot4_wfn(name="pf2p1n_m3", seed=92):
+    """<name>_pivot4.wfn: the header of the reference's <name>.wfn with four
+    seeded Gaussian vectors (norms 1.0 .. 1.3), none an eigenstate: pivots
+    for block runs that read nkeep states of the file."""
+    import struct
+    sys.path.insert(0, os.path.dirname(HERE))
+    from wfn_io import read_wfn
+    ref = read_wfn(os.path.join(HERE, name + ".wfn"))
+    rng = np.random.default_rng(seed)
+    vecs = []
+    for k in range(4):
+        g = rng.standard_normal(ref["dim"])
+        vecs.append((1.0 + 0.1 * k) * g / np.linalg.norm(g))
+    with open(os.path.join(HERE, name + "_pivot4.wfn"), "wb") as f:
+        f.write(ref["header"])
+        f.write(struct.pack("<i", len(vecs)))
+        for k, v in enumerate(vecs):
+            f.write(struct.pack("<2i3f", 27182818, k + 1, 0.0, 0.0, 0.0))
+            f.write(v.astype("<f4").tobytes())
+
+
 # strength functions (menu s / ss) from the pivots of <name>_pivot.wfn with
 # the Hamiltonian that made <name>.wfn: (name, menu, pivot, nkeep, niter)
 STRENGTH = [("sd2p2n", "s", 1, 5, 20), ("sd2p2n", "ss", 2, 3, 30), ("pf2p1n_m3", "s", 2, 4, 25)]
@@ -525,18 +546,23 @@ THICK = [("pf2p1n_m3", "tf", ["4 24", "10", "3"]),
          ("pf2p1n_m3", "tc", ["4 30", "12", "40", "0", "1e-4"]),
          ("sd1p2n_m1", "td", ["4 20", "40"]),
          ("sd1p2n_m1", "tx", ["4 30", "12", "200", "-16.2"]),
-         ("pf2p1n_m3", "tx", ["4 30", "12", "200", "-26.5"])]
+         ("pf2p1n_m3", "tx", ["4 30", "12", "200", "-26.5"]),
+         ("pf2p1n_m3", "tb", ["4 4 4", "8", "50"])]
 
 
-def thick_cycles():
+def thick_cycles(only=None):
     """The reference's FP64 build on its thick-restart options
     (src/blanczosmain.f90:993-1053, :1091-1145): tf (fixed restarts), tc
     (test 0 at 1e-4), td (defaults: nthick max(nkeep + 5, 3 nkeep) capped at
     niter - 5, test 0 at 1e-3) and tx (states nearest a target energy) on
     pf 2p1n (2Jz = 3) and sd 1p2n (2Jz = 1), whose low states all overlap the
-    uniform pivot: .res and .lcoef (the last cycle's coefficients after the
-    kept Ritz values and couplings) kept as <name>_<option>_fp64.*."""
+    uniform pivot, and tb (thick-restart block Lanczos from the reference's
+    random pivot block: only its converged .res is comparable): .res and, but
+    for tb, .lcoef (the last cycle's coefficients after the kept Ritz values
+    and couplings) kept as <name>_<option>_fp64.*."""
     for name, option, answers in THICK:
+        if only and option not in only:
+            continue
         meta = [c for c in SMALL if c[0] == name][0]
         case = small_case(*meta[1:5], seed=meta[5])
         case.name = name
@@ -547,7 +573,7 @@ def thick_cycles():
             script = "\n".join(["n", out, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), case.name,
                                 "1 1 1 1", "end", option] + answers) + "\n"
             run_ref(wd, script, exe=REF64)
-            for ext in (".res", ".lcoef"):
+            for ext in (".res", ".lcoef") if option != "tb" else (".res",):
                 shutil.copy(os.path.join(wd, out + ext), os.path.join(HERE, out + ext))
             print(out, read_res(os.path.join(wd, out + ".res"))[:, 0])
         finally:
@@ -795,8 +821,12 @@ if __name__ == "__main__":
     for c in CONVERGE:
         if "converge" in want:
             converge(*c)
+    if "pivot4" in want:
+        pivot4_wfn()
     if "thick" in want:
         thick_cycles()
+    if "tb" in want:
+        thick_cycles(only=("tb",))
     if "long" in want:
         long_rows()
     if "np" in want:

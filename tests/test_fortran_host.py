@@ -732,8 +732,8 @@ def test_fortran_host_pivot_spectrum_matches_reference(casedir):
 
 @needs_driver
 def test_fortran_host_refuses_unsupported_option(casedir):
-    """A diagonalization option the driver does not implement (tb: thick-
-    restart block Lanczos) is refused with a message, not misread;
+    """A diagonalization option the driver does not implement (tp: thick
+    restart with a power filter) is refused with a message, not misread;
     upper-case spellings of supported ones pass that check."""
     from bigstickpublick_amd.inputs import small_case, write_case
     case = small_case("sd", 2, 2, 0, seed=77)
@@ -741,7 +741,7 @@ def test_fortran_host_refuses_unsupported_option(casedir):
     wd = os.path.join(casedir, "fopt_refuse")
     write_case(case, wd)
     head = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end"]
-    r = _run("\n".join(head + ["tb", "5 30"]) + "\n", wd)
+    r = _run("\n".join(head + ["tp", "5 30"]) + "\n", wd)
     assert r.returncode != 0 and "is not supported" in r.stdout, r.stdout + r.stderr
     r = _run("\n".join(head + ["LC", "5 300", "2", "3", "1e-6"]) + "\n", wd)
     assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
@@ -806,3 +806,26 @@ def test_fortran_host_thick_restart_menus_match_reference(name, option, answers,
     rk, ra, rb = read_lcoef(base + ".lcoef")
     assert np.array_equal(k, rk)
     assert np.abs(a - ra).max() < 1e-4 and np.abs(np.abs(b) - np.abs(rb)).max() < 1e-4
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_block_thick_restart_menu(casedir):
+    """Option tb (nkeep, block size, block iterations; kept vectors; max
+    restarts: src/blanczosmain.f90:1179-1212) on pf 2p1n, FP64: the reference
+    starts from its own random pivot block and this engine from a seeded one,
+    so the energies agree to the default test's reach (7e-4 MeV), J and T to
+    the printed digits of converged states."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case("pf", 2, 1, 3, seed=83)
+    case.name = "pf2p1n_m3"
+    wd = os.path.join(casedir, "ftb")
+    write_case(case, wd)
+    lines = ["n", "tb_out", case.space, "2 1", "3", case.name, "1 1 1 1", "end", "tb", "4 4 4", "8", "50"]
+    r = _run("\n".join(lines) + "\n", wd, {"BS_DTYPE": "f64"})
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_res(os.path.join(wd, "tb_out.res"))
+    ref = read_res(os.path.join(GOLD, "pf2p1n_m3_tb_fp64.res"))
+    assert got.shape == ref.shape == (4, 3)
+    assert np.abs(got[:, 0] - ref[:, 0]).max() < 7e-4, (got, ref)
+    assert np.abs(got[:3, 1:] - ref[:3, 1:]).max() < 2e-3, (got, ref)
