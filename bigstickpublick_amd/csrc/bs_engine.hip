@@ -950,14 +950,14 @@ struct DensTables {
   const int32_t* pp_src;
   const int32_t* pp_me;
   const int8_t* pp_ph;
-  // NN: per neutron sector, flat lists of (target col | source col << 12 | sign << 24) and compact element id
+  // NN: per neutron sector, flat lists of (target col | source col << 24 | sign << 48) and compact element id
   const int64_t* nn_off;     // [nsecN + 1]
-  const uint32_t* nn_pk;
+  const uint64_t* nn_pk;
   const int32_t* nn_mc;
   int nn_ncomp;
   // PN: lists per (target neutron sector, class, neutron operator)
   const int64_t* pn_off;     // [(c*nd + k)*(maxcls+1) + nl]
-  const uint32_t* pn_pk;     // target col | source col << 12 | sign << 24
+  const uint64_t* pn_pk;     // target col | source col << 24 | sign << 48
   int maxcls1;
 };
 
@@ -966,9 +966,18 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The density sums accumulate in 64-bit fixed point (value * scale, rounded,
+// two's complement): integer addition is associative, so the result does not
+// depend on the order in which workgroups and lanes add -- every run gives
+// the same bits.  The host picks scale = 2^k so that no element can leave the
+// int64 range (|element| <= |psi|^2: every operator here has norm <= 1).
+__device__ __forceinline__ unsigned long long dfix(double v, double scale) {
+  return (unsigned long long)__double2ll_rn(v * scale);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_density_pp(DevTables tb, DensTables dn, const T* __restrict__ x,
-                                                    double* __restrict__ rho) {
+                                                    unsigned long long* __restrict__ rho, double scale) {
   const int f = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int L = tb.nlen[tb.pconj[tb.psec[f]]];
@@ -978,21 +987,23 @@ __global__ __launch_bounds__(256) void k_density_pp(DevTables tb, DensTables dn,
     double a = 0.0;
     for (int c = lane; c < L; c += 64) a += (double)x[bf + c] * (double)x[bs + c];
     a = wave_sum(a);
-    if (lane == 0) atomicAdd(&rho[dn.pp_me[e]], (double)dn.pp_ph[e] * a);
+    if (lane == 0) atomicAdd(&rho[dn.pp_me[e]], dfix((double)dn.pp_ph[e] * a, scale));
   }
 }
 
+// NN: LDS-staged tiles of R rows with per-workgroup element sums in LDS
+// (needs R rows and the element accumulators in LDS)
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_density_nn(DevTables tb, DensTables dn, const T* __restrict__ x,
-                                                    double* __restrict__ rho_comp) {
+                                                    unsigned long long* __restrict__ rho_comp, double scale) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* rl = reinterpret_cast<double*>(smem);
-  T* xs = reinterpret_cast<T*>(smem + (size_t)dn.nn_ncomp * sizeof(double));
+  unsigned long long* rl = reinterpret_cast<unsigned long long*>(smem);
+  T* xs = reinterpret_cast<T*>(smem + (size_t)dn.nn_ncomp * sizeof(unsigned long long));
   const int tile = blockIdx.x;
   const int f0 = tb.tile_f0[tile], nr = tb.tile_nr[tile];
   const int cn = tb.pconj[tb.psec[f0]];
   const int L = tb.nlen[cn];
-  for (int i = threadIdx.x; i < dn.nn_ncomp; i += 256) rl[i] = 0.0;
+  for (int i = threadIdx.x; i < dn.nn_ncomp; i += 256) rl[i] = 0ull;
   for (int r = 0; r < R; r++) {
     const int64_t bf = r < nr ? tb.pstart[f0 + r] : 0;
     for (int c = threadIdx.x; c < L; c += 256) xs[c * R + r] = r < nr ? x[bf + c] : T(0);
@@ -1000,32 +1011,50 @@ __global__ __launch_bounds__(256) void k_density_nn(DevTables tb, DensTables dn,
   __syncthreads();
   const VecG<T, R>* xv = reinterpret_cast<const VecG<T, R>*>(xs);
   for (int64_t q = dn.nn_off[cn] + threadIdx.x; q < dn.nn_off[cn + 1]; q += 256) {
-    const uint32_t pk = dn.nn_pk[q];
-    const VecG<T, R> a = xv[pk & 0xfff];
-    const VecG<T, R> b = xv[(pk >> 12) & 0xfff];
+    const uint64_t pk = dn.nn_pk[q];
+    const VecG<T, R> a = xv[pk & 0xffffffu];
+    const VecG<T, R> b = xv[(pk >> 24) & 0xffffffu];
     double v = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) v += (double)a.v[r] * (double)b.v[r];
-    atomicAdd(&rl[dn.nn_mc[q]], ((pk >> 24) & 1) ? -v : v);
+    atomicAdd(&rl[dn.nn_mc[q]], dfix(((pk >> 48) & 1) ? -v : v, scale));
   }
   __syncthreads();
   for (int i = threadIdx.x; i < dn.nn_ncomp; i += 256)
-    if (rl[i] != 0.0) atomicAdd(&rho_comp[i], rl[i]);
+    if (rl[i] != 0ull) atomicAdd(&rho_comp[i], rl[i]);
 }
 
-template <typename T, int G>
+// NN without LDS (rows or element sets too large for it): a workgroup per
+// proton row, the row's values gathered from global memory
+template <typename T>
+__global__ __launch_bounds__(256) void k_density_nn_g(DevTables tb, DensTables dn, const T* __restrict__ x,
+                                                      unsigned long long* __restrict__ rho_comp, double scale) {
+  const int f = blockIdx.x;
+  const int cn = tb.pconj[tb.psec[f]];
+  const int64_t bf = tb.pstart[f];
+  for (int64_t q = dn.nn_off[cn] + threadIdx.x; q < dn.nn_off[cn + 1]; q += 256) {
+    const uint64_t pk = dn.nn_pk[q];
+    const double v = (double)x[bf + (int64_t)(pk & 0xffffffu)] * (double)x[bf + (int64_t)((pk >> 24) & 0xffffffu)];
+    atomicAdd(&rho_comp[dn.nn_mc[q]], dfix(((pk >> 48) & 1) ? -v : v, scale));
+  }
+}
+
+// PN: per proton row, batches of G source rows of one class; GLB = 0 stages
+// them and the row itself in LDS, GLB = 1 gathers from global memory
+template <typename T, int G, bool GLB>
 __global__ __launch_bounds__(256) void k_density_pn(DevTables tb, DensTables dn, const T* __restrict__ x,
-                                                    double* __restrict__ rho_w) {
+                                                    unsigned long long* __restrict__ rho_w, double scale) {
   extern __shared__ __align__(16) unsigned char smem[];
-  T* xs = reinterpret_cast<T*>(smem);                               // [lmax][G] staged sources
-  T* yf = reinterpret_cast<T*>(smem + (size_t)G * tb.lmax * sizeof(T));  // own row
+  T* xs = reinterpret_cast<T*>(smem);                                      // [lmax][G] staged sources
+  T* yf = reinterpret_cast<T*>(smem + (GLB ? 0 : (size_t)G * tb.lmax * sizeof(T)));  // own row
   const int f = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int sp = tb.psec[f];
   const int cn = tb.pconj[sp];
   const int L = tb.nlen[cn];
   const int64_t bf = tb.pstart[f];
-  for (int c = threadIdx.x; c < L; c += 256) yf[c] = x[bf + c];
+  if constexpr (!GLB)
+    for (int c = threadIdx.x; c < L; c += 256) yf[c] = x[bf + c];
   const VecG<T, G>* xg = reinterpret_cast<const VecG<T, G>*>(xs);
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k = 0; k < nd; k++) {
@@ -1038,30 +1067,43 @@ __global__ __launch_bounds__(256) void k_density_pn(DevTables tb, DensTables dn,
     const int64_t lbase = ((int64_t)cn * nd + kn) * dn.maxcls1;
     for (int b0 = p0; b0 < p1; b0 += G) {
       const int nb = min(G, p1 - b0);
-      __syncthreads();
-      for (int g = 0; g < G; g++) {
-        const int64_t bi = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : 0;
-        for (int c = threadIdx.x; c < Li; c += 256) xs[c * G + g] = g < nb ? x[bi + c] : T(0);
+      int64_t bi[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) bi[g] = g < nb ? tb.pstart[tb.p1_src[b0 + g]] : -1;
+      if constexpr (!GLB) {
+        __syncthreads();
+        for (int g = 0; g < G; g++)
+          for (int c = threadIdx.x; c < Li; c += 256) xs[c * G + g] = bi[g] >= 0 ? x[bi[g] + c] : T(0);
+        __syncthreads();
       }
-      __syncthreads();
       for (int nl = wave; nl < ws; nl += 4) {
         double a[G];
 #pragma unroll
         for (int g = 0; g < G; g++) a[g] = 0.0;
         for (int64_t q = dn.pn_off[lbase + nl] + lane; q < dn.pn_off[lbase + nl + 1]; q += 64) {
-          const uint32_t pk = dn.pn_pk[q];
-          const double yv = ((pk >> 24) & 1) ? -(double)yf[pk & 0xfff] : (double)yf[pk & 0xfff];
-          const VecG<T, G> xv = xg[(pk >> 12) & 0xfff];
+          const uint64_t pk = dn.pn_pk[q];
+          const int tc = (int)(pk & 0xffffffu), sc = (int)((pk >> 24) & 0xffffffu);
+          double yv;
+          if constexpr (GLB) yv = (double)x[bf + tc];
+          else yv = (double)yf[tc];
+          if ((pk >> 48) & 1) yv = -yv;
+          if constexpr (GLB) {
 #pragma unroll
-          for (int g = 0; g < G; g++) a[g] += yv * (double)xv.v[g];
+            for (int g = 0; g < G; g++)
+              if (bi[g] >= 0) a[g] += yv * (double)x[bi[g] + sc];
+          } else {
+            const VecG<T, G> xv = xg[sc];
+#pragma unroll
+            for (int g = 0; g < G; g++) a[g] += yv * (double)xv.v[g];
+          }
         }
 #pragma unroll
         for (int g = 0; g < G; g++) a[g] = wave_sum(a[g]);
         if (lane == 0)
           for (int g = 0; g < nb; g++) {
             const int meta = tb.p1_meta[b0 + g];
-            const double s = (meta >> 8) ? -a[g] : a[g];
-            atomicAdd(&rho_w[tb.woff[k] + (meta & 0xff) * ws + nl], s);
+            const double sgn = (meta >> 8) ? -a[g] : a[g];
+            atomicAdd(&rho_w[tb.woff[k] + (meta & 0xff) * ws + nl], dfix(sgn, scale));
           }
       }
     }
@@ -1866,7 +1908,7 @@ struct bs_system {
   DevBuf<int64_t> d_pp_ptr, d_nn_off, d_pn_off;
   DevBuf<int32_t> d_pp_src, d_pp_me, d_nn_mc;
   DevBuf<int8_t> d_pp_ph;
-  DevBuf<uint32_t> d_nn_pk, d_pn_pk;
+  DevBuf<uint64_t> d_nn_pk, d_pn_pk;
   std::vector<int32_t> nn_comp_to_me;
   bool jac_ready = false;
   std::vector<bs::ParamColumn> jac;
@@ -2907,9 +2949,6 @@ void build_density_tables(bs_system* s) {
   const bs::Species& P = t.sp[0];
   const bs::Species& N = t.sp[1];
   const int nd = t.ndelta;
-  // the density kernels pack neutron columns in 12 bits and stage rows in LDS
-  if (s->dt.lmax > 4096)
-    throw std::runtime_error("the density and gradient pass takes neutron sectors of at most 4096 SDs");
   // PP: unmerged proton operators
   bs::TwoBodyOps op;
   bs::build_twobody_ops(t, 0, op);
@@ -2923,7 +2962,7 @@ void build_density_tables(bs_system* s) {
   std::vector<int32_t> me_to_comp((size_t)t.npairxx * t.npairxx, -1);
   s->nn_comp_to_me.clear();
   std::vector<int64_t> off(N.nsec + 1, 0);
-  std::vector<uint32_t> pk;
+  std::vector<uint64_t> pk;
   std::vector<int32_t> mc;
   for (int c = 0; c < N.nsec; c++) {
     off[c] = (int64_t)pk.size();
@@ -2936,7 +2975,7 @@ void build_density_tables(bs_system* s) {
           cid = (int32_t)s->nn_comp_to_me.size();
           s->nn_comp_to_me.push_back(on.me[q]);
         }
-        pk.push_back((uint32_t)nf | ((uint32_t)ns << 12) | (on.phase[q] < 0 ? (1u << 24) : 0u));
+        pk.push_back((uint64_t)nf | ((uint64_t)ns << 24) | (on.phase[q] < 0 ? (1ull << 48) : 0ull));
         mc.push_back(cid);
       }
   }
@@ -2948,8 +2987,8 @@ void build_density_tables(bs_system* s) {
   // PN: per (target neutron sector, class, neutron operator) lists
   const int mc1 = t.max_class + 1;
   std::vector<int64_t> poff((size_t)N.nsec * nd * mc1 + 1, 0);
-  std::vector<std::vector<uint32_t>> bucket(mc1);
-  std::vector<uint32_t> ppk;
+  std::vector<std::vector<uint64_t>> bucket(mc1);
+  std::vector<uint64_t> ppk;
   const bs::OneBody& O1 = t.ob[1];
   for (int c = 0; c < N.nsec; c++)
     for (int k = 0; k < nd; k++) {
@@ -2961,8 +3000,8 @@ void build_density_tables(bs_system* s) {
         for (int64_t q = O1.rowptr[n]; q < O1.rowptr[n + 1]; q++) {
           if (t.class_of_op[O1.op[q]] != k) continue;
           const int64_t nf = n - N.secfirst[c], ni = O1.src[q] - N.secfirst[want];
-          bucket[t.local_of_op[O1.op[q]]].push_back((uint32_t)nf | ((uint32_t)ni << 12) |
-                                                    (O1.phase[q] < 0 ? (1u << 24) : 0u));
+          bucket[t.local_of_op[O1.op[q]]].push_back((uint64_t)nf | ((uint64_t)ni << 24) |
+                                                    (O1.phase[q] < 0 ? (1ull << 48) : 0ull));
         }
       for (int nl = 0; nl < mc1; nl++) {
         poff[((size_t)c * nd + k) * mc1 + nl] = (int64_t)ppk.size();
@@ -2996,34 +3035,15 @@ void run_density(bs_system* s, const T* psi, double* rho_pp, double* rho_nn, dou
   const bs::Tables& t = s->t;
   const int64_t D = t.dim;
   const int npx = t.npairxx, nd = t.ndelta;
-  DevBuf<double> rpp, rnn, rw, part, nrm;
+  DevBuf<unsigned long long> rpp, rnn, rw;
+  DevBuf<double> part, nrm;
   rpp.alloc((size_t)std::max(npx, 1) * std::max(npx, 1));
   rnn.alloc(std::max<size_t>(s->nn_comp_to_me.size(), 1));
   rw.alloc(std::max<size_t>(t.wpn.size(), 1));
-  HIP_OK(hipMemset(rpp.p, 0, rpp.n * sizeof(double)));
-  HIP_OK(hipMemset(rnn.p, 0, rnn.n * sizeof(double)));
-  HIP_OK(hipMemset(rw.p, 0, rw.n * sizeof(double)));
-  DevTables d = s->dt;
-  const int nP = d.nP;
-  if (t.in.Z >= 2 && nP > 0) k_density_pp<T><<<nP, 256>>>(d, s->dn, psi, rpp.p);
-  if (t.in.N >= 2 && s->dn.nn_ncomp > 0) {
-    d.tile_f0 = s->tile_f0[2].p;
-    d.tile_nr = s->tile_nr[2].p;
-    const size_t lds = (size_t)s->dn.nn_ncomp * sizeof(double) + (size_t)4 * d.lmax * sizeof(T);
-    if (lds > kLdsLimit) throw std::runtime_error("NN density tables too large for LDS");
-    auto fn = k_density_nn<T, 4>;
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kLdsLimit));
-    hipLaunchKernelGGL(fn, dim3(s->ntiles[2]), dim3(256), lds, nullptr, d, s->dn, psi, rnn.p);
-  }
-  if (t.in.Z >= 1 && t.in.N >= 1 && nP > 0) {
-    const size_t lds = (size_t)5 * d.lmax * sizeof(T);
-    auto fn = k_density_pn<T, 4>;
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kLdsLimit));
-    hipLaunchKernelGGL(fn, dim3(nP), dim3(256), lds, nullptr, d, s->dn, psi, rw.p);
-  }
-  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemset(rpp.p, 0, rpp.n * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(rnn.p, 0, rnn.n * sizeof(unsigned long long)));
+  HIP_OK(hipMemset(rw.p, 0, rw.n * sizeof(unsigned long long)));
+  // |psi|^2 first: it bounds every element, and so fixes the fixed-point scale
   const int nb = std::min(grid_for(D), 1024);
   part.alloc(nb);
   nrm.alloc(1);
@@ -3032,19 +3052,58 @@ void run_density(bs_system* s, const T* psi, double* rho_pp, double* rho_nn, dou
   HIP_OK(hipGetLastError());
   double n2 = 0.0;
   HIP_OK(hipMemcpy(&n2, nrm.p, sizeof(double), hipMemcpyDeviceToHost));
+  if (!std::isfinite(n2)) throw std::runtime_error("psi is not finite");
+  // elements and partial sums below 64 |psi|^2, summed below 2^62
+  int ex = 0;
+  std::frexp(std::max(n2, 1e-300) * 64.0, &ex);
+  const double scale = std::ldexp(1.0, std::min(62 - ex, 1000));
+  DevTables d = s->dt;
+  const int nP = d.nP;
+  if (t.in.Z >= 2 && nP > 0) k_density_pp<T><<<nP, 256>>>(d, s->dn, psi, rpp.p, scale);
+  if (t.in.N >= 2 && s->dn.nn_ncomp > 0) {
+    d.tile_f0 = s->tile_f0[2].p;
+    d.tile_nr = s->tile_nr[2].p;
+    const size_t lds = (size_t)s->dn.nn_ncomp * sizeof(unsigned long long) + (size_t)4 * d.lmax * sizeof(T);
+    if (lds <= kLdsLimit) {
+      auto fn = k_density_nn<T, 4>;
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsLimit));
+      hipLaunchKernelGGL(fn, dim3(s->ntiles[2]), dim3(256), lds, nullptr, d, s->dn, psi, rnn.p, scale);
+    } else if (nP > 0) {
+      k_density_nn_g<T><<<nP, 256>>>(d, s->dn, psi, rnn.p, scale);
+    }
+  }
+  if (t.in.Z >= 1 && t.in.N >= 1 && nP > 0) {
+    const size_t lds = (size_t)5 * d.lmax * sizeof(T);
+    if (lds <= kLdsLimit) {
+      auto fn = k_density_pn<T, 4, false>;
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsLimit));
+      hipLaunchKernelGGL(fn, dim3(nP), dim3(256), lds, nullptr, d, s->dn, psi, rw.p, scale);
+    } else {
+      k_density_pn<T, 4, true><<<nP, 256>>>(d, s->dn, psi, rw.p, scale);
+    }
+  }
+  HIP_OK(hipGetLastError());
+  auto unfix = [&](const DevBuf<unsigned long long>& b, size_t n) {
+    std::vector<unsigned long long> u(n);
+    if (n) HIP_OK(hipMemcpy(u.data(), b.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = (double)(int64_t)u[i] / scale;
+    return v;
+  };
   if (norm2) *norm2 = n2;
-  if (rho_pp) {
-    if (npx > 0) HIP_OK(hipMemcpy(rho_pp, rpp.p, sizeof(double) * (size_t)npx * npx, hipMemcpyDeviceToHost));
+  if (rho_pp && npx > 0) {
+    const std::vector<double> v = unfix(rpp, (size_t)npx * npx);
+    std::copy(v.begin(), v.end(), rho_pp);
   }
   if (rho_nn) {
-    std::vector<double> comp(s->nn_comp_to_me.size());
-    if (!comp.empty()) HIP_OK(hipMemcpy(comp.data(), rnn.p, sizeof(double) * comp.size(), hipMemcpyDeviceToHost));
+    const std::vector<double> comp = unfix(rnn, s->nn_comp_to_me.size());
     std::fill(rho_nn, rho_nn + (size_t)npx * npx, 0.0);
     for (size_t i = 0; i < comp.size(); i++) rho_nn[s->nn_comp_to_me[i]] = comp[i];
   }
   if (rho_pn) {
-    std::vector<double> w(t.wpn.size());
-    if (!w.empty()) HIP_OK(hipMemcpy(w.data(), rw.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+    const std::vector<double> w = unfix(rw, t.wpn.size());
     const int ns = t.nsps, npn = ns * ns;
     std::fill(rho_pn, rho_pn + (size_t)npn * npn, 0.0);
     for (int a = 0; a < ns; a++)

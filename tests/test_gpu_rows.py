@@ -103,8 +103,24 @@ def test_long_rows_block_apply(wide):
         assert torch.allclose(Y[:, k], y, rtol=0, atol=1e-11 * float(y.abs().max()))
 
 
-def test_long_rows_gradient_is_refused(wide):
-    s, _, _ = wide["pf1p8n_m1"]
-    psi = torch.ones(s.dim, device="cuda", dtype=torch.float64) / np.sqrt(s.dim)
-    with pytest.raises(RuntimeError, match="4096"):
-        s.gradient(psi)
+@pytest.mark.parametrize("name", ["pf1p8n_m1", "sdg1p2n_m1", "pf0p8n"])
+def test_gradient_beyond_lds_and_deterministic(name, wide, tmp_path):
+    """The adjoint pass where its LDS staging does not fit (rows of 12,022
+    SDs; sdg's NN element set): H(theta) = sum_p theta_p H_p (no mass
+    scaling), so for any psi sum_p theta_p dE/dtheta_p = <psi|H|psi> /
+    <psi|psi>, with <psi|H|psi> from the row kernel; and the fixed-point sums
+    make a second pass bit-identical."""
+    from bigstickpublick_amd.inputs import small_case
+    import json, os
+    from golden_cases import GOLD
+    s, _, _ = wide[name]
+    meta = json.load(open(os.path.join(GOLD, name + ".json")))
+    case = small_case(meta["space"], meta["Z"], meta["N"], meta["jz2"], seed=meta["seed"])
+    theta = list(case.interaction.spe) + [t[6] for t in case.interaction.tbmes]
+    psi = torch.tensor(np.random.default_rng(8).standard_normal(s.dim), device="cuda")
+    g = s.gradient(psi)
+    assert len(g) == len(theta) and np.all(np.isfinite(g))
+    e = float(torch.dot(psi, s.apply(psi)) / torch.dot(psi, psi))
+    euler = float(np.dot(theta, g))
+    assert abs(euler - e) < 1e-5 * max(1.0, abs(e)), (euler, e)
+    assert np.array_equal(s.gradient(psi), g)
