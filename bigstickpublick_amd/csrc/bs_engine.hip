@@ -766,44 +766,80 @@ __global__ void k_combine(U* __restrict__ out, int64_t ldo, const T* __restrict_
 }
 
 // X[:, 0:ncol] = V[:, 0:m] Y[0:m, 0:ncol] (thick-restart rotation and Ritz
-// vectors, src/breorthog.f90:1211-1243) on the FP64 matrix cores: one wave per
-// strip of 16 rows, v_mfma_f64_16x16x4f64 over m in steps of 4, V converted to
-// FP64 as it is loaded, ncol <= 16.  Operand layouts of the 16x16x4 FP64
-// instruction: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, and D[i][j]
-// in lane j + 16*(i % 4), register i / 4.
+// vectors, src/breorthog.f90:1211-1243) on the FP64 matrix cores, ncol <= 16.
+// One wave per strip of 64 rows: lane (li, lk) = (lane % 16, lane / 16) loads
+// rows 4 li .. 4 li + 3 of vector k + lk as one contiguous chunk (16 lanes
+// cover the strip's 64 rows of a vector: whole 256-byte segments for FP32),
+// converts them to FP64 and feeds four v_mfma_f64_16x16x4f64 tiles, tile u
+// holding rows 4 i + u (i = 0..15).  Operand layouts of the instruction:
+// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
+// j + 16 (i % 4), register i / 4.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 template <typename T, typename U>
 __global__ __launch_bounds__(256) void k_combine_mfma(U* __restrict__ X, int64_t ldo, const T* __restrict__ V,
                                                       int64_t ld, int m, const double* __restrict__ Y, int ys,
                                                       int ncol, int64_t D) {
+  __shared__ double tile[4][16 * 64];
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int64_t nstrip = (D + 15) / 16;
+  const int64_t nstrip = (D + 63) / 64;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwave = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t st = wave; st < nstrip; st += nwave) {
-    const int64_t i0 = st * 16;
-    const int64_t row = i0 + li;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-    // four k-steps per pass: their loads are in flight together
-    for (int k0 = 0; k0 < m; k0 += 16) {
-      double a[4], b[4];
+    const int64_t i0 = st * 64;
+    const int64_t r0 = i0 + 4 * li;
+    const bool full = i0 + 64 <= D;
+    dbl4 acc[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int k = k0 + 4 * u + lk;
-        a[u] = (k < m && row < D) ? (double)V[(int64_t)k * ld + row] : 0.0;
-        b[u] = (k < m && li < ncol) ? Y[(int64_t)k * ys + li] : 0.0;
+    for (int u = 0; u < 4; u++) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // four k-steps per pass: their loads (kept in the vectors' own type) are
+    // in flight together
+    constexpr int KS = 4;
+    for (int k0 = 0; k0 < m; k0 += 4 * KS) {
+      T a[KS][4];
+      double b[KS];
+#pragma unroll
+      for (int s4 = 0; s4 < KS; s4++) {
+        const int k = k0 + 4 * s4 + lk;
+        b[s4] = (k < m && li < ncol) ? Y[(int64_t)k * ys + li] : 0.0;
+        const T* src = V + (int64_t)(k < m ? k : 0) * ld + r0;
+        if (k < m && full) {
+          if constexpr (sizeof(T) == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(src);
+            a[s4][0] = q.x; a[s4][1] = q.y; a[s4][2] = q.z; a[s4][3] = q.w;
+          } else {
+            const double2 q0 = *reinterpret_cast<const double2*>(src);
+            const double2 q1 = *reinterpret_cast<const double2*>(src + 2);
+            a[s4][0] = q0.x; a[s4][1] = q0.y; a[s4][2] = q1.x; a[s4][3] = q1.y;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; u++) a[s4][u] = (k < m && r0 + u < D) ? src[u] : T(0);
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+      for (int s4 = 0; s4 < KS; s4++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[s4][u], b[s4], acc[u], 0, 0, 0);
     }
+    // through this wave's LDS tile [column][64 rows], so that each column's
+    // 64 rows leave as one contiguous store per wave
+    double* tl = tile[threadIdx.x >> 6];
     if (li < ncol) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t i = i0 + lk + 4 * r;
-        if (i < D) X[(int64_t)li * ldo + i] = (U)acc[r];
-      }
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) tl[li * 64 + 4 * (lk + 4 * r) + u] = acc[u][r];
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (i0 + lane < D)
+      for (int j = 0; j < ncol; j++) X[(int64_t)j * ldo + i0 + lane] = (U)tl[j * 64 + lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -1884,7 +1920,7 @@ std::vector<double> tridiag_lowest(const std::vector<double>& a, const std::vect
 
 // one wave per 16-row strip, at most 2048 workgroups of 4 waves
 int mfma_grid(int64_t D) {
-  const int64_t waves = (D + 15) / 16;
+  const int64_t waves = (D + 63) / 64;
   return (int)std::min<int64_t>(std::max<int64_t>((waves + 3) / 4, 1), 2048);
 }
 
