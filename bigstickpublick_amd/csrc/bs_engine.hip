@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1624,6 +1625,24 @@ __global__ void k_bloss(const double* __restrict__ dots, int m, int K, int* __re
   if (dmax <= 1e-5) flags[k] = 0;
 }
 
+// single Lanczos run: step i needs the host's decision when beta_i is below
+// 0.1 of its scale (beta_{i-1}, or |H v_i| on a cycle's first step, with the
+// arrow couplings beta[0:kst]) or not finite; the first such step is kept
+__global__ void k_bflag(const double* __restrict__ alpha, const double* __restrict__ beta, int i, int kst,
+                        int* __restrict__ first) {
+  if (threadIdx.x != 0) return;
+  const double a = alpha[i], b = beta[i];
+  double scale;
+  if (i == kst) {
+    double s2 = a * a + b * b;
+    for (int j = 0; j < kst; j++) s2 += beta[j] * beta[j];
+    scale = sqrt(s2);
+  } else {
+    scale = beta[i - 1];
+  }
+  if (!(b >= 0.1 * scale) || b == 0.0 || !isfinite(a) || !isfinite(b)) atomicMin(first, i);
+}
+
 // w -= T(sum_j d_j V_j) and then w /= norm, for members with flags >= minflag
 template <typename T>
 __global__ void k_bmultiaxpy_m(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m,
@@ -2593,7 +2612,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t bV = up(sizeof(T) * (size_t)mtot * ld), bw = up(sizeof(T) * (size_t)ld);
     const size_t bp = up(sizeof(double) * (size_t)nb * mtot), bd = up(sizeof(double) * (size_t)mtot);
-    const size_t ba = up(sizeof(double) * (2 * (size_t)mtot + 2));
+    const size_t ba = up(sizeof(double) * (2 * (size_t)mtot + 4));
     const size_t need = bV + bw + bp + bd + ba;
     if (need > ext_bytes)
       return set_err("this run needs " + std::to_string(need) + " bytes of scratch; the no-allocation entry point has " +
@@ -2605,13 +2624,13 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     w.wrap(b + bV, ld);
     part.wrap(b + bV + bw, (size_t)nb * mtot);
     dots.wrap(b + bV + bw + bp, mtot);
-    ab.wrap(b + bV + bw + bp + bd, 2 * (size_t)mtot + 2);
+    ab.wrap(b + bV + bw + bp + bd, 2 * (size_t)mtot + 4);
   } else {
     V.alloc((size_t)mtot * ld);
     w.alloc(D);
     part.alloc((size_t)nb * mtot);
     dots.alloc(mtot);
-    ab.alloc(2 * (size_t)mtot + 2);  // alpha[0..], beta[mtot..]
+    ab.alloc(2 * (size_t)mtot + 4);  // alpha[0..], beta[mtot..], then the first flagged step (an int)
   }
   double* d_alpha = ab.p;
   double* d_beta = ab.p + mtot;
@@ -2641,7 +2660,6 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   // first step of a cycle has no previous beta; its scale is |H v_i|, which
   // is sqrt(alpha^2 + beta^2 + sum of the arrow couplings^2) there.  When the
   // Krylov space is the whole space (i + 1 == D) v_{i+1} is set to zero.
-  double beta_prev = 0.0;
   int random_restarts = 0, second_passes = 0;
   // convergence test parameters (src/blanczosmain.f90:1469-1512)
   if (p->conv_test != 0 && p->conv_test != 1) return set_err("conv_test must be 0 (mean energy change) or 1 (largest)");
@@ -2663,6 +2681,16 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     }
     return p->conv_test == 1 ? mx : sum / std::sqrt((double)nt + 1.0) * (1.0 + rst);
   };
+  // A step needs no host round trip: v_{i+1} = w / beta_i at once, and
+  // k_bflag records on the device the first step whose beta falls below 0.1
+  // of its scale (or is not finite).  Each time the host reads alpha/beta
+  // anyway (a convergence check, a restart, the end) it first reads that
+  // step; if there is one, the steps after it are discarded and it is redone
+  // with the decision on the host (next_vector_host) -- the same decisions,
+  // in the same order, as deciding at every step.
+  int* d_first = reinterpret_cast<int*>(ab.p + 2 * (size_t)mtot + 2);
+  auto reset_first = [&]() { k_fill<int><<<1, 64, 0, st>>>(d_first, INT_MAX, 1); };
+  reset_first();
   auto orthonormalize = [&](T* v, int m) {
     k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, m, v, D, part.p);
     k_reduce<<<m, kBlock, 0, st>>>(part.p, nb, m, dots.p, 0);
@@ -2671,25 +2699,33 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, dots.p + m, 1);
     k_normalize<T><<<g, kBlock, 0, st>>>(v, v, dots.p + m, D);
   };
-  auto next_vector = [&](int i, int kst, const std::vector<double>& arr) {
+  auto next_vector = [&](int i, int kst) {
     T* vn = V.p + (int64_t)(i + 1) * ld;
-    double ab[2];
-    HIP_OK(hipMemcpyAsync(&ab[0], d_alpha + i, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(&ab[1], d_beta + i, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    const double bi = ab[1];
-    if (!std::isfinite(bi) || !std::isfinite(ab[0]))
-      throw std::runtime_error("Lanczos step " + std::to_string(i) + ": alpha or beta is not finite");
-    double scale = beta_prev;
-    if (i == kst) {
-      double s2 = ab[0] * ab[0] + bi * bi;
-      for (int j = 0; j < kst; j++) s2 += arr[j] * arr[j];
-      scale = std::sqrt(s2);
-    }
-    beta_prev = bi;
     if (i + 1 >= D) {
       k_fill<T><<<g, kBlock, 0, st>>>(vn, T(0), D);
       return;
+    }
+    // the arrow couplings of a thick restart sit in beta[0:kst]
+    k_bflag<<<1, 64, 0, st>>>(d_alpha, d_beta, i, kst, d_first);
+    k_normalize<T><<<g, kBlock, 0, st>>>(vn, w.p, d_beta + i, D);
+  };
+  auto next_vector_host = [&](int i, int kst) {
+    T* vn = V.p + (int64_t)(i + 1) * ld;
+    double a = 0.0, bi = 0.0, bp = 0.0;
+    HIP_OK(hipMemcpy(&a, d_alpha + i, sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(&bi, d_beta + i, sizeof(double), hipMemcpyDeviceToHost));
+    if (!std::isfinite(bi) || !std::isfinite(a))
+      throw std::runtime_error("Lanczos step " + std::to_string(i) + ": alpha or beta is not finite");
+    double scale;
+    if (i == kst) {
+      std::vector<double> arr(kst);
+      if (kst) HIP_OK(hipMemcpy(arr.data(), d_beta, sizeof(double) * kst, hipMemcpyDeviceToHost));
+      double s2 = a * a + bi * bi;
+      for (int j = 0; j < kst; j++) s2 += arr[j] * arr[j];
+      scale = std::sqrt(s2);
+    } else {
+      HIP_OK(hipMemcpy(&bp, d_beta + i - 1, sizeof(double), hipMemcpyDeviceToHost));
+      scale = bp;
     }
     if (!(bi >= kSmallBeta * scale) || bi == 0.0) {
       k_pivot_random<T><<<g, kBlock, 0, st>>>(vn, D, 0x7a3f0e1d5c2bULL + 0x9E3779B9ULL * (uint64_t)(i + 1));
@@ -2701,7 +2737,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       random_restarts++;
       return;
     }
-    k_normalize<T><<<g, kBlock, 0, st>>>(vn, w.p, d_beta + i, D);
+    // v_{i+1} = w_i / beta_i is already in place (next_vector ran first; w
+    // itself has been overwritten by the later steps since)
     if (bi < 0.1 * scale) {
       const int m = i + 1;
       k_multidot<T><<<nb, kBlock, 0, st>>>(V.p, ld, m, vn, D, part.p);
@@ -2716,9 +2753,26 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       }
     }
   };
+  // before alpha/beta of steps < upto are used: the first step there that
+  // needs the host's decision, redone now (the caller resumes after it), or -1
+  int total_mv = 0;
+  auto redo_flagged = [&](int upto, int kst) -> int {
+    int f = INT_MAX;
+    HIP_OK(hipMemcpy(&f, d_first, sizeof(int), hipMemcpyDeviceToHost));
+    if (f >= upto) return -1;
+    total_mv -= upto - 1 - f;  // the discarded steps f+1 .. upto-1
+    reset_first();
+    next_vector_host(f, kst);
+    HIP_OK(hipGetLastError());
+    return f;
+  };
+  auto check_finite = [&](const std::vector<double>& a, const std::vector<double>& b, int n) {
+    for (int i = 0; i < n; i++)
+      if (!std::isfinite(a[i]) || !std::isfinite(b[i]))
+        throw std::runtime_error("Lanczos step " + std::to_string(i) + ": alpha or beta is not finite");
+  };
 
   std::vector<double> alpha_h, beta_h;  // alpha/beta of the final projected matrix
-  int total_mv = 0;
   int kstart = 0;             // number of locked (thick-restart) vectors at the front
   std::vector<double> arrow;  // coupling of the locked vectors to v_kstart
   int restarts = 0;
@@ -2739,7 +2793,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     }
     k_multidot<T><<<nb, kBlock, 0, st>>>(w.p, D, 1, w.p, D, part.p);
     k_reduce<<<1, kBlock, 0, st>>>(part.p, nb, 1, d_beta + i, 1);
-    next_vector(i, kstart, arrow);
+    next_vector(i, kstart);
     HIP_OK(hipGetLastError());
   };
   // Ritz pairs of the projected matrix on v_0 .. v_{m-1}: the locked Ritz
@@ -2751,6 +2805,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     beta_h.assign(m, 0.0);
     HIP_OK(hipMemcpy(alpha_h.data(), d_alpha, sizeof(double) * m, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(beta_h.data(), d_beta, sizeof(double) * m, hipMemcpyDeviceToHost));
+    check_finite(alpha_h, beta_h, m);
     std::vector<double> A((size_t)m * m, 0.0);
     for (int i = 0; i < m; i++) A[(size_t)i * m + i] = alpha_h[i];
     for (int i = kstart; i + 1 < m; i++) {
@@ -2775,23 +2830,32 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     // first ntest of them (sum |dE| / sqrt(ntest + 1), test 0) or their
     // largest change (test 1) is below tol
     std::vector<double> e_check;
-    for (int i = 0; i < niter; i++) {
+    int i = 0;
+    while (i < niter) {
       step(i);
-      if (p->tol > 0.0) {
-        const int iter = i + 1;
-        if (iter % itskip == 0 && iter > ncheck && iter < niter) {
-          std::vector<double> a(iter), b(iter);
-          HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * iter, hipMemcpyDeviceToHost));
-          HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * iter, hipMemcpyDeviceToHost));
-          std::vector<double> ek = tridiag_lowest(a, b, ncheck);
-          const bool stop = !e_check.empty() && energy_change(e_check, ek, ntest, 0) < p->tol;
-          e_check = ek;
-          if (stop) {
-            niter = iter;
-            break;
-          }
+      const int iter = i + 1;
+      const bool check = p->tol > 0.0 && iter % itskip == 0 && iter > ncheck && iter < niter;
+      if (check || iter == niter) {
+        const int f = redo_flagged(iter, kstart);
+        if (f >= 0 && f + 1 < iter) {
+          i = f + 1;
+          continue;
         }
       }
+      if (check) {
+        std::vector<double> a(iter), b(iter);
+        HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * iter, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * iter, hipMemcpyDeviceToHost));
+        check_finite(a, b, iter);
+        std::vector<double> ek = tridiag_lowest(a, b, ncheck);
+        const bool stop = !e_check.empty() && energy_change(e_check, ek, ntest, 0) < p->tol;
+        e_check = ek;
+        if (stop) {
+          niter = iter;
+          break;
+        }
+      }
+      i++;
     }
     m = niter;
     ritz(m);
@@ -2820,6 +2884,13 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       iter++;
       step(iter - 1);
       const bool fly = iter % itskip == 0;
+      if ((fly && iter > std::max(ncheck, nkeep)) || iter + 1 >= niter) {
+        const int f = redo_flagged(iter, kstart);
+        if (f >= 0 && f + 1 < iter) {
+          iter = f + 1;
+          continue;
+        }
+      }
       if ((fly && iter > std::max(ncheck, nkeep)) || iter + 1 == niter) {
         ritz(iter);
         for (int j = 0; j < ncheck; j++) ecur[j] = j < iter ? e[j] : 0.0;
