@@ -150,10 +150,11 @@ __device__ __forceinline__ u32x4 buf_u32x4(__amdgpu_buffer_rsrc_t r, uint32_t vo
 // (column q: entry count Eq[q], wave-uniform), four entries to a 16-byte
 // load ([e/4][column][4], qstep bytes per quad row).  The next quad (the
 // rest of this column, or the next column's first) is in flight while the
-// current one is applied; body(q, pk) applies one entry.
-template <int CPT, typename Body>
+// current one is applied; load(pk) reads one entry's operands, apply(q, e) adds
+// its products.
+template <int CPT, typename Load, typename Apply>
 __device__ __forceinline__ void walk_quads(__amdgpu_buffer_rsrc_t rpk, const uint32_t (&cl)[CPT], const int (&Eq)[CPT],
-                                           uint32_t qstep, Body&& body) {
+                                           uint32_t qstep, Load&& load, Apply&& apply) {
   u32x4 pq = {0u, 0u, 0u, 0u};
   if (Eq[0] > 0) pq = buf_u32x4(rpk, cl[0] * 16u, 0u);
 #pragma unroll
@@ -166,15 +167,19 @@ __device__ __forceinline__ void walk_quads(__amdgpu_buffer_rsrc_t rpk, const uin
     for (int e = 0; e < Ee; e += 4) {
       const u32x4 cur = pq;
       if (e + 4 < Ee) pq = buf_u32x4(rpk, cl[q] * 16u, (uint32_t)((e >> 2) + 1) * qstep);
-      if (Ee - e >= 4) {
-        body(q, cur.x);
-        body(q, cur.y);
-        body(q, cur.z);
-        body(q, cur.w);
+      // the reads of a quad (or of a pair, for a tail of one or two) are
+      // issued together, then its products; slots past Ee hold padding
+      // entries (a zero coefficient row), so a whole pair or quad is applied
+      if (Ee - e > 2) {
+        const auto l0 = load(cur.x), l1 = load(cur.y), l2 = load(cur.z), l3 = load(cur.w);
+        apply(q, l0);
+        apply(q, l1);
+        apply(q, l2);
+        apply(q, l3);
       } else {
-        body(q, cur.x);
-        if (Ee - e > 1) body(q, cur.y);
-        if (Ee - e > 2) body(q, cur.z);
+        const auto l0 = load(cur.x), l1 = load(cur.y);
+        apply(q, l0);
+        apply(q, l1);
       }
     }
     pq = pn;
@@ -190,6 +195,14 @@ template <> struct Vec16<double> { using type = double2; static constexpr int n 
 template <typename T, int G>
 struct alignas(sizeof(T) * G) VecG {
   T v[G];
+};
+
+// One neutron entry of a PN batch, as read from LDS: the G staged source
+// values at its source column and the G signed coefficients of its operator.
+template <typename T, int G>
+struct PnEntry {
+  VecG<T, G> x;
+  VecG<float, G> w;
 };
 
 // One workgroup per tile of up to R proton rows of one sector; BLOCK threads,
@@ -443,12 +456,16 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
             }
           }
         }
-        walk_quads<CPT>(rpk, cl, Eq, (uint32_t)L * 16u, [&](int q, uint32_t pk) {
-          const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
-          const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
+        walk_quads<CPT>(
+            rpk, cl, Eq, (uint32_t)L * 16u,
+            [&](uint32_t pk) {
+              return PnEntry<T, G>{*reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu)),
+                                   *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17))};
+            },
+            [&](int q, const PnEntry<T, G>& en) {
 #pragma unroll
-          for (int g = 0; g < G; g++) acc[g][q] = fma((T)wv.v[g], xv.v[g], acc[g][q]);
-        });
+              for (int g = 0; g < G; g++) acc[g][q] = fma((T)en.w.v[g], en.x.v[g], acc[g][q]);
+            });
       }
     }
   } else
@@ -576,12 +593,16 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
         T pa[CPT];
 #pragma unroll
         for (int q = 0; q < CPT; q++) pa[q] = T(0);
-        walk_quads<CPT>(rpk, cl, Eq, (uint32_t)L * 16u, [&](int q, uint32_t pk) {
-          const VecG<T, G> xv = *reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu));
-          const VecG<float, G> wv = *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17));
+        walk_quads<CPT>(
+            rpk, cl, Eq, (uint32_t)L * 16u,
+            [&](uint32_t pk) {
+              return PnEntry<T, G>{*reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu)),
+                                   *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17))};
+            },
+            [&](int q, const PnEntry<T, G>& en) {
 #pragma unroll
-          for (int g = 0; g < G; g++) pa[q] = fma((T)wv.v[g], xv.v[g], pa[q]);
-        });
+              for (int g = 0; g < G; g++) pa[q] = fma((T)en.w.v[g], en.x.v[g], pa[q]);
+            });
 #pragma unroll
         for (int q = 0; q < CPT; q++) acc[r][q] += pa[q];
       }
