@@ -4684,6 +4684,31 @@ int bs_batch_init(const char* sps, const char* const* intfiles, int K, int Z, in
   }
 }
 
+int bs_batch_set_interactions(bs_batch* b, const char* const* intfiles) {
+  try {
+    if (!b || !intfiles) return set_err("null argument");
+    for (int k = 0; k < b->K; k++)
+      if (!intfiles[k]) return set_err("null interaction file name");
+    const std::vector<std::string> old = b->intfiles;
+    const bool old_same = b->same_params;
+    b->intfiles.assign(intfiles, intfiles + b->K);
+    b->same_params = true;
+    try {
+      // every file is read and checked before any device buffer changes
+      batch_upload_values(b);
+    } catch (...) {
+      b->intfiles = old;
+      b->same_params = old_same;
+      throw;
+    }
+    b->work.reset();  // captured graphs hold the old value buffers
+    HIP_OK(hipDeviceSynchronize());
+    return 0;
+  } catch (const std::exception& e) {
+    return set_err(e.what());
+  }
+}
+
 int bs_batch_finalize(bs_batch* b) {
   delete b;
   return 0;

@@ -87,6 +87,7 @@ def load_library():
     L.bs_batch_apply.argtypes = [vp, vp, vp, i32]
     L.bs_batch_lanczos.argtypes = [vp, ctypes.POINTER(_Params), vp, vp, vp, vp, vp, ctypes.POINTER(i32)]
     L.bs_batch_gradient.argtypes = [vp, vp, i32, vp]
+    L.bs_batch_set_interactions.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p)]
     _lib = L
     return L
 
@@ -618,6 +619,16 @@ class Batch:
         if getattr(self, "h", None):
             self._lib.bs_batch_finalize(self.h)
             self.h = None
+
+    def set_interactions(self, intfiles):
+        """Take the values of K other .int files (same orbits, particle numbers
+        and scaling) on the same basis and jump tables, which are not rebuilt.
+        On an error the batch keeps its previous values."""
+        if len(intfiles) != self.K:
+            raise ValueError("need %d interaction files (got %d)" % (self.K, len(intfiles)))
+        names = (ctypes.c_char_p * self.K)(*[str(f).encode() for f in intfiles])
+        _check(load_library().bs_batch_set_interactions(self.h, names))
+        self.intfiles = list(intfiles)
 
     __del__ = close
 

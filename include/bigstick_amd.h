@@ -384,6 +384,17 @@ int bs_batch_size(const bs_batch *b, int *K, int64_t *dim);
 /* The system holding the shared basis and jump tables (values of the first
  * interaction); owned by the batch. */
 bs_system *bs_batch_system(bs_batch *b);
+/* Replace the K interactions' values by those of K other .int files on the
+ * same orbits, particle numbers and scaling, read as bs_batch_init reads
+ * them (src/binput_tbme.f90:898-1250 per file); the basis and jump tables
+ * are kept, so an optimisation loop over interaction parameters builds them
+ * once.  The tables are structural (every pair the orbits allow, zero
+ * values included), so any values fit them.  On an error (a file that cannot
+ * be read, a different single-particle term) the batch keeps its previous
+ * values.  Drops the captured Lanczos graphs; must not overlap other calls
+ * on the batch.  No counterpart in the reference, which reads one
+ * interaction per run. */
+int bs_batch_set_interactions(bs_batch *b, const char *const *intfiles);
 /* bs_multi_variant for the batch's own K (drops a captured Lanczos graph). */
 int bs_batch_set_variant(bs_batch *b, int dtype, int v);
 /* y_k = H_k x_k, interleaved device vectors. */

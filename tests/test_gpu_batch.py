@@ -224,3 +224,35 @@ def test_batch_gradient_vs_single_and_finite_differences(casedir):
         fd2 = (e[2 * (n + i)] - e[2 * (n + i) + 1]) / steps[n + i]
         assert abs(fd4 - g[0, p]) < 2e-3, (p, fd4, g[0, p])
         assert abs(fd2 - g[0, p]) < 5e-5, (p, fd2, g[0, p])
+
+
+def test_batch_set_interactions_equals_fresh_batch(casedir):
+    """New values on the kept basis and jump tables: members 4..7 of the
+    ensemble put into a batch built from members 0..3 give the same H x,
+    bit for bit, and the same Lanczos energies as a batch built from 4..7;
+    a file that cannot be read leaves the previous values in place."""
+    from bigstickpublick_amd import Batch
+    cases, d, fx = ensemble_fixture(casedir, 8)
+    first = [paths(c, d)[1] for c in cases[:4]]
+    second = [paths(c, d)[1] for c in cases[4:]]
+    b = Batch(os.path.join(d, "sd.sps"), first, 4, 4, 0)
+    fresh = Batch(os.path.join(d, "sd.sps"), second, 4, 4, 0)
+    x = torch.tensor(np.random.default_rng(12).standard_normal((b.dim, 4)), device="cuda", dtype=torch.float32)
+    y_first = b.apply(x)
+    r_first = b.lanczos(nkeep=2, niter=120, dtype="f64", polish=True)
+    b.set_interactions(second)
+    assert torch.equal(b.apply(x), fresh.apply(x))
+    r = b.lanczos(nkeep=2, niter=120, dtype="f64", polish=True)
+    want = fresh.lanczos(nkeep=2, niter=120, dtype="f64", polish=True)
+    print("set_interactions", r.evals, want.evals)
+    assert np.abs(r.evals - want.evals).max() < 1e-10
+    assert np.abs(r.evals - r_first.evals).max() > 1e-3  # different Hamiltonians
+    with pytest.raises(RuntimeError):
+        b.set_interactions(first[:3] + [os.path.join(d, "missing.int")])
+    assert torch.equal(b.apply(x), fresh.apply(x))
+    b.set_interactions(first)
+    assert torch.equal(b.apply(x), y_first)
+    with pytest.raises(ValueError):
+        b.set_interactions(first[:3])
+    b.close()
+    fresh.close()
