@@ -2619,6 +2619,9 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     return set_err("thick must be 0 or in [nkeep, niter - 2]");
   if (p->target < 0 || p->target > 2) return set_err("target must be 0 (lowest states), 1 or 2 (states nearest etarget)");
   if (p->target && p->thick <= 0) return set_err("targeting an energy needs a thick-restart run (thick > 0)");
+  // the run is synchronous: it starts after all earlier device work, on
+  // every stream (a pivot written on a caller's non-blocking stream included)
+  HIP_OK(hipDeviceSynchronize());
   const int mtot = niter + 1;
   const int nb = grid_for(D) < 1024 ? grid_for(D) : 1024;
   // Lanczos vectors at a stride of whole 64-element blocks: every v_i starts
@@ -3521,6 +3524,7 @@ int run_block_lanczos(bs_system* s, const bs_lanczos_params* p, double* evals, v
   if (nkeep < 1) return set_err("nkeep must be >= 1");
   if (D < K) return set_err("block larger than the basis");
   if (p->thick < 0) return set_err("thick must be >= 0");
+  HIP_OK(hipDeviceSynchronize());  // synchronous: after all earlier device work, every stream
   // thick restart (option tb): kb whole blocks of Ritz vectors kept; a
   // Krylov space as large as the basis needs none
   const int kb = (p->thick > 0 && p->niter < D) ? p->thick / K : 0;
@@ -4441,6 +4445,7 @@ int bs_observable(bs_system* s, int which, const void* psi, int dtype, double* v
     if (which != 0 && which != 1) return set_err("which must be 0 (J^2) or 1 (T^2)");
     if (!psi || !value) return set_err("null argument");
     if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
+    HIP_OK(hipDeviceSynchronize());  // psi may come from a caller's non-blocking stream
     if (!s->obs[which]) {
       std::unique_ptr<bs_system> o(new bs_system());
       o->t = bs::build_tables_text(s->t.in.orbits, bs::observable_int_text(s->t.in.orbits, which), s->t.in.Z,
@@ -4545,6 +4550,7 @@ int bs_density(bs_system* s, const void* psi, int dtype, double* rho_pp, double*
     checked(s);
     if (!s->uploaded) return set_err("tables not uploaded (call bs_upload)");
     if (!psi) return set_err("null vector");
+    HIP_OK(hipDeviceSynchronize());  // psi may come from a caller's non-blocking stream
     if (dtype == BS_F32) run_density<float>(s, (const float*)psi, rho_pp, rho_nn, rho_pn, norm2);
     else if (dtype == BS_F64) run_density<double>(s, (const double*)psi, rho_pp, rho_nn, rho_pn, norm2);
     else return set_err("dtype must be BS_F32 or BS_F64");

@@ -4,6 +4,12 @@
  * Plain C types only; every function returns 0 on success or a negative error
  * code (message in bs_last_error()).  Device pointers are HIP device memory on
  * the current device; `stream` is a hipStream_t (NULL = default stream).
+ * bs_apply_h and bs_apply_block are asynchronous on the caller's stream; the
+ * solvers and the calls that return host results (bs_lanczos, bs_strength,
+ * bs_block_lanczos, bs_density, bs_gradient, bs_observable, ...) are
+ * synchronous: they start after all earlier device work on every stream, so
+ * inputs written on a caller's non-blocking stream are complete, and their
+ * device outputs are complete when they return.
  * Calls on one system are not reentrant (the reference's apply routine is not
  * either, src/bapplyhlibOMP.f90:934).
  *
