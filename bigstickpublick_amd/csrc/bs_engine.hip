@@ -1310,18 +1310,24 @@ __global__ __launch_bounds__(256) void k_apply_rows(DevTables tb, const int32_t*
 // design (one workgroup per proton row, CPT columns per thread, per-chunk
 // entry bounds, signed coefficient tables, buffer-load table walks) with every
 // jump-table read, index decode and barrier serving K vectors.  Dynamic LDS:
-//   [0, kWbRows*G*K*4)  Wb[2*op + s][g][k], the batch's signed coefficients
-//                       of each Hamiltonian k
-//   [kWbRows*G*K*4, ..) staged rows [col][K] (NN) or [col][G][K] (PN)
+//   [0, kWbRows*G*WK*4)  Wb[2*op + s][g][k], the batch's signed coefficients
+//                        of each Hamiltonian k (WK = K), or of the one
+//                        Hamiltonian (SH, WK = 1)
+//   [kWbRows*G*WK*4, ..) staged rows [col][K] (NN) or [col][G][K] (PN)
 // Neutron entries come as src | (2*op + sign) << 16 and are scaled to LDS
 // byte offsets with two shifts.  Values per Hamiltonian (vs = K, vk = 1) or
-// shared (vs = 1, vk = 0).
-template <typename T, int BLOCK, int CPT, int K, int G>
+// shared (vs = 1, vk = 0; launched as SH when K > 1).
+template <typename T, int BLOCK, int CPT, int K, int G, bool SH>
 __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals bv, const T* __restrict__ x,
                                                        T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int GK = G * K;
-  constexpr int XS = kWbRows * GK * (int)sizeof(float);
+  // SH: one Hamiltonian for all K vectors (bv.vk == 0), so each coefficient
+  // row holds G values, not G*K copies: an entry reads 4*G bytes of
+  // coefficients beside its G*K staged source values
+  constexpr int WK = SH ? 1 : K;
+  constexpr int GW = G * WK;
+  constexpr int XS = kWbRows * GW * (int)sizeof(float);
   T* xs = reinterpret_cast<T*>(smem + XS);
   float* wb = reinterpret_cast<float*>(smem);
   const int tid = threadIdx.x;
@@ -1399,9 +1405,9 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
   }
   // ---- PN: per Delta class, batches of G source rows staged [col][G][K]
   using GV = VecG<T, GK>;
-  using WV = VecG<float, GK>;
+  using WV = VecG<float, GW>;
   constexpr int XSH = __builtin_ctz(GK * (int)sizeof(T));  // log2 bytes per staged column
-  constexpr int WSH = __builtin_ctz(GK * (int)sizeof(float));
+  constexpr int WSH = __builtin_ctz(GW * (int)sizeof(float));
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k0 = 0; k0 < nd; k0++) {
     const int p0 = tb.pcls[(int64_t)f * nd1 + k0], p1 = tb.pcls[(int64_t)f * nd1 + k0 + 1];
@@ -1439,8 +1445,8 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
           *reinterpret_cast<GV*>(xs + (size_t)c * GK) = v;
         }
       }
-      for (int i = tid; i < 2 * (wsd + 1) * GK; i += BLOCK) {
-        const int row = i / GK, r2 = i - row * GK, g = r2 / K, k = r2 - g * K, nl = row >> 1;
+      for (int i = tid; i < 2 * (wsd + 1) * GW; i += BLOCK) {
+        const int row = i / GW, r2 = i - row * GW, g = r2 / WK, k = r2 - g * WK, nl = row >> 1;
         float w = 0.0f;
         if (g < nb && nl < wsd) {
           const int meta = tb.p1_meta[b0 + g];
@@ -1468,7 +1474,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
 #pragma unroll
             for (int k = 0; k < K; k++)
 #pragma unroll
-              for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * K + k], xv.v[g * K + k], acc[q][k]);
+              for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * WK + (SH ? 0 : k)], xv.v[g * K + k], acc[q][k]);
           }
         }
         for (; e < Ee; e++) {
@@ -1478,7 +1484,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
 #pragma unroll
           for (int k = 0; k < K; k++)
 #pragma unroll
-            for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * K + k], xv.v[g * K + k], acc[q][k]);
+            for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * WK + (SH ? 0 : k)], xv.v[g * K + k], acc[q][k]);
         }
       }
     }
@@ -3303,16 +3309,25 @@ void launch_multi_kg(const bs_system* s, int map, const BatchVals& bv, const T* 
   d.tile_f0 = s->tile_f0[0].p;
   d.tile_nr = s->tile_nr[0].p;
   d.tile_map = s->tile_map[0][map].p;
-  const size_t lds = (size_t)G * K * d.lmax * sizeof(T) + (size_t)kWbRows * G * K * sizeof(float);
+  // one Hamiltonian on all K vectors: coefficients staged once, not K times
+  const bool sh = K > 1 && bv.vk == 0;
+  const size_t lds = (size_t)G * K * d.lmax * sizeof(T) + (size_t)kWbRows * G * (sh ? 1 : K) * sizeof(float);
   if (lds > kLdsLimit) throw std::runtime_error("multi-vector variant does not fit in LDS");
-  auto fn = k_apply_multi<T, kApplyBlock, CPT, K, G>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kLdsLimit));
-    attr_set = true;
+  auto launch = [&](auto fn, bool& attr_set) {
+    if (!attr_set) {
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kLdsLimit));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(s->nslots[0][map]), dim3(kApplyBlock), lds, st, d, bv, x, y);
+  };
+  static bool attr_sep = false, attr_sh = false;
+  if constexpr (K > 1) {
+    if (sh) launch(k_apply_multi<T, kApplyBlock, CPT, K, G, true>, attr_sh);
+    else launch(k_apply_multi<T, kApplyBlock, CPT, K, G, false>, attr_sep);
+  } else {
+    launch(k_apply_multi<T, kApplyBlock, CPT, K, G, false>, attr_sep);
   }
-  hipLaunchKernelGGL(fn, dim3(s->nslots[0][map]), dim3(kApplyBlock), lds, st, d, bv, x, y);
   HIP_OK(hipGetLastError());
 }
 
