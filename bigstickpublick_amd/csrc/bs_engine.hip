@@ -1306,6 +1306,14 @@ __global__ __launch_bounds__(256) void k_apply_rows(DevTables tb, const int32_t*
   y[bf + c] = accumulate ? y[bf + c] + acc : acc;
 }
 
+// One neutron entry of a K-vector PN batch, as read from LDS: the G*K staged
+// source values at its source column and its GW signed coefficients.
+template <typename T, int GK, int GW>
+struct MultiEntry {
+  VecG<T, GK> x;
+  VecG<float, GW> w;
+};
+
 // K interleaved vectors per basis state on large bases: the tile kernel's
 // design (one workgroup per proton row, CPT columns per thread, per-chunk
 // entry bounds, signed coefficient tables, buffer-load table walks) with every
@@ -1417,7 +1425,10 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
     const int Li = tb.nlen[tb.pconj[si]];
     const int wsd = tb.wstride[k0];
     const int64_t woff = tb.woff[k0];
-    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(tb.n1_pkr + tb.n1_base[(int64_t)cn * nd + kn]);
+    // the single-vector kernel's quad table for 4 staged FP32 rows: x offset
+    // src * 16 bytes (bits 0-16), coefficient offset (2*op + sign) * 16 bytes
+    // (bits 17-31), rescaled here to this kernel's column and row widths
+    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(tb.n1_pkv[2][0] + tb.n1q_base[(int64_t)cn * nd + kn]);
     const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
     int Eq[CPT];
 #pragma unroll
@@ -1456,37 +1467,21 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
         wb[i] = w;
       }
       __syncthreads();
-      const uint32_t step = (uint32_t)L * 4u;
-#pragma unroll
-      for (int q = 0; q < CPT; q++) {
-        const int Ee = Eq[q];
-        const uint32_t vo = cl[q] * 4u;
-        int e = 0;
-        // four entries per step, their loads in flight together
-        for (; e + 4 <= Ee; e += 4) {
-          uint32_t pk[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) pk[u] = buf_u32(rpk, vo, (uint32_t)(e + u) * step);
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const GV xv = *reinterpret_cast<const GV*>(smem + XS + ((pk[u] & 0xffffu) << XSH));
-            const WV wv = *reinterpret_cast<const WV*>(smem + ((pk[u] >> 16) << WSH));
+      walk_quads<CPT>(
+          rpk, cl, Eq, (uint32_t)L * 16u,
+          [&](uint32_t pk) {
+            const uint32_t xo = XSH >= 4 ? (pk & 0x1ffffu) << (XSH - 4) : (pk & 0x1ffffu) >> (4 - XSH);
+            const uint32_t wo = WSH >= 4 ? (pk >> 17) << (WSH - 4) : (pk >> 17) >> (4 - WSH);
+            return MultiEntry<T, GK, GW>{*reinterpret_cast<const GV*>(smem + XS + xo),
+                                         *reinterpret_cast<const WV*>(smem + wo)};
+          },
+          [&](int q, const MultiEntry<T, GK, GW>& en) {
 #pragma unroll
             for (int k = 0; k < K; k++)
 #pragma unroll
-              for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * WK + (SH ? 0 : k)], xv.v[g * K + k], acc[q][k]);
-          }
-        }
-        for (; e < Ee; e++) {
-          const uint32_t pk = buf_u32(rpk, vo, (uint32_t)e * step);
-          const GV xv = *reinterpret_cast<const GV*>(smem + XS + ((pk & 0xffffu) << XSH));
-          const WV wv = *reinterpret_cast<const WV*>(smem + ((pk >> 16) << WSH));
-#pragma unroll
-          for (int k = 0; k < K; k++)
-#pragma unroll
-            for (int g = 0; g < G; g++) acc[q][k] = fma((T)wv.v[g * WK + (SH ? 0 : k)], xv.v[g * K + k], acc[q][k]);
-        }
-      }
+              for (int g = 0; g < G; g++)
+                acc[q][k] = fma((T)en.w.v[g * WK + (SH ? 0 : k)], en.x.v[g * K + k], acc[q][k]);
+          });
     }
   }
 #pragma unroll
