@@ -1496,6 +1496,248 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
   }
 }
 
+// One Hamiltonian on K interleaved vectors with the single-vector kernel's
+// row tiles: a workgroup of 1024 threads takes two proton rows of one sector
+// (4 columns per thread, rows up to 4096 columns), accumulators
+// [row][column][k].  PP: the union of the two rows' source rows, each read
+// once with K-wide loads, two in flight.  NN: both rows staged [col][row][k].
+// PN: each batch stages one jump of each row ([col][slot][k]) and the signed
+// coefficients [2*op + s][slot] once for all K vectors; the neutron entries
+// come by quads from the four-row FP32 quad table (offsets rescaled), so an
+// entry reads 2*K staged values and 2 coefficients for 2*K FMAs.  Static LDS:
+// kWbRows * 2 floats, then 2 * K * sizeof(T) bytes per column.
+template <typename T, int K>
+__global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __restrict__ x, T* __restrict__ y) {
+  constexpr int R = 2, G = 2, BLOCK = 1024, CPT = 4;
+  constexpr int GK = G * K;
+  constexpr int XS = kWbRows * G * (int)sizeof(float);
+  __shared__ __align__(16) unsigned char smem[XS + GK * kMaxCols * sizeof(T)];
+  T* xs = reinterpret_cast<T*>(smem + XS);
+  float* wb = reinterpret_cast<float*>(smem);
+  using XV = VecG<T, K>;
+  using SV = VecG<T, GK>;
+  const int tid = threadIdx.x;
+  const int tile = tb.tile_map[blockIdx.x];
+  if (tile < 0) return;  // padding slot of the XCD map (uniform per workgroup)
+  const int f0 = tb.tile_f0[tile];
+  const int nr = tb.tile_nr[tile];
+  const int sp = tb.psec[f0];
+  const int cn = tb.pconj[sp];
+  const int L = tb.nlen[cn];
+  const int nf0 = tb.nfirst[cn];
+  uint32_t cl[CPT];  // clamped columns: loads never leave the row
+#pragma unroll
+  for (int q = 0; q < CPT; q++) cl[q] = (uint32_t)min(tid + q * BLOCK, L - 1);
+  int64_t bfr[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) bfr[r] = r < nr ? tb.pstart[f0 + r] : -1;
+  T acc[R][CPT][K];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int q = 0; q < CPT; q++)
+#pragma unroll
+      for (int k = 0; k < K; k++) acc[r][q][k] = T(0);
+  // ---- SPE
+  if (tb.call_spe) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (r < nr) {
+        const T ep = (T)tb.pspe_sd[f0 + r];
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+          const XV xv = *reinterpret_cast<const XV*>(x + (bfr[r] + cl[q]) * K);
+          const T e = ep + (T)tb.nspe_sd[nf0 + cl[q]];
+#pragma unroll
+          for (int k = 0; k < K; k++) acc[r][q][k] = fma(e, xv.v[k], acc[r][q][k]);
+        }
+      }
+    }
+  }
+  // ---- PP: the union of the tile rows' source rows, two in flight
+  {
+    const int64_t e0 = tb.tpp_ptr[tile], e1 = tb.tpp_ptr[tile + 1];
+    auto apply_src = [&](int64_t e, uint32_t sm, const XV (&xx)[CPT]) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (((sm >> 27) >> r) & 1u) {
+          const T v = vload<T>(tb.tpp_val32, tb.tpp_val64, e * R + r);
+#pragma unroll
+          for (int q = 0; q < CPT; q++)
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[r][q][k] = fma(v, xx[q].v[k], acc[r][q][k]);
+        }
+    };
+    int64_t e = e0;
+    for (; e + 2 <= e1; e += 2) {
+      const uint32_t sa = tb.tpp_src[e], sb = tb.tpp_src[e + 1];
+      const T* xa = x + tb.pstart[sa & 0x7ffffffu] * K;
+      const T* xb = x + tb.pstart[sb & 0x7ffffffu] * K;
+      XV ua[CPT], ub[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        ua[q] = *reinterpret_cast<const XV*>(xa + (int64_t)cl[q] * K);
+        ub[q] = *reinterpret_cast<const XV*>(xb + (int64_t)cl[q] * K);
+      }
+      apply_src(e, sa, ua);
+      apply_src(e + 1, sb, ub);
+    }
+    if (e < e1) {
+      const uint32_t sa = tb.tpp_src[e];
+      const T* xa = x + tb.pstart[sa & 0x7ffffffu] * K;
+      XV ua[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) ua[q] = *reinterpret_cast<const XV*>(xa + (int64_t)cl[q] * K);
+      apply_src(e, sa, ua);
+    }
+  }
+  // ---- NN: both rows staged [col][row][k]; one entry's value serves both
+  // rows and all K vectors
+  {
+    for (int c = tid; c < L; c += BLOCK) {
+      SV v;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        XV u;
+        if (bfr[r] >= 0) u = *reinterpret_cast<const XV*>(x + (bfr[r] + c) * K);
+        else
+#pragma unroll
+          for (int k = 0; k < K; k++) u.v[k] = T(0);
+#pragma unroll
+        for (int k = 0; k < K; k++) v.v[r * K + k] = u.v[k];
+      }
+      *reinterpret_cast<SV*>(xs + (size_t)c * GK) = v;
+    }
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(tb.nn_src + tb.nn_base[cn]);
+    const __amdgpu_buffer_rsrc_t rv =
+        sizeof(T) == 4 ? buf_rsrc(tb.nn_val32 + tb.nn_base[cn]) : buf_rsrc(tb.nn_val64 + tb.nn_base[cn]);
+    const int E = tb.nn_emax[cn];
+    for (int e = 0; e < E; e++) {
+      const uint32_t so = (uint32_t)e * (uint32_t)L * 4u;
+      int sl[CPT];
+      T v[CPT];
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        sl[q] = (int)buf_u32(rs, cl[q] * 4u, so);
+        if constexpr (sizeof(T) == 4) v[q] = __uint_as_float(buf_u32(rv, cl[q] * 4u, so));
+        else v[q] = buf_f64(rv, cl[q] * 8u, so * 2u);
+      }
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        const SV xv = *reinterpret_cast<const SV*>(xs + (size_t)sl[q] * GK);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int k = 0; k < K; k++) acc[r][q][k] = fma(v[q], xv.v[r * K + k], acc[r][q][k]);
+      }
+    }
+  }
+  // ---- PN: per Delta class, batch b holds jump b of each tile row (slot g =
+  // row g; a row with fewer jumps leaves its slot's coefficients zero)
+  constexpr int XSH = __builtin_ctz(GK * (int)sizeof(T));  // log2 bytes per staged column
+  constexpr int WSH = __builtin_ctz(G * (int)sizeof(float));
+  constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
+  const int nd = tb.ndelta, nd1 = nd + 1;
+  for (int k0 = 0; k0 < nd; k0++) {
+    int p0[R], nj[R];
+    int nbmax = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      p0[r] = nj[r] = 0;
+      if (r < nr) {
+        p0[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k0];
+        nj[r] = tb.pcls[(int64_t)(f0 + r) * nd1 + k0 + 1] - p0[r];
+        nbmax = max(nbmax, nj[r]);
+      }
+    }
+    if (nbmax == 0) continue;
+    const int kn = tb.negk[k0];
+    const int si = tb.srcsec[sp * nd + k0];
+    const int Li = tb.nlen[tb.pconj[si]];
+    const int wsd = tb.wstride[k0];
+    const float* Wk = tb.W + tb.woff[k0];
+    // the four-row FP32 quad table: x offset src * 16 bytes (bits 0-16),
+    // coefficient offset (2*op + sign) * 16 bytes (bits 17-31)
+    const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(tb.n1_pkv[2][0] + tb.n1q_base[(int64_t)cn * nd + kn]);
+    const uint8_t* ch = tb.n1_chunk + ((int64_t)cn * nd + kn) * 64 + __builtin_amdgcn_readfirstlane(tid >> 6);
+    int Eq[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) Eq[q] = ch[q * (BLOCK / 64)];
+    for (int b = 0; b < nbmax; b++) {
+      float wreg[kWbPer];
+#pragma unroll
+      for (int u = 0; u < kWbPer; u++) {
+        const int i = tid + u * BLOCK;
+        const int row = i / G, g = i - row * G, nl = row >> 1;
+        float w = 0.0f;
+        const int njg = g == 0 ? nj[0] : nj[1], p0g = g == 0 ? p0[0] : p0[1];
+        if (i < 2 * (wsd + 1) * G && b < njg && nl < wsd) {
+          const int meta = tb.p1_meta[p0g + b];
+          w = Wk[(meta & 0xff) * wsd + nl];
+          if ((meta >> 8) ^ (row & 1)) w = -w;
+        }
+        wreg[u] = w;
+      }
+      __syncthreads();
+      {
+        int64_t bi[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) bi[g] = b < nj[g] ? tb.pstart[tb.p1_src[p0[g] + b]] : -1;
+        for (int c = tid; c < Li; c += BLOCK) {
+          SV v;
+#pragma unroll
+          for (int g = 0; g < G; g++) {
+            XV u;
+            if (bi[g] >= 0) u = *reinterpret_cast<const XV*>(x + (bi[g] + c) * K);
+            else
+#pragma unroll
+              for (int k = 0; k < K; k++) u.v[k] = T(0);
+#pragma unroll
+            for (int k = 0; k < K; k++) v.v[g * K + k] = u.v[k];
+          }
+          *reinterpret_cast<SV*>(xs + (size_t)c * GK) = v;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kWbPer; u++) {
+        const int i = tid + u * BLOCK;
+        if (i < 2 * (wsd + 1) * G) wb[i] = wreg[u];
+      }
+      __syncthreads();
+      walk_quads<CPT>(
+          rpk, cl, Eq, (uint32_t)L * 16u,
+          [&](uint32_t pk) {
+            const uint32_t xo = XSH >= 4 ? (pk & 0x1ffffu) << (XSH - 4) : (pk & 0x1ffffu) >> (4 - XSH);
+            const uint32_t wo = (pk >> 17) >> (4 - WSH);
+            return MultiEntry<T, GK, G>{*reinterpret_cast<const SV*>(smem + XS + xo),
+                                        *reinterpret_cast<const VecG<float, G>*>(smem + wo)};
+          },
+          [&](int q, const MultiEntry<T, GK, G>& en) {
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+              for (int k = 0; k < K; k++) acc[g][q][k] = fma((T)en.w.v[g], en.x.v[g * K + k], acc[g][q][k]);
+          });
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    if (r < nr) {
+#pragma unroll
+      for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * BLOCK;
+        if (c < L) {
+          XV o;
+#pragma unroll
+          for (int k = 0; k < K; k++) o.v[k] = acc[r][q][k];
+          *reinterpret_cast<XV*>(y + (bfr[r] + c) * K) = o;
+        }
+      }
+    }
+  }
+}
+
 // part[b*m*K + j*K + k] = partial sum over this block's states of V_j[s][k] * w[s][k]
 template <typename T>
 __global__ __launch_bounds__(256) void k_bmultidot(const T* __restrict__ V, int64_t ldv, int m, const T* __restrict__ w,
@@ -3296,7 +3538,7 @@ int batch_blocks(int64_t D) { return (int)std::min<int64_t>(std::max<int64_t>((D
 // Multi-vector kernels: variant 0 = one wave per basis state (lane = vector;
 // small bases), 1 + 2*gi + map = k_apply_multi with G = 1 << gi staged rows
 // and identity (map 0) or XCD-aware (map 1) workgroup order.
-constexpr int kNumMulti = 5;
+constexpr int kNumMulti = 7;
 
 template <typename T, int CPT, int K, int G>
 void launch_multi_kg(const bs_system* s, int map, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
@@ -3344,12 +3586,30 @@ void launch_multi_cpt(const bs_system* s, int v, const BatchVals& bv, const T* x
   }
 }
 
-bool multi_fits(const bs_system* s, int v, int K, size_t tsize) {
+bool multi_fits(const bs_system* s, int v, int K, size_t tsize, bool shared) {
   if (v == 0) return true;
   if (!s->tiles_ok) return false;
+  // two-row tiles: one Hamiltonian, FP32 with K = 2 or 4, FP64 with K = 2
+  if (v >= 5) return shared && (K == 2 || (K == 4 && tsize == 4));
   if (K != 1 && K != 2 && K != 4 && K != 8) return false;
   const int G = (v - 1) / 2 == 0 ? 1 : 2;
   return (size_t)G * K * s->dt.lmax * tsize + (size_t)kWbRows * G * K * sizeof(float) <= kLdsLimit;
+}
+
+// k_apply_ktiles on the two-row tiles (map 0: sector order, 1: XCD-grouped)
+template <typename T, int K>
+void launch_ktiles(const bs_system* s, int map, const T* x, T* y, hipStream_t st) {
+  if (s->dt.lmax > kMaxCols) throw std::runtime_error("neutron rows too long for the two-row K-vector tiles");
+  DevTables d = s->dt;
+  d.tile_f0 = s->tile_f0[1].p;
+  d.tile_nr = s->tile_nr[1].p;
+  d.tile_map = s->tile_map[1][map].p;
+  d.tpp_ptr = s->tpp_ptr[1].p;
+  d.tpp_src = s->tpp_src[1].p;
+  d.tpp_val32 = s->tpp_val32[1].p;
+  d.tpp_val64 = s->tpp_val64[1].p;
+  hipLaunchKernelGGL((k_apply_ktiles<T, K>), dim3(s->nslots[1][map]), dim3(1024), 0, st, d, x, y);
+  HIP_OK(hipGetLastError());
 }
 
 template <typename T>
@@ -3360,6 +3620,13 @@ void launch_batched_v(const bs_system* s, int v, const BatchVals& bv, const T* x
     const int ny = (d.lmax + 3) / 4;
     hipLaunchKernelGGL(k_apply_batched<T>, dim3(d.nP, ny), dim3(256), 0, st, d, bv, x, y);
     HIP_OK(hipGetLastError());
+    return;
+  }
+  if (v >= 5) {
+    if (bv.vk != 0) throw std::runtime_error("the two-row K-vector tiles take one Hamiltonian");
+    if (bv.K == 2) launch_ktiles<T, 2>(s, v - 5, x, y, st);
+    else if (bv.K == 4 && sizeof(T) == 4) launch_ktiles<T, sizeof(T) == 4 ? 4 : 2>(s, v - 5, x, y, st);
+    else throw std::runtime_error("the two-row K-vector tiles take K = 2 (FP32, FP64) or 4 (FP32)");
     return;
   }
   switch (s->cpt) {
@@ -3390,7 +3657,7 @@ int batch_variant(bs_system* s, const BatchVals& bv) {
   float best = 1e30f;
   int bestv = 0;
   for (int v = 0; v < kNumMulti; v++) {
-    if (!multi_fits(s, v, bv.K, sizeof(T))) continue;
+    if (!multi_fits(s, v, bv.K, sizeof(T), bv.vk == 0)) continue;
     launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);
     HIP_OK(hipEventRecord(e0, nullptr));
     launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);
@@ -4648,7 +4915,8 @@ int bs_multi_variant(bs_system* s, int dtype, int K, int shared, int v) {
     if (dtype != BS_F32 && dtype != BS_F64) return set_err("dtype must be BS_F32 or BS_F64");
     if (K < 1 || K > kMaxBatch) return set_err("K must be in 1..64");
     if (v < 0 || v >= kNumMulti) return set_err("no such multi-vector variant");
-    if (!multi_fits(s, v, K, dtype == BS_F32 ? 4 : 8)) return set_err("variant does not fit for this system and K");
+    if (!multi_fits(s, v, K, dtype == BS_F32 ? 4 : 8, shared != 0))
+      return set_err("variant does not fit for this system and K");
     const int key = (dtype == BS_F64 ? 1000 : 0) + (shared ? 100 : 0) + K;
     s->multi_choice[key] = v;
     return 0;

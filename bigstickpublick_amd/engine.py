@@ -486,7 +486,7 @@ class System:
         return StrengthResult(ritz, stren, pn.value, ev, alpha, beta, res, it.value, vec)
 
     def set_block_variant(self, dtype, K, v):
-        """Force multi-vector kernel variant v (0..4) for apply_block with K
+        """Force multi-vector kernel variant v (0..6) for apply_block with K
         vectors; False if it does not fit."""
         self.upload()
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
@@ -639,7 +639,7 @@ class Batch:
         return a.value + b.value
 
     def set_variant(self, dtype, v):
-        """Force multi-vector kernel variant v (0..4); False if it does not fit."""
+        """Force multi-vector kernel variant v (0..6); False if it does not fit."""
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         return load_library().bs_batch_set_variant(self.h, dt, v) == 0
 
@@ -648,7 +648,7 @@ class Batch:
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         L = load_library()
         c = ctypes.c_int()
-        ms = np.zeros(5, np.float32)
+        ms = np.zeros(7, np.float32)
         _check(L.bs_multi_info(L.bs_batch_system(self.h), dt, self.K, 0, ctypes.byref(c), ms.ctypes.data))
         return c.value, ms
 

@@ -369,13 +369,15 @@ int bs_apply_block(bs_system *sys, const void *x, void *y, int K, int dtype);
 
 /* Multi-vector kernel variant for (dtype, K, shared: 1 = one Hamiltonian as
  * in bs_apply_block, 0 = one per vector as in bs_batch_apply): 0 = one wave
- * per basis state, 1..4 = row tiles with G = 1 (1, 2) or 2 (3, 4) staged
- * source rows, identity (odd) or XCD-aware (even) order.  Without a call the
- * fastest is timed and kept on first use.  Nonzero if it does not fit. */
+ * per basis state, 1..4 = one proton row per workgroup with G = 1 (1, 2) or 2
+ * (3, 4) staged source rows, identity (odd) or XCD-aware (even) order; 5, 6 =
+ * two proton rows per workgroup (shared only; FP32 K = 2 or 4, FP64 K = 2),
+ * identity (5) or XCD-aware (6) order.  Without a call the fastest is timed
+ * and kept on first use.  Nonzero if it does not fit. */
 int bs_multi_variant(bs_system *sys, int dtype, int K, int shared, int v);
 
 /* The variant in use for (dtype, K, shared) (-1 if none yet) and the
- * autotuner's timings in ms for variants 0..4 (0 = not timed). */
+ * autotuner's timings in ms for variants 0..6 (0 = not timed). */
 int bs_multi_info(const bs_system *sys, int dtype, int K, int shared, int *chosen, float *ms);
 
 /* K Hamiltonians on one basis: one iso .int file each, same orbits, particle

@@ -81,7 +81,7 @@ def test_block_apply_matches_single(casedir):
             cols = list(range(0, K, max(1, K // 4)))
             want = {k: o.apply(np.ascontiguousarray(X[:, k])) for k in cols}
             xd = torch.tensor(X, device="cuda")
-            for v in range(5):
+            for v in range(7):
                 if s.set_block_variant("f64", K, v):
                     Y = s.apply_block(xd).cpu().numpy()
                     for k in cols:
