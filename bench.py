@@ -56,6 +56,9 @@ def parse():
                     help="also time K Hamiltonians on the same basis (the workload's interaction and K-1 "
                          "more seeded ones) as one multi-vector matvec; with --solve, the batched "
                          "mixed-precision eigensolve with gradients, scored against FP64")
+    ap.add_argument("--block", type=int, default=0,
+                    help="also time the workload's H on K interleaved vectors (the block apply of block "
+                         "Lanczos) and report H x/s per vector")
     ap.add_argument("--variant", type=int, default=-1,
                     help="force this kernel variant (index into System.variants()) instead of autotuning")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
@@ -202,6 +205,31 @@ def solve_leg(s, case, max_restarts):
                       "abs_err_vs_scoring": err.tolist(), "within_1e-4": ok, "eigenpairs_per_s": ok / t}
     out["speedup_mixed_vs_fp64"] = out["mixed"]["eigenpairs_per_s"] / max(out["fp64"]["eigenpairs_per_s"], 1e-30)
     return out
+
+
+def block_leg(s, K, steps, warmup, g):
+    """One Hamiltonian on K interleaved FP32 vectors (bs_apply_block, the block
+    apply of block Lanczos): the autotuned multi-vector variant, its time per
+    launch and H x/s per vector."""
+    import torch
+    X = torch.randn(s.dim, K, device="cuda", dtype=torch.float32, generator=g)
+    Y = s.apply_block(X)  # picks the variant
+    for _ in range(warmup):
+        s.apply_block(X, Y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        s.apply_block(X, Y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    chosen, tms = s.block_variant_info("f32", K)
+    del X, Y
+    torch.cuda.empty_cache()
+    return {"K": K, "kernel_ms": ms, "hv_per_s_per_vector": K * 1000.0 / ms, "variant": chosen,
+            "autotune_ms": {str(v): round(float(t), 3) for v, t in enumerate(tms) if t > 0},
+            "what": "0 = one wave per state, 1-4 = one proton row per workgroup, 5-6 = two-row tiles"}
 
 
 def batch_leg(sps, case, K, steps, warmup, solve, max_restarts, work, n_ops):
@@ -518,6 +546,8 @@ def main():
                            "xcd_map": System.variants()[variant][2],
                            "prefetch": System.variants()[variant][3]},
     }
+    if a.block:
+        out["block"] = block_leg(s, a.block, a.steps, a.warmup, g)
     if a.solve or a.batch:
         del x, y
         x = y = None

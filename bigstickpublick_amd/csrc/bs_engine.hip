@@ -3658,6 +3658,10 @@ int batch_variant(bs_system* s, const BatchVals& bv) {
   int bestv = 0;
   for (int v = 0; v < kNumMulti; v++) {
     if (!multi_fits(s, v, bv.K, sizeof(T), bv.vk == 0)) continue;
+    // one wave per basis state only pays on small bases: where the tiles fit
+    // and the basis is large it is two orders of magnitude slower (8 s per
+    // launch on pf 6p6n), so it is not timed there
+    if (v == 0 && s->tiles_ok && s->t.dim > (int64_t)4000000) continue;
     launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);
     HIP_OK(hipEventRecord(e0, nullptr));
     launch_batched_v<T>(s, v, bv, x.p, y.p, nullptr);

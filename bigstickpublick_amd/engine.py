@@ -492,6 +492,15 @@ class System:
         dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
         return load_library().bs_multi_variant(self.h, dt, K, 1, v) == 0
 
+    def block_variant_info(self, dtype, K):
+        """(multi-vector variant in use for apply_block with K vectors or -1,
+        the autotuner's ms per variant)."""
+        dt = BS_F32 if dtype in ("f32", "float32") else BS_F64
+        c = ctypes.c_int()
+        ms = np.zeros(7, np.float32)
+        _check(load_library().bs_multi_info(self.h, dt, K, 1, ctypes.byref(c), ms.ctypes.data))
+        return c.value, ms
+
     def operator_matrix(self, intfile, vecs, scaling=(1.0, 1.0, 1.0, 1.0)):
         """<v_f|O|v_i> for the two-body operator in `intfile` (iso .int; None:
         this system's Hamiltonian) and an (nvec, dim) CUDA tensor of vectors;
