@@ -1184,6 +1184,8 @@ struct BatchVals {
   const float* W;      // [pn block element][vs]
   const float* pspe;   // [proton SD][vs]
   const float* nspe;   // [neutron SD][vs]
+  const float* kt32;   // one Hamiltonian per vector: [two-row PP union entry][row][K]
+  const double* kt64;
 };
 
 template <typename T> __device__ __forceinline__ T bval(const float* a, const double* b, int64_t i);
@@ -1506,11 +1508,14 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
 // come by quads from the four-row FP32 quad table (offsets rescaled), so an
 // entry reads 2*K staged values and 2 coefficients for 2*K FMAs.  Static LDS:
 // kWbRows * 2 floats, then 2 * K * sizeof(T) bytes per column.
-template <typename T, int K>
-__global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __restrict__ x, T* __restrict__ y) {
+template <typename T, int K, bool SH>
+__global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals bv, const T* __restrict__ x,
+                                                       T* __restrict__ y) {
   constexpr int R = 2, G = 2, BLOCK = 1024, CPT = 4;
   constexpr int GK = G * K;
-  constexpr int XS = kWbRows * G * (int)sizeof(float);
+  constexpr int WK = SH ? 1 : K;  // coefficients per slot: one Hamiltonian, or one per vector
+  constexpr int GW = G * WK;
+  constexpr int XS = kWbRows * GW * (int)sizeof(float);
   __shared__ __align__(16) unsigned char smem[XS + GK * kMaxCols * sizeof(T)];
   T* xs = reinterpret_cast<T*>(smem + XS);
   float* wb = reinterpret_cast<float*>(smem);
@@ -1543,13 +1548,15 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (r < nr) {
-        const T ep = (T)tb.pspe_sd[f0 + r];
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
           const XV xv = *reinterpret_cast<const XV*>(x + (bfr[r] + cl[q]) * K);
-          const T e = ep + (T)tb.nspe_sd[nf0 + cl[q]];
 #pragma unroll
-          for (int k = 0; k < K; k++) acc[r][q][k] = fma(e, xv.v[k], acc[r][q][k]);
+          for (int k = 0; k < K; k++) {
+            const T e = SH ? (T)tb.pspe_sd[f0 + r] + (T)tb.nspe_sd[nf0 + cl[q]]
+                           : (T)bv.pspe[(int64_t)(f0 + r) * K + k] + (T)bv.nspe[(int64_t)(nf0 + cl[q]) * K + k];
+            acc[r][q][k] = fma(e, xv.v[k], acc[r][q][k]);
+          }
         }
       }
     }
@@ -1561,11 +1568,15 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
 #pragma unroll
       for (int r = 0; r < R; r++)
         if (((sm >> 27) >> r) & 1u) {
-          const T v = vload<T>(tb.tpp_val32, tb.tpp_val64, e * R + r);
+          T v[K];
+#pragma unroll
+          for (int k = 0; k < K; k++)
+            v[k] = SH ? vload<T>(tb.tpp_val32, tb.tpp_val64, e * R + r)
+                      : bval<T>(bv.kt32, bv.kt64, (e * R + r) * K + k);
 #pragma unroll
           for (int q = 0; q < CPT; q++)
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[r][q][k] = fma(v, xx[q].v[k], acc[r][q][k]);
+            for (int k = 0; k < K; k++) acc[r][q][k] = fma(v[k], xx[q].v[k], acc[r][q][k]);
         }
     };
     int64_t e = e0;
@@ -1613,15 +1624,23 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
     const __amdgpu_buffer_rsrc_t rv =
         sizeof(T) == 4 ? buf_rsrc(tb.nn_val32 + tb.nn_base[cn]) : buf_rsrc(tb.nn_val64 + tb.nn_base[cn]);
     const int E = tb.nn_emax[cn];
+    const int64_t nb0 = tb.nn_base[cn];
     for (int e = 0; e < E; e++) {
       const uint32_t so = (uint32_t)e * (uint32_t)L * 4u;
       int sl[CPT];
-      T v[CPT];
+      T v[CPT][WK];
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
         sl[q] = (int)buf_u32(rs, cl[q] * 4u, so);
-        if constexpr (sizeof(T) == 4) v[q] = __uint_as_float(buf_u32(rv, cl[q] * 4u, so));
-        else v[q] = buf_f64(rv, cl[q] * 8u, so * 2u);
+        if constexpr (SH) {
+          if constexpr (sizeof(T) == 4) v[q][0] = __uint_as_float(buf_u32(rv, cl[q] * 4u, so));
+          else v[q][0] = buf_f64(rv, cl[q] * 8u, so * 2u);
+        } else {
+          // this slot's K values, one per Hamiltonian
+          const int64_t o = nb0 + (int64_t)e * L + cl[q];
+#pragma unroll
+          for (int k = 0; k < WK; k++) v[q][k] = bval<T>(bv.nn32, bv.nn64, o * K + k);
+        }
       }
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
@@ -1629,15 +1648,15 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-          for (int k = 0; k < K; k++) acc[r][q][k] = fma(v[q], xv.v[r * K + k], acc[r][q][k]);
+          for (int k = 0; k < K; k++) acc[r][q][k] = fma(v[q][SH ? 0 : k], xv.v[r * K + k], acc[r][q][k]);
       }
     }
   }
   // ---- PN: per Delta class, batch b holds jump b of each tile row (slot g =
   // row g; a row with fewer jumps leaves its slot's coefficients zero)
   constexpr int XSH = __builtin_ctz(GK * (int)sizeof(T));  // log2 bytes per staged column
-  constexpr int WSH = __builtin_ctz(G * (int)sizeof(float));
-  constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
+  constexpr int WSH = __builtin_ctz(GW * (int)sizeof(float));  // log2 bytes per coefficient row
+  constexpr int kWbPer = (kWbRows * GW + BLOCK - 1) / BLOCK;
   const int nd = tb.ndelta, nd1 = nd + 1;
   for (int k0 = 0; k0 < nd; k0++) {
     int p0[R], nj[R];
@@ -1657,6 +1676,7 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
     const int Li = tb.nlen[tb.pconj[si]];
     const int wsd = tb.wstride[k0];
     const float* Wk = tb.W + tb.woff[k0];
+    const float* Wb = bv.W + (int64_t)tb.woff[k0] * K;  // per Hamiltonian: [element][K]
     // the four-row FP32 quad table: x offset src * 16 bytes (bits 0-16),
     // coefficient offset (2*op + sign) * 16 bytes (bits 17-31)
     const __amdgpu_buffer_rsrc_t rpk = buf_rsrc(tb.n1_pkv[2][0] + tb.n1q_base[(int64_t)cn * nd + kn]);
@@ -1669,12 +1689,12 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
 #pragma unroll
       for (int u = 0; u < kWbPer; u++) {
         const int i = tid + u * BLOCK;
-        const int row = i / G, g = i - row * G, nl = row >> 1;
+        const int row = i / GW, r2 = i - row * GW, g = r2 / WK, k = r2 - g * WK, nl = row >> 1;
         float w = 0.0f;
         const int njg = g == 0 ? nj[0] : nj[1], p0g = g == 0 ? p0[0] : p0[1];
-        if (i < 2 * (wsd + 1) * G && b < njg && nl < wsd) {
+        if (i < 2 * (wsd + 1) * GW && b < njg && nl < wsd) {
           const int meta = tb.p1_meta[p0g + b];
-          w = Wk[(meta & 0xff) * wsd + nl];
+          w = SH ? Wk[(meta & 0xff) * wsd + nl] : Wb[(int64_t)((meta & 0xff) * wsd + nl) * K + k];
           if ((meta >> 8) ^ (row & 1)) w = -w;
         }
         wreg[u] = w;
@@ -1702,22 +1722,23 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, const T* __
 #pragma unroll
       for (int u = 0; u < kWbPer; u++) {
         const int i = tid + u * BLOCK;
-        if (i < 2 * (wsd + 1) * G) wb[i] = wreg[u];
+        if (i < 2 * (wsd + 1) * GW) wb[i] = wreg[u];
       }
       __syncthreads();
       walk_quads<CPT>(
           rpk, cl, Eq, (uint32_t)L * 16u,
           [&](uint32_t pk) {
             const uint32_t xo = XSH >= 4 ? (pk & 0x1ffffu) << (XSH - 4) : (pk & 0x1ffffu) >> (4 - XSH);
-            const uint32_t wo = (pk >> 17) >> (4 - WSH);
-            return MultiEntry<T, GK, G>{*reinterpret_cast<const SV*>(smem + XS + xo),
-                                        *reinterpret_cast<const VecG<float, G>*>(smem + wo)};
+            const uint32_t wo = WSH >= 4 ? (pk >> 17) << (WSH - 4) : (pk >> 17) >> (4 - WSH);
+            return MultiEntry<T, GK, GW>{*reinterpret_cast<const SV*>(smem + XS + xo),
+                                         *reinterpret_cast<const VecG<float, GW>*>(smem + wo)};
           },
-          [&](int q, const MultiEntry<T, GK, G>& en) {
+          [&](int q, const MultiEntry<T, GK, GW>& en) {
 #pragma unroll
             for (int g = 0; g < G; g++)
 #pragma unroll
-              for (int k = 0; k < K; k++) acc[g][q][k] = fma((T)en.w.v[g], en.x.v[g * K + k], acc[g][q][k]);
+              for (int k = 0; k < K; k++)
+                acc[g][q][k] = fma((T)en.w.v[g * WK + (SH ? 0 : k)], en.x.v[g * K + k], acc[g][q][k]);
           });
     }
   }
@@ -2219,6 +2240,8 @@ struct bs_system {
   std::unique_ptr<bs_system> obs[2];   // J^2 and T^2 on the same basis (built on demand)
   DevBuf<unsigned char> hostx, hosty;  // staging for bs_apply_h_host
   std::vector<int64_t> nn_slot;        // ELL slot of each merged NN entry (batched mode)
+  std::vector<int32_t> tpp_csr2;       // two-row PP union: [entry][row] merged PP entry, or -1
+  bool kt_ready = false;               // a batch on this system holds two-row union values
   std::map<int, int> multi_choice;     // multi-vector kernel variant per (dtype, shared values, K)
   std::map<int, std::array<float, 8>> multi_ms;  // its autotuner timings
   // adjoint tables (built on first bs_density)
@@ -2505,6 +2528,7 @@ void upload_tables(bs_system* s) {
       std::vector<float> v32;
       std::vector<double> v64;
       std::vector<std::pair<int32_t, int64_t>> ent;  // (source SD, entry e of row r: e * 4 + r)
+      if (R == 2) s->tpp_csr2.clear();
       for (int ti = 0; ti < nt; ti++) {
         ent.clear();
         for (int r = 0; r < nr[ti]; r++)
@@ -2518,6 +2542,7 @@ void upload_tables(bs_system* s) {
           const size_t b = v32.size();
           v32.resize(b + R, 0.0f);
           v64.resize(b + R, 0.0);
+          if (R == 2) s->tpp_csr2.insert(s->tpp_csr2.end(), {-1, -1});
           while (u < ent.size() && ent[u].first == sd) {
             // the row's merged value for this source (one entry per pair)
             const int r = (int)(ent[u].second & 3);
@@ -2525,6 +2550,10 @@ void upload_tables(bs_system* s) {
             v64[b + r] = T.val[e];
             v32[b + r] = (float)T.val[e];
             mask |= 1u << r;
+            if (R == 2) {
+              if (e > INT32_MAX) throw std::runtime_error("too many PP entries for the two-row union map");
+              s->tpp_csr2[s->tpp_csr2.size() - 2 + r] = (int32_t)e;
+            }
             u++;
           }
           src.push_back((uint32_t)sd | (mask << 27));
@@ -3508,8 +3537,8 @@ struct bs_batch {
   int K = 0;
   std::vector<std::string> intfiles;
   bool same_params = true;  // every interaction lists the same TBMEs as the first
-  DevBuf<float> pp32, nn32, W, pspe, nspe;
-  DevBuf<double> pp64, nn64;
+  DevBuf<float> pp32, nn32, W, pspe, nspe, kt32;
+  DevBuf<double> pp64, nn64, kt64;
   BatchVals bv{};
   // Lanczos workspace and its captured graphs: [0] the first Krylov cycle,
   // [1] a restarted cycle, [2] the thick restart, [3] Ritz vectors + polish
@@ -3589,8 +3618,9 @@ void launch_multi_cpt(const bs_system* s, int v, const BatchVals& bv, const T* x
 bool multi_fits(const bs_system* s, int v, int K, size_t tsize, bool shared) {
   if (v == 0) return true;
   if (!s->tiles_ok) return false;
-  // two-row tiles: one Hamiltonian, FP32 with K = 2 or 4, FP64 with K = 2
-  if (v >= 5) return shared && (K == 2 || (K == 4 && tsize == 4));
+  // two-row tiles: FP32 with K = 2 or 4, FP64 with K = 2 (one Hamiltonian,
+  // or one per vector with the batch's two-row union values)
+  if (v >= 5) return (K == 2 || (K == 4 && tsize == 4)) && (shared || s->kt_ready);
   if (K != 1 && K != 2 && K != 4 && K != 8) return false;
   const int G = (v - 1) / 2 == 0 ? 1 : 2;
   return (size_t)G * K * s->dt.lmax * tsize + (size_t)kWbRows * G * K * sizeof(float) <= kLdsLimit;
@@ -3598,7 +3628,7 @@ bool multi_fits(const bs_system* s, int v, int K, size_t tsize, bool shared) {
 
 // k_apply_ktiles on the two-row tiles (map 0: sector order, 1: XCD-grouped)
 template <typename T, int K>
-void launch_ktiles(const bs_system* s, int map, const T* x, T* y, hipStream_t st) {
+void launch_ktiles(const bs_system* s, int map, const BatchVals& bv, const T* x, T* y, hipStream_t st) {
   if (s->dt.lmax > kMaxCols) throw std::runtime_error("neutron rows too long for the two-row K-vector tiles");
   DevTables d = s->dt;
   d.tile_f0 = s->tile_f0[1].p;
@@ -3608,7 +3638,12 @@ void launch_ktiles(const bs_system* s, int map, const T* x, T* y, hipStream_t st
   d.tpp_src = s->tpp_src[1].p;
   d.tpp_val32 = s->tpp_val32[1].p;
   d.tpp_val64 = s->tpp_val64[1].p;
-  hipLaunchKernelGGL((k_apply_ktiles<T, K>), dim3(s->nslots[1][map]), dim3(1024), 0, st, d, x, y);
+  if (bv.vk == 0) {
+    hipLaunchKernelGGL((k_apply_ktiles<T, K, true>), dim3(s->nslots[1][map]), dim3(1024), 0, st, d, bv, x, y);
+  } else {
+    if (!bv.kt32 || !bv.kt64) throw std::runtime_error("two-row union values missing for this batch");
+    hipLaunchKernelGGL((k_apply_ktiles<T, K, false>), dim3(s->nslots[1][map]), dim3(1024), 0, st, d, bv, x, y);
+  }
   HIP_OK(hipGetLastError());
 }
 
@@ -3623,9 +3658,8 @@ void launch_batched_v(const bs_system* s, int v, const BatchVals& bv, const T* x
     return;
   }
   if (v >= 5) {
-    if (bv.vk != 0) throw std::runtime_error("the two-row K-vector tiles take one Hamiltonian");
-    if (bv.K == 2) launch_ktiles<T, 2>(s, v - 5, x, y, st);
-    else if (bv.K == 4 && sizeof(T) == 4) launch_ktiles<T, sizeof(T) == 4 ? 4 : 2>(s, v - 5, x, y, st);
+    if (bv.K == 2) launch_ktiles<T, 2>(s, v - 5, bv, x, y, st);
+    else if (bv.K == 4 && sizeof(T) == 4) launch_ktiles<T, sizeof(T) == 4 ? 4 : 2>(s, v - 5, bv, x, y, st);
     else throw std::runtime_error("the two-row K-vector tiles take K = 2 (FP32, FP64) or 4 (FP32)");
     return;
   }
@@ -3703,6 +3737,10 @@ void batch_upload_values(bs_batch* b) {
   std::vector<double> pp64(pp32.size(), 0.0), nn64(nn32.size(), 0.0);
   std::vector<float> W((size_t)std::max<int64_t>(nw, 1) * K, 0.0f);
   std::vector<float> ps((size_t)std::max<int64_t>(nP, 1) * K, 0.0f), ns((size_t)std::max<int64_t>(nN, 1) * K, 0.0f);
+  // the two-row tiles' PP union values, [entry][row][K]
+  const std::vector<int32_t>& csr2 = b->base->tpp_csr2;
+  std::vector<float> kt32(std::max<size_t>(csr2.size(), 1) * K, 0.0f);
+  std::vector<double> kt64(kt32.size(), 0.0);
   for (int k = 0; k < K; k++) {
     const bs::MeTables me = bs::build_me_tables(t, b->intfiles[k]);
     if (me.call_spe != t.call_spe) throw std::runtime_error("interactions disagree on the single-particle term");
@@ -3713,6 +3751,11 @@ void batch_upload_values(bs_batch* b) {
       pp64[(size_t)e * K + k] = vp[e];
       pp32[(size_t)e * K + k] = (float)vp[e];
     }
+    for (size_t i = 0; i < csr2.size(); i++)
+      if (csr2[i] >= 0) {
+        kt64[i * K + k] = vp[csr2[i]];
+        kt32[i * K + k] = (float)vp[csr2[i]];
+      }
     for (size_t q = 0; q < vn.size(); q++) {
       const int64_t o = b->base->nn_slot[q];
       nn64[(size_t)o * K + k] = vn[q];
@@ -3738,6 +3781,9 @@ void batch_upload_values(bs_batch* b) {
   b->W.upload(W);
   b->pspe.upload(ps);
   b->nspe.upload(ns);
+  b->kt32.upload(kt32);
+  b->kt64.upload(kt64);
+  b->base->kt_ready = !csr2.empty();
   BatchVals& v = b->bv;
   v.K = K;
   v.vs = K;
@@ -3749,6 +3795,8 @@ void batch_upload_values(bs_batch* b) {
   v.W = b->W.p;
   v.pspe = b->pspe.p;
   v.nspe = b->nspe.p;
+  v.kt32 = b->kt32.p;
+  v.kt64 = b->kt64.p;
 }
 
 BatchVals shared_vals(const bs_system* s, int K) {

@@ -371,8 +371,8 @@ int bs_apply_block(bs_system *sys, const void *x, void *y, int K, int dtype);
  * in bs_apply_block, 0 = one per vector as in bs_batch_apply): 0 = one wave
  * per basis state, 1..4 = one proton row per workgroup with G = 1 (1, 2) or 2
  * (3, 4) staged source rows, identity (odd) or XCD-aware (even) order; 5, 6 =
- * two proton rows per workgroup (shared only; FP32 K = 2 or 4, FP64 K = 2),
- * identity (5) or XCD-aware (6) order.  Without a call the fastest is timed
+ * two proton rows per workgroup (FP32 K = 2 or 4, FP64 K = 2; one Hamiltonian
+ * per vector only on a batch's system), identity (5) or XCD-aware (6) order.  Without a call the fastest is timed
  * and kept on first use.  Nonzero if it does not fit. */
 int bs_multi_variant(bs_system *sys, int dtype, int K, int shared, int v);
 

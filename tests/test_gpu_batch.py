@@ -41,6 +41,26 @@ def test_batch_apply_matches_oracle(casedir):
             assert _rel(Y32[:, k], want[k]) < 2e-5, (v, k)
 
 
+def test_batch_two_row_tiles_match_oracle(casedir):
+    """The two-row K-vector tiles (variants 5, 6) with one Hamiltonian per
+    vector: four members of the sd 4p4n ensemble, FP32 at K = 4 and FP64 at
+    K = 2, against the oracle."""
+    from bigstickpublick_amd import Batch
+    cases, d, fx = ensemble_fixture(casedir, 4)
+    for K, dt, tol in ((4, "f32", 2e-5), (2, "f64", 1e-12)):
+        b = Batch(os.path.join(d, "sd.sps"), [paths(c, d)[1] for c in cases[:K]], 4, 4, 0)
+        X = np.random.default_rng(40 + K).standard_normal((b.dim, K))
+        want = [Oracle(*paths(c, d), c.Z, c.N, c.jz2).apply(np.ascontiguousarray(X[:, k]))
+                for k, c in enumerate(cases[:K])]
+        xd = torch.tensor(X, device="cuda", dtype=torch.float32 if dt == "f32" else torch.float64)
+        for v in (5, 6):
+            assert b.set_variant(dt, v), (K, v)
+            Y = b.apply(xd).cpu().numpy()
+            for k in range(K):
+                assert _rel(Y[:, k], want[k]) < tol, (K, v, k)
+        b.close()
+
+
 def test_batch_apply_pf_matches_oracle(casedir):
     """Two pf Hamiltonians on the pf 4p4n basis (D = 1,963,461), every
     multi-vector variant, against the oracle."""
@@ -57,7 +77,7 @@ def test_batch_apply_pf_matches_oracle(casedir):
     want = [Oracle(*paths(c, d), c.Z, c.N, c.jz2).apply(np.ascontiguousarray(X[:, k]))
             for k, c in enumerate(cases)]
     xd = torch.tensor(X, device="cuda")
-    for v in range(5):
+    for v in range(7):
         if b.set_variant("f64", v):
             Y64 = b.apply(xd).cpu().numpy()
             for k in range(2):
