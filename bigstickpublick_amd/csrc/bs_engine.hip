@@ -1498,16 +1498,21 @@ __global__ __launch_bounds__(BLOCK) void k_apply_multi(DevTables tb, BatchVals b
   }
 }
 
-// One Hamiltonian on K interleaved vectors with the single-vector kernel's
-// row tiles: a workgroup of 1024 threads takes two proton rows of one sector
-// (4 columns per thread, rows up to 4096 columns), accumulators
-// [row][column][k].  PP: the union of the two rows' source rows, each read
-// once with K-wide loads, two in flight.  NN: both rows staged [col][row][k].
-// PN: each batch stages one jump of each row ([col][slot][k]) and the signed
-// coefficients [2*op + s][slot] once for all K vectors; the neutron entries
-// come by quads from the four-row FP32 quad table (offsets rescaled), so an
-// entry reads 2*K staged values and 2 coefficients for 2*K FMAs.  Static LDS:
-// kWbRows * 2 floats, then 2 * K * sizeof(T) bytes per column.
+// K interleaved vectors with the single-vector kernel's row tiles: a
+// workgroup of 1024 threads takes two proton rows of one sector (4 columns per
+// thread, rows up to 4096 columns), accumulators [row][column][k].  PP: the
+// union of the two rows' source rows, each read once with K-wide loads, two in
+// flight.  NN: both rows staged.  PN: each batch stages one jump of each row
+// (slot g) and the signed
+// coefficients [2*op + s][slot] (SH: once for all K vectors of the one
+// Hamiltonian; else [2*op + s][slot][k], one per vector's Hamiltonian); the
+// neutron entries come by quads from the four-row FP32 quad table (offsets
+// rescaled), so an entry reads 2*K staged values and 2 (or 2*K) coefficients
+// for 2*K FMAs.  With 32-byte columns (FP32 K = 4, FP64 K = 2) one array per
+// slot keeps consecutive lanes' reads at a 16-byte stride (a [col][slot][k]
+// layout would put 32 bytes between them and halve the LDS banks a pass can
+// use); 16-byte columns stay [col][slot][k], one read per entry.  Static LDS:
+// kWbRows * 2 (* K) floats, then 2 * K * sizeof(T) bytes per column.
 template <typename T, int K, bool SH>
 __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals bv, const T* __restrict__ x,
                                                        T* __restrict__ y) {
@@ -1516,8 +1521,15 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
   constexpr int WK = SH ? 1 : K;  // coefficients per slot: one Hamiltonian, or one per vector
   constexpr int GW = G * WK;
   constexpr int XS = kWbRows * GW * (int)sizeof(float);
-  __shared__ __align__(16) unsigned char smem[XS + GK * kMaxCols * sizeof(T)];
-  T* xs = reinterpret_cast<T*>(smem + XS);
+  // staged rows: slot (or tile row) g of column c at XS + g * SLOTB + c * COLB.
+  // Columns wider than 16 bytes are split into one array per slot, so that
+  // consecutive lanes' reads stride 16 bytes; narrower ones stay [col][slot][k]
+  // (one 16-byte read per entry)
+  constexpr int KB = K * (int)sizeof(T);
+  constexpr bool SPLIT = G * KB > 16;
+  constexpr int SLOTB = SPLIT ? kMaxCols * KB : KB;
+  constexpr int COLB = SPLIT ? KB : G * KB;
+  __shared__ __align__(16) unsigned char smem[XS + G * kMaxCols * KB];
   float* wb = reinterpret_cast<float*>(smem);
   using XV = VecG<T, K>;
   using SV = VecG<T, GK>;
@@ -1614,10 +1626,12 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
         else
 #pragma unroll
           for (int k = 0; k < K; k++) u.v[k] = T(0);
+        if constexpr (SPLIT) *reinterpret_cast<XV*>(smem + XS + r * SLOTB + c * COLB) = u;
+        else
 #pragma unroll
-        for (int k = 0; k < K; k++) v.v[r * K + k] = u.v[k];
+          for (int k = 0; k < K; k++) v.v[r * K + k] = u.v[k];
       }
-      *reinterpret_cast<SV*>(xs + (size_t)c * GK) = v;
+      if constexpr (!SPLIT) *reinterpret_cast<SV*>(smem + XS + c * COLB) = v;  // one wide store
     }
     __syncthreads();
     const __amdgpu_buffer_rsrc_t rs = buf_rsrc(tb.nn_src + tb.nn_base[cn]);
@@ -1644,7 +1658,17 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
       }
 #pragma unroll
       for (int q = 0; q < CPT; q++) {
-        const SV xv = *reinterpret_cast<const SV*>(xs + (size_t)sl[q] * GK);
+        SV xv;
+        if constexpr (SPLIT) {
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const XV u = *reinterpret_cast<const XV*>(smem + XS + r * SLOTB + sl[q] * COLB);
+#pragma unroll
+            for (int k = 0; k < K; k++) xv.v[r * K + k] = u.v[k];
+          }
+        } else {
+          xv = *reinterpret_cast<const SV*>(smem + XS + sl[q] * COLB);
+        }
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -1654,7 +1678,7 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
   }
   // ---- PN: per Delta class, batch b holds jump b of each tile row (slot g =
   // row g; a row with fewer jumps leaves its slot's coefficients zero)
-  constexpr int XSH = __builtin_ctz(GK * (int)sizeof(T));  // log2 bytes per staged column
+  constexpr int XSH = __builtin_ctz(COLB);  // log2 bytes between staged columns
   constexpr int WSH = __builtin_ctz(GW * (int)sizeof(float));  // log2 bytes per coefficient row
   constexpr int kWbPer = (kWbRows * GW + BLOCK - 1) / BLOCK;
   const int nd = tb.ndelta, nd1 = nd + 1;
@@ -1713,10 +1737,12 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
             else
 #pragma unroll
               for (int k = 0; k < K; k++) u.v[k] = T(0);
+            if constexpr (SPLIT) *reinterpret_cast<XV*>(smem + XS + g * SLOTB + c * COLB) = u;
+            else
 #pragma unroll
-            for (int k = 0; k < K; k++) v.v[g * K + k] = u.v[k];
+              for (int k = 0; k < K; k++) v.v[g * K + k] = u.v[k];
           }
-          *reinterpret_cast<SV*>(xs + (size_t)c * GK) = v;
+          if constexpr (!SPLIT) *reinterpret_cast<SV*>(smem + XS + c * COLB) = v;  // one wide store
         }
       }
 #pragma unroll
@@ -1730,8 +1756,19 @@ __global__ __launch_bounds__(1024) void k_apply_ktiles(DevTables tb, BatchVals b
           [&](uint32_t pk) {
             const uint32_t xo = XSH >= 4 ? (pk & 0x1ffffu) << (XSH - 4) : (pk & 0x1ffffu) >> (4 - XSH);
             const uint32_t wo = WSH >= 4 ? (pk >> 17) << (WSH - 4) : (pk >> 17) >> (4 - WSH);
-            return MultiEntry<T, GK, GW>{*reinterpret_cast<const SV*>(smem + XS + xo),
-                                         *reinterpret_cast<const VecG<float, GW>*>(smem + wo)};
+            MultiEntry<T, GK, GW> en;
+            if constexpr (SPLIT) {
+#pragma unroll
+              for (int g = 0; g < G; g++) {
+                const XV u = *reinterpret_cast<const XV*>(smem + XS + g * SLOTB + xo);
+#pragma unroll
+                for (int k = 0; k < K; k++) en.x.v[g * K + k] = u.v[k];
+              }
+            } else {
+              en.x = *reinterpret_cast<const SV*>(smem + XS + xo);
+            }
+            en.w = *reinterpret_cast<const VecG<float, GW>*>(smem + wo);
+            return en;
           },
           [&](int q, const MultiEntry<T, GK, GW>& en) {
 #pragma unroll
