@@ -56,9 +56,9 @@ def parse():
                     help="also time K Hamiltonians on the same basis (the workload's interaction and K-1 "
                          "more seeded ones) as one multi-vector matvec; with --solve, the batched "
                          "mixed-precision eigensolve with gradients, scored against FP64")
-    ap.add_argument("--block", type=int, default=0,
+    ap.add_argument("--block", type=int, default=4,
                     help="also time the workload's H on K interleaved vectors (the block apply of block "
-                         "Lanczos) and report H x/s per vector")
+                         "Lanczos) and report H x/s per vector (default K = 4; 0 skips it)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="force this kernel variant (index into System.variants()) instead of autotuning")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
