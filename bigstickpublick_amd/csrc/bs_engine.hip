@@ -2995,7 +2995,9 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
   // Krylov space is the whole space (i + 1 == D) v_{i+1} is set to zero.
   int random_restarts = 0, second_passes = 0;
   // convergence test parameters (src/blanczosmain.f90:1469-1512)
-  if (p->conv_test != 0 && p->conv_test != 1) return set_err("conv_test must be 0 (mean energy change) or 1 (largest)");
+  if (p->conv_test < 0 || p->conv_test > 3)
+    return set_err("conv_test must be 0 (mean energy change), 1 (largest), 2 (mean Ritz-vector change) or 3 (largest)");
+  if (p->conv_test >= 2 && p->target) return set_err("the Ritz-vector tests (conv_test 2, 3) take no target energy");
   if (p->ncheck_ex < 0) return set_err("ncheck_ex must be >= 0");
   const int itskip = std::max(10, (int)std::sqrt((double)nkeep));
   // a targeted run tests only the nkeep kept energies (src/blanczosmain.f90:1497-1499)
@@ -3013,6 +3015,29 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
       mx = std::max(mx, d);
     }
     return p->conv_test == 1 ? mx : sum / std::sqrt((double)nt + 1.0) * (1.0 + rst);
+  };
+  // tests 2 and 3 (convergence_test's altconverge branch,
+  // src/blanczoslib.f90:2741-2756): the first ncheck Ritz vectors of the
+  // projected matrix (m x m, Y[k*m + j] component k of vector j) against the
+  // previous solve's (mo x mo, zero beyond mo, as the reference's arrays are
+  // zeroed before each solve), over components k < iter - 1:
+  // vdot_j = sum_k Y[k][j] Yo[k][j]; test 2 sqrt(sum_j (|vdot_j| - 1)^2) /
+  // ncheck, test 3 max_j ||vdot_j| - 1|.  nvec: columns that hold vectors (a
+  // plain run of the reference solves for the lowest nkeep only,
+  // src/blanczoslib.f90:230-234; the others count as zero)
+  auto vector_change = [&](const std::vector<double>& Yn, int m, const std::vector<double>& Yo, int mo, int iter,
+                           int nvec) {
+    const int kmax = std::min(iter - 1, std::min(m, mo));
+    double s2 = 0.0, mx = 0.0;
+    for (int j = 0; j < ncheck; j++) {
+      double vd = 0.0;
+      if (j < nvec && j < m && j < mo)
+        for (int k = 0; k < kmax; k++) vd += Yn[(size_t)k * m + j] * Yo[(size_t)k * mo + j];
+      const double d = std::fabs(vd) - 1.0;
+      s2 += d * d;
+      mx = std::max(mx, std::fabs(d));
+    }
+    return p->conv_test == 3 ? mx : std::sqrt(s2) / ncheck;
   };
   // A step needs no host round trip: v_{i+1} = w / beta_i at once, and
   // k_bflag records on the device the first step whose beta falls below 0.1
@@ -3162,7 +3187,8 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     // stop when, against the previous such check, the mean change of the
     // first ntest of them (sum |dE| / sqrt(ntest + 1), test 0) or their
     // largest change (test 1) is below tol
-    std::vector<double> e_check;
+    std::vector<double> e_check, Y_check;  // the previous check's energies / Ritz vectors
+    int m_check = 0;
     int i = 0;
     while (i < niter) {
       step(i);
@@ -3180,9 +3206,22 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
         HIP_OK(hipMemcpy(a.data(), d_alpha, sizeof(double) * iter, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(b.data(), d_beta, sizeof(double) * iter, hipMemcpyDeviceToHost));
         check_finite(a, b, iter);
-        std::vector<double> ek = tridiag_lowest(a, b, ncheck);
-        const bool stop = !e_check.empty() && energy_change(e_check, ek, ntest, 0) < p->tol;
-        e_check = ek;
+        bool stop;
+        if (p->conv_test >= 2) {
+          std::vector<double> A((size_t)iter * iter, 0.0), ek, Yk;
+          for (int r = 0; r < iter; r++) A[(size_t)r * iter + r] = a[r];
+          for (int r = 0; r + 1 < iter; r++) A[(size_t)r * iter + r + 1] = A[(size_t)(r + 1) * iter + r] = b[r];
+          jacobi_eigen(iter, A, ek, Yk);
+          // (before the first check the earlier vectors are zero, as in the
+          // reference's arrays, so its dots are 0)
+          stop = vector_change(Yk, iter, Y_check, m_check, iter, nkeep) < p->tol;
+          Y_check = std::move(Yk);
+          m_check = iter;
+        } else {
+          std::vector<double> ek = tridiag_lowest(a, b, ncheck);
+          stop = !e_check.empty() && energy_change(e_check, ek, ntest, 0) < p->tol;
+          e_check = ek;
+        }
         if (stop) {
           niter = iter;
           break;
@@ -3211,6 +3250,7 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
     // the nkeep kept energies instead.
     const int ntest_t = p->target == 1 ? 0 : ncheck;
     std::vector<double> ecur(ncheck, 0.0), eold(ncheck, 0.0);
+    int m_solved = 0;  // size of the projected matrix of the last Ritz solve
     int iter = 0;
     bool finished = false;
     while (!finished) {
@@ -3224,20 +3264,33 @@ int run_lanczos(bs_system* s, const bs_lanczos_params* p, double* alpha_out, dou
           continue;
         }
       }
+      std::vector<double> Ylast;  // the Ritz vectors before this step's solve
+      int mlast = m_solved;
       if ((fly && iter > std::max(ncheck, nkeep)) || iter + 1 == niter) {
+        if (p->conv_test >= 2) Ylast = Y;
         ritz(iter);
+        m_solved = iter;
         for (int j = 0; j < ncheck; j++) ecur[j] = j < iter ? e[j] : 0.0;
       }
       if (iter == niter) {
         finished = true;
       } else if (p->tol > 0.0) {
-        if (iter > ncheck && fly && energy_change(eold, ecur, ntest_t, restarts) < p->tol) finished = true;
+        if (iter > ncheck && fly) {
+          if (p->conv_test >= 2) {
+            // a check compares with the vectors of the solve before it (zero
+            // before the first solve, as the reference's arrays)
+            if (m_solved == iter && vector_change(Y, iter, Ylast, mlast, iter, ncheck) < p->tol) finished = true;
+          } else if (energy_change(eold, ecur, ntest_t, restarts) < p->tol) {
+            finished = true;
+          }
+        }
         eold = ecur;
       }
       if (finished || iter + 1 != niter) continue;
       if (restarts > p->max_restarts) break;
       const int mr = iter - 1;
       ritz(mr);
+      m_solved = mr;
       for (int j = 0; j < ncheck; j++) ecur[j] = j < mr ? e[j] : 0.0;
       DevBuf<double> dY;
       std::vector<double> Yk((size_t)mr * thick);

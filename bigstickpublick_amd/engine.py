@@ -28,6 +28,11 @@ def load_library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libbigstick_amd.so is not built; run __graft_entry__.build()")
+    # torch's HIP runtime sets the device up before this library makes its
+    # first HIP call: in a process whose first HIP call comes from the library,
+    # torch finds no GPU afterwards
+    import torch
+    torch.cuda.is_available()
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64p = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)
     f32 = ctypes.c_float
@@ -353,7 +358,8 @@ class System:
     def lanczos(self, nkeep, niter, thick=0, max_restarts=0, dtype="f32", tol=0.0, polish=True,
                 want_vectors=False, conv_test=1, ncheck_ex=0, target=None, target_test="reference"):
         """Lanczos from the uniform pivot.  With tol > 0 the run stops by the
-        reference's energy test conv_test (0: mean change, 1: largest change)
+        reference's test conv_test (0: mean energy change, 1: largest energy
+        change, 2 / 3: mean / largest change of the Ritz vectors)
         over nkeep + ncheck_ex Ritz values (the reference's ld is conv_test=0,
         ncheck_ex=5, tol=1e-3).  target (MeV, thick > 0): keep the Ritz
         pairs nearest that energy instead of the lowest (option tx);

@@ -199,12 +199,12 @@ contains
     if (lopt(2:2) == 'c' .or. lopt(2:2) == 'C') then
       ! lc / bc / tc: [additional states for the test,] test choice, tolerance
       ! (src/blanczosmain.f90:943-985, :873-921, :1117-1175): the energy tests
-      ! 0 (mean change) and 1 (largest change); the wave-function tests 2 and
-      ! 3 are not supported
+      ! 0 (mean change) and 1 (largest change), and for lc / tc the Ritz-vector
+      ! tests 2 (mean change) and 3 (largest change)
       if (lopt(1:1) /= 't') read(5, *) ncheck
       read(5, *) ctest
-      if (ctest /= 0 .and. ctest /= 1) then
-        write(6, *) ' bs_driver: only the energy convergence tests (0, 1) are supported'
+      if (ctest < 0 .or. ctest > 3 .or. (lopt(1:1) == 'b' .and. ctest > 1)) then
+        write(6, *) ' bs_driver: convergence test not supported here (0-3 for lc and tc, 0-1 for bc)'
         stop 1
       end if
       read(5, *) ctol

@@ -188,7 +188,13 @@ typedef struct {
                        run takes the energies at every such step of a cycle beyond max(ncheck,
                        nkeep) and at iter = niter - 1, and compares at every such step beyond
                        ncheck with the previous iteration's energies (a restart's own Ritz
-                       values after a restart). */
+                       values after a restart).  2, 3 = the reference's Ritz-vector tests
+                       (altconverge, src/blanczoslib.f90:2741-2756): vdot_j of the first ncheck
+                       Ritz vectors of the projected matrix with the previous solve's (components
+                       below iter - 1, zero beyond the earlier size), 2 = sqrt(sum_j (|vdot_j| -
+                       1)^2) / ncheck, 3 = max_j ||vdot_j| - 1|; a plain run has vectors for the
+                       lowest nkeep only, as the reference (the ncheck_ex extra ones count as
+                       zero, so with ncheck_ex > 0 the test never passes); not with a target. */
   int ncheck_ex;    /* additional Ritz values in the test (the reference's ld uses 5) */
   int target;       /* 0: the lowest states.  1, 2: the Ritz pairs nearest etarget (option tx,
                        txchoice 1, src/blanczoslib.f90:277-327; thick > 0): every Ritz solve orders

@@ -43,20 +43,43 @@ def convergence_stop(alpha, beta, nkeep, ncheck_ex=5, tol=1e-3, test=0, niter=No
     itskipeig-th iteration (10, or int(sqrt(nkeep)) if larger) beyond ncheck =
     nkeep + ncheck_ex, the lowest ncheck eigenvalues of the tridiagonal so far
     are compared with those of the previous such check; test 0 stops when
-    sum |dE| / sqrt(ncheck + 1) < tol, test 1 when max |dE| < tol.  Returns the
-    stopping iteration (1-based): niter when the test never passes before it,
-    None when the coefficients end before either."""
+    sum |dE| / sqrt(ncheck + 1) < tol, test 1 when max |dE| < tol.  Tests 2
+    and 3 (altconverge, src/blanczoslib.f90:2741-2756) compare the check's
+    lowest ncheck eigenvectors of the tridiagonal with the previous check's
+    instead: vdot_j = sum over components k < it - 1 of the two (the earlier
+    vector zero beyond its length: find_lanczos_eigenvalues zeroes the array
+    before each solve, src/blanczoslib.f90:196), test 2 stops when
+    sqrt(sum_j (|vdot_j| - 1)^2) / ncheck < tol, test 3 when max_j
+    ||vdot_j| - 1| < tol.  A plain run solves for the lowest nkeep
+    eigenvectors only (DSYEVR over indices 1..nkeep, src/blanczoslib.f90:
+    230-234, with use_switch_esolver on by default), so the ncheck_ex extra
+    columns are zero, their vdot is 0, and with ncheck_ex > 0 tests 2 and 3
+    never pass.  Returns the stopping iteration (1-based): niter when the test
+    never passes before it, None when the coefficients end before either."""
     niter = len(alpha) if niter is None else niter
     itskip = max(10, int(np.sqrt(nkeep)))
     ncheck = nkeep + ncheck_ex
     ntest = ncheck if nkeep <= 100 else int(np.sqrt(ncheck) + 1)
     eold = None
+    yold = None
     for it in range(1, len(alpha) + 1):
         if it >= niter:
             return niter
         if it % itskip or it <= ncheck:
             continue
         T = np.diag(alpha[:it]) + np.diag(beta[:it - 1], 1) + np.diag(beta[:it - 1], -1)
+        if test in (2, 3):
+            e, V = np.linalg.eigh(T)
+            y = V[:, :ncheck].copy()
+            y[:, nkeep:] = 0.0
+            if yold is not None:
+                n = min(it - 1, yold.shape[0])
+                vd = np.abs(np.einsum("kj,kj->j", y[:n], yold[:n])) - 1.0
+                v = np.sqrt((vd ** 2).sum()) / ncheck if test == 2 else np.abs(vd).max()
+                if v < tol:
+                    return it
+            yold = y
+            continue
         e = np.linalg.eigvalsh(T)[:ncheck]
         if eold is not None:
             d = np.abs(eold[:ntest] - e[:ntest])
@@ -104,7 +127,9 @@ def thick_lanczos(apply, dim, nkeep, niter, nthick, max_restarts, tol=0.0, test=
     nkeep), and at iter = niter - 1; at every multiple beyond ncheck (ncheck
     = nkeep + ncheck_ex, nkeep for a targeted run) the test compares them
     with the previous iteration's: test 0 sum |dE| / sqrt(ntest + 1) times
-    (1 + restarts), test 1 max |dE|, over the first ntest values.  At iter =
+    (1 + restarts), test 1 max |dE|, over the first ntest values; tests 2 and
+    3 compare the first ncheck Ritz vectors with the previous solve's instead
+    (as convergence_stop does; every vector exists in a thick run).  At iter =
     niter - 1 a run that has not converged restarts -- after max_restarts + 1
     restarts it stops instead: the matrix on the first niter - 2 vectors is
     diagonalized, its nthick leading Ritz vectors replace v_1 .. v_nthick,
@@ -148,8 +173,21 @@ def thick_lanczos(apply, dim, nkeep, niter, nthick, max_restarts, tol=0.0, test=
             return 0.0
         return d.max() if test == 1 else d.sum() / np.sqrt(ntest + 1) * (1 + restarts)
 
+    def vchange(Yn, Yo, it):
+        # tests 2, 3 (src/blanczoslib.f90:2741-2756): the first ncheck Ritz
+        # vectors against the previous solve's, components below it - 1, the
+        # earlier ones zero beyond their length (none before the first solve)
+        vd = np.zeros(ncheck)
+        if Yo is not None:
+            n = min(it - 1, Yn.shape[0], Yo.shape[0])
+            c = min(ncheck, Yn.shape[1], Yo.shape[1])
+            vd[:c] = np.einsum("kj,kj->j", Yn[:n, :c], Yo[:n, :c])
+        d = np.abs(vd) - 1.0
+        return np.abs(d).max() if test == 3 else np.sqrt((d ** 2).sum()) / ncheck
+
     ecur = np.zeros(ncheck)
     eold = np.zeros(ncheck)
+    ylast = None  # Ritz vectors of the last solve
     it = 0
     finished = False
     while not finished:
@@ -166,14 +204,18 @@ def thick_lanczos(apply, dim, nkeep, niter, nthick, max_restarts, tol=0.0, test=
         beta[i] = b
         V[i + 1] = w / b
         fly = it % itskip == 0
+        yprev = ylast
         if (fly and it > max(ncheck, nkeep)) or it + 1 == niter:
-            e, _ = ritz(it)
+            e, ylast = ritz(it)
             ecur = np.array([e[j] if j < it else 0.0 for j in range(ncheck)])
         if it == niter:
             finished = True
         elif tol > 0.0:
-            if it > ncheck and fly and change(eold, ecur) < tol:
-                finished = True
+            if it > ncheck and fly:
+                if test in (2, 3):
+                    finished = vchange(ylast, yprev, it) < tol
+                elif change(eold, ecur) < tol:
+                    finished = True
             eold = ecur.copy()
         if finished or it + 1 != niter:
             continue
@@ -181,6 +223,7 @@ def thick_lanczos(apply, dim, nkeep, niter, nthick, max_restarts, tol=0.0, test=
             break
         m = it - 1
         e, Y = ritz(m)
+        ylast = Y
         ecur = np.array([e[j] if j < m else 0.0 for j in range(ncheck)])
         keep = Y[:, :nthick].T @ V[:m]
         V[nthick] = V[m].copy()

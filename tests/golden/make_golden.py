@@ -379,6 +379,34 @@ def converge(name, option, answers):
         shutil.rmtree(wd)
 
 
+# Option lc with the wave-function tests (converge_test 2, 3): (case, label,
+# answers after lc: nkeep maxiter, ncheck_ex, test, tol)
+ALTCONV = [("sd2p2n", "lc2", ["5 300", "0", "2", "1e-4"]),
+           ("sd2p2n", "lc3", ["5 300", "0", "3", "1e-4"]),
+           ("pf2p1n_m3", "lc3", ["5 300", "0", "3", "1e-5"]),
+           ("sd1p2n_m1", "lc2x", ["5 300", "2", "2", "1e-4"])]
+
+
+def altconverge(name, label, answers):
+    """The reference's option lc run with a wave-function convergence test
+    (convergence_test's altconverge branch, src/blanczoslib.f90:2741-2756),
+    kept as <name>_<label>.lcoef: its length is where the test stopped the
+    run (or the iteration cap)."""
+    meta = [c for c in SMALL if c[0] == name][0]
+    case = small_case(*meta[1:5], seed=meta[5])
+    case.name = name
+    lines = ["n", name, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), name, "1 1 1 1", "end",
+             "lc"] + answers
+    wd = tempfile.mkdtemp(prefix="bsgold_")
+    try:
+        write_case(case, wd)
+        run_ref(wd, "\n".join(lines) + "\n")
+        shutil.copy(os.path.join(wd, name + ".lcoef"), os.path.join(HERE, "%s_%s.lcoef" % (name, label)))
+        print(name, label)
+    finally:
+        shutil.rmtree(wd)
+
+
 def pivot_spectrum():
     """The reference's menu np with option lf (src/bmenu.f90:700-707): 40
     Lanczos iterations on sd 2p2n from state 1 of sd2p2n_pivot.wfn, kept as
@@ -547,7 +575,11 @@ THICK = [("pf2p1n_m3", "tf", ["4 24", "10", "3"]),
          ("sd1p2n_m1", "td", ["4 20", "40"]),
          ("sd1p2n_m1", "tx", ["4 30", "12", "200", "-16.2"]),
          ("pf2p1n_m3", "tx", ["4 30", "12", "200", "-26.5"]),
-         ("pf2p1n_m3", "tb", ["4 4 4", "8", "50"])]
+         ("pf2p1n_m3", "tb", ["4 4 4", "8", "50"]),
+         # tc with the Ritz-vector tests (2: mean change, 3: largest), kept as
+         # <name>_tc2 / _tc3
+         ("sd2p2n", "tc", ["5 30", "12", "40", "2", "1e-4"], "tc2"),
+         ("pf2p1n_m3", "tc", ["5 30", "12", "40", "3", "1e-4"], "tc3")]
 
 
 def thick_cycles(only=None):
@@ -560,8 +592,10 @@ def thick_cycles(only=None):
     random pivot block: only its converged .res is comparable): .res and, but
     for tb, .lcoef (the last cycle's coefficients after the kept Ritz values
     and couplings) kept as <name>_<option>_fp64.*."""
-    for name, option, answers in THICK:
-        if only and option not in only:
+    for entry in THICK:
+        name, option, answers = entry[:3]
+        label = entry[3] if len(entry) > 3 else option
+        if only and label not in only:
             continue
         meta = [c for c in SMALL if c[0] == name][0]
         case = small_case(*meta[1:5], seed=meta[5])
@@ -569,7 +603,7 @@ def thick_cycles(only=None):
         wd = tempfile.mkdtemp(prefix="bsgold_")
         try:
             write_case(case, wd)
-            out = "%s_%s_fp64" % (name, option)
+            out = "%s_%s_fp64" % (name, label)
             script = "\n".join(["n", out, case.space, "%d %d" % (case.Z, case.N), str(case.jz2), case.name,
                                 "1 1 1 1", "end", option] + answers) + "\n"
             run_ref(wd, script, exe=REF64)
@@ -821,12 +855,17 @@ if __name__ == "__main__":
     for c in CONVERGE:
         if "converge" in want:
             converge(*c)
+    for c in ALTCONV:
+        if "altconverge" in want:
+            altconverge(*c)
     if "pivot4" in want:
         pivot4_wfn()
     if "thick" in want:
         thick_cycles()
     if "tb" in want:
         thick_cycles(only=("tb",))
+    if "tc23" in want:
+        thick_cycles(only=("tc2", "tc3"))
     if "long" in want:
         long_rows()
     if "np" in want:

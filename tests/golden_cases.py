@@ -72,6 +72,9 @@ THICK_RUNS = [
     ("sd1p2n_m1", "td", dict(nkeep=4, niter=20, nthick=12, max_restarts=40, tol=1e-3, test=0)),
     ("sd1p2n_m1", "tx", dict(nkeep=4, niter=30, nthick=12, max_restarts=200, tol=1e-3, test=0, etarget=-16.2)),
     ("pf2p1n_m3", "tx", dict(nkeep=4, niter=30, nthick=12, max_restarts=200, tol=1e-3, test=0, etarget=-26.5)),
+    # option tc with the Ritz-vector tests 2 and 3
+    ("sd2p2n", "tc2", dict(nkeep=5, niter=30, nthick=12, max_restarts=40, tol=1e-4, test=2)),
+    ("pf2p1n_m3", "tc3", dict(nkeep=5, niter=30, nthick=12, max_restarts=40, tol=1e-4, test=3)),
 ]
 
 

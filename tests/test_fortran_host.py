@@ -655,17 +655,40 @@ def test_fortran_host_default_convergence_matches_reference(dtype, casedir):
 
 
 @needs_driver
-def test_fortran_host_refuses_wavefunction_convergence_test(casedir):
-    """Option lc with the wave-function convergence tests (2, 3) is refused
-    with a message, before any table is built."""
+def test_fortran_host_refuses_unknown_convergence_test(casedir):
+    """A convergence test number outside the reference's 0-3 (option lc), or
+    a Ritz-vector test for block Lanczos (bc), is refused with a message,
+    before any table is built."""
     from bigstickpublick_amd.inputs import small_case, write_case
     case = small_case("sd", 2, 2, 0, seed=77)
     case.name = "sd2p2n"
     wd = os.path.join(casedir, "fconv_refuse")
     write_case(case, wd)
-    lines = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end", "lc", "5 300", "2", "3", "1e-6"]
-    r = _run("\n".join(lines) + "\n", wd)
-    assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
+    head = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end"]
+    for tail in (["lc", "5 300", "2", "4", "1e-6"], ["bc", "5 3 100", "2", "2", "1e-6"]):
+        r = _run("\n".join(head + tail) + "\n", wd)
+        assert r.returncode != 0 and "convergence test not supported" in r.stdout, r.stdout + r.stderr
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_host_ritz_vector_convergence_matches_reference(casedir):
+    """Option lc with the Ritz-vector tests (2 at 1e-4, 3 at 1e-4, no extra
+    states) on sd 2p2n: the .lcoef ends where the reference's run ended (80
+    and 90 iterations)."""
+    from bigstickpublick_amd.inputs import small_case, write_case
+    case = small_case("sd", 2, 2, 0, seed=77)
+    case.name = "sd2p2n"
+    for test, label, stop in (("2", "lc2", 80), ("3", "lc3", 90)):
+        wd = os.path.join(casedir, "fconv%s" % test)
+        write_case(case, wd)
+        lines = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end", "lc", "5 300", "0", test, "1e-4"]
+        r = _run("\n".join(lines) + "\n", wd)
+        assert r.returncode == 0, r.stdout + r.stderr
+        lc = np.loadtxt(os.path.join(wd, "sd2p2n.lcoef"))
+        rlc = np.loadtxt(os.path.join(GOLD, "sd2p2n_%s.lcoef" % label))
+        assert lc.shape == rlc.shape == (stop, 3), (lc.shape, rlc.shape)
+        assert np.abs(lc[:20, 1:] - rlc[:20, 1:]).max() < 2e-4
 
 
 @needs_driver
@@ -743,8 +766,8 @@ def test_fortran_host_refuses_unsupported_option(casedir):
     head = ["n", "sd2p2n", "sd", "2 2", "0", "sd2p2n", "1 1 1 1", "end"]
     r = _run("\n".join(head + ["tp", "5 30"]) + "\n", wd)
     assert r.returncode != 0 and "is not supported" in r.stdout, r.stdout + r.stderr
-    r = _run("\n".join(head + ["LC", "5 300", "2", "3", "1e-6"]) + "\n", wd)
-    assert r.returncode != 0 and "energy convergence tests" in r.stdout, r.stdout + r.stderr
+    r = _run("\n".join(head + ["LC", "5 300", "2", "5", "1e-6"]) + "\n", wd)
+    assert r.returncode != 0 and "convergence test not supported" in r.stdout, r.stdout + r.stderr
 
 
 @needs_driver

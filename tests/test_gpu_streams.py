@@ -3,6 +3,7 @@ stream: a pivot or wave function that a caller writes on a non-blocking
 stream, behind a long kernel, is complete when bs_strength or bs_density
 reads it (called here through the C ABI with no host synchronization)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -71,3 +72,27 @@ def test_density_waits_for_a_vector_on_a_side_stream(system):
     assert norm2.value == n2
     for a, b in zip(out, (rpp, rnn, rpn)):
         assert np.array_equal(a, b)
+
+
+def test_torch_keeps_the_gpu_when_the_library_comes_first():
+    """A fresh process that builds and uploads a system before it imports
+    torch still has torch's GPU afterwards (the binding lets torch set the
+    device up before the library's first HIP call)."""
+    import subprocess
+    import sys
+    script = r'''
+import os, sys, tempfile
+sys.path.insert(0, %r)
+from bigstickpublick_amd import System
+from bigstickpublick_amd.inputs import small_case, write_case
+c = small_case("sd", 2, 2, 0, seed=77)
+d = tempfile.mkdtemp()
+write_case(c, d)
+s = System(os.path.join(d, "sd.sps"), os.path.join(d, c.name + ".int"), 2, 2, 0, upload=True)
+import torch
+assert torch.cuda.is_available()
+x = torch.ones(s.dim, device="cuda", dtype=torch.float64)
+print("ok", float(s.apply(x).sum()))
+''' % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr

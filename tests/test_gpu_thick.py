@@ -146,3 +146,39 @@ def _res(path):
                 rows.append(float(p[1]))
             break
     return np.array(rows)
+
+
+@pytest.mark.parametrize("name,label,test,tol,nex,stop,exact", [
+    ("sd2p2n", "lc2", 2, 1e-4, 0, 80, False), ("sd2p2n", "lc3", 3, 1e-4, 0, 90, True),
+    ("pf2p1n_m3", "lc3", 3, 1e-5, 0, 90, True), ("sd1p2n_m1", "lc2x", 2, 1e-4, 2, 128, True)])
+def test_ritz_vector_convergence_tests_stop_where_reference_does(name, label, test, tol, nex, stop, exact,
+                                                                 tmp_path):
+    """Plain runs with the Ritz-vector tests (option lc, tests 2 and 3): each
+    run (FP64 and mixed) stops where the restated rule stops on its own
+    coefficients, and that is where the reference's own run ended (its .lcoef
+    length) unless rounding decides it: on sd 2p2n three low states enter the
+    Krylov space of the uniform pivot only through rounding (as with option
+    ld), and at test 2's 1e-4 the FP64 run's vectors still move at iteration
+    80 (it stops at 90) where the reference's FP32 run's had settled.  With
+    extra check states (sd 1p2n) the reference has no vectors for them and
+    runs to the cap, and so does this one."""
+    from bigstickpublick_amd import System
+    from lanczos_ref import convergence_stop
+    case, cd, _ = small_fixture(name, str(tmp_path))
+    s = System(*paths(case, cd), case.Z, case.N, case.jz2, upload=True)
+    ref = read_lcoef(os.path.join(GOLD, "%s_%s.lcoef" % (name, label)))[1]
+    assert len(ref) == stop
+    cap = min(300, s.dim)
+    for dtype in ("f64", "f32"):
+        r = s.lanczos(nkeep=5, niter=300, dtype=dtype, tol=tol, conv_test=test, ncheck_ex=nex, polish=False)
+        steps = s.lanczos_stats()[2]
+        print(name, label, dtype, "steps", steps, "reference", stop)
+        assert convergence_stop(r.alpha[:steps], r.beta[:steps], 5, ncheck_ex=nex, tol=tol, test=test,
+                                niter=cap) == steps, dtype
+        if exact:
+            assert steps == stop, (dtype, steps)
+        else:
+            assert stop <= steps <= stop + 10, (dtype, steps)
+    with pytest.raises(RuntimeError, match="target"):
+        s.lanczos(nkeep=4, niter=30, thick=12, max_restarts=2, dtype="f64", tol=1e-4, conv_test=2, target=-20.0)
+    s.close()

@@ -300,6 +300,33 @@ def test_convergence_rule_reproduces_reference_stopping_points():
         assert len(a1) == stop and convergence_stop(a1, b1, 5, niter=cap) == stop, name
 
 
+def test_wavefunction_convergence_rule_reproduces_reference_stopping_points():
+    """Tests 2 and 3 (the Ritz vectors of successive checks compared), the
+    restatement replayed on the reference's own coefficients of its option-lc
+    runs, stop exactly where each .lcoef ends: sd 2p2n test 2 at 1e-4 after 80
+    iterations, test 3 at 1e-4 after 90, pf 2p1n test 3 at 1e-5 after 90; with
+    two extra check states (sd 1p2n) the reference's plain run has no vectors
+    for them (only the lowest nkeep are solved for), so the test never passes
+    and the run goes to its cap."""
+    from lanczos_ref import convergence_stop
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, test, tol, nex, stop in (("sd2p2n_lc2", 2, 1e-4, 0, 80), ("sd2p2n_lc3", 3, 1e-4, 0, 90),
+                                       ("pf2p1n_m3_lc3", 3, 1e-5, 0, 90), ("sd1p2n_m1_lc2x", 2, 1e-4, 2, 128)):
+        a, b = _read_lcoef(os.path.join(gold, name + ".lcoef"))
+        assert len(a) == stop, name
+        assert convergence_stop(a, b, 5, ncheck_ex=nex, tol=tol, test=test, niter=max(stop, 300) if nex == 0
+                                else stop) == stop, name
+    # planted variants do not stop there: the other wave-function test, a
+    # tighter tolerance, vectors for the extra states
+    a, b = _read_lcoef(os.path.join(gold, "sd2p2n_lc2.lcoef"))
+    a3, b3 = _read_lcoef(os.path.join(gold, "sd2p2n_lc3.lcoef"))
+    assert convergence_stop(a3, b3, 5, ncheck_ex=0, tol=1e-4, test=2, niter=300) == 80
+    assert convergence_stop(a3, b3, 5, ncheck_ex=0, tol=1e-4, test=3, niter=300) == 90
+    assert convergence_stop(a3, b3, 5, ncheck_ex=0, tol=1e-7, test=2, niter=300) != 80
+    ax, bx = _read_lcoef(os.path.join(gold, "sd1p2n_m1_lc2x.lcoef"))
+    assert convergence_stop(ax, bx, 7, ncheck_ex=0, tol=1e-4, test=2, niter=128) < 128
+
+
 def _read_lcoef(path):
     a, b = [], []
     for line in open(path):
