@@ -359,6 +359,43 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
   float* wb = reinterpret_cast<float*>(smem);
   constexpr int kWbPer = (kWbRows * G + BLOCK - 1) / BLOCK;
   if constexpr (R == G) {
+    // staged columns wider than 16 bytes (FP64, G = 4) are split in two
+    // halves in two LDS arrays, so that consecutive lanes' reads stride 16
+    // bytes over all 64 banks; xo is the table's offset (column * G *
+    // sizeof(T))
+    constexpr bool SPLIT = G * sizeof(T) > 16;
+    constexpr int HB = G * (int)sizeof(T) / 2;          // bytes per half column
+    constexpr size_t HALF = (size_t)kMaxCols * HB;      // offset of the second array
+    using HV = VecG<T, G / 2>;
+    auto store_col = [&](int c, const VecG<T, G>& v) {
+      if constexpr (SPLIT) {
+        HV lo, hi;
+#pragma unroll
+        for (int g = 0; g < G / 2; g++) {
+          lo.v[g] = v.v[g];
+          hi.v[g] = v.v[g + G / 2];
+        }
+        *reinterpret_cast<HV*>(smem + XS + (size_t)c * HB) = lo;
+        *reinterpret_cast<HV*>(smem + XS + HALF + (size_t)c * HB) = hi;
+      } else {
+        *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = v;
+      }
+    };
+    auto load_col = [&](uint32_t xo) {
+      VecG<T, G> v;
+      if constexpr (SPLIT) {
+        const HV lo = *reinterpret_cast<const HV*>(smem + XS + xo / 2);
+        const HV hi = *reinterpret_cast<const HV*>(smem + XS + HALF + xo / 2);
+#pragma unroll
+        for (int g = 0; g < G / 2; g++) {
+          v.v[g] = lo.v[g];
+          v.v[g + G / 2] = hi.v[g];
+        }
+      } else {
+        v = *reinterpret_cast<const VecG<T, G>*>(smem + XS + xo);
+      }
+      return v;
+    };
     // one staged row per tile row: batch b of class k holds jump b of every
     // tile row (slot g = row g; rows with fewer jumps leave their slot empty,
     // zero coefficients), so one walk of the neutron entries serves all R
@@ -424,7 +461,7 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
 #pragma unroll
           for (int q = 0; q < CPT; q++) {
             const int c = tid + q * BLOCK;
-            if (c < Li) *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = pre[q];
+            if (c < Li) store_col(c, pre[q]);
           }
         } else {
           int64_t bi[G];
@@ -434,7 +471,7 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
             VecG<T, G> v;
 #pragma unroll
             for (int g = 0; g < G; g++) v.v[g] = bi[g] >= 0 ? x[bi[g] + c] : T(0);
-            *reinterpret_cast<VecG<T, G>*>(xs + (size_t)c * G) = v;
+            store_col(c, v);
           }
         }
 #pragma unroll
@@ -459,7 +496,7 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
         walk_quads<CPT>(
             rpk, cl, Eq, (uint32_t)L * 16u,
             [&](uint32_t pk) {
-              return PnEntry<T, G>{*reinterpret_cast<const VecG<T, G>*>(smem + XS + (pk & 0x1ffffu)),
+              return PnEntry<T, G>{load_col(pk & 0x1ffffu),
                                    *reinterpret_cast<const VecG<float, G>*>(smem + (pk >> 17))};
             },
             [&](int q, const PnEntry<T, G>& en) {
