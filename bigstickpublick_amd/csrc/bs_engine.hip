@@ -2837,9 +2837,22 @@ void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumu
   d.tpp_src = s->tpp_src[ri].p;
   d.tpp_val32 = s->tpp_val32[ri].p;
   d.tpp_val64 = s->tpp_val64[ri].p;
-  // static LDS (lds_bytes_rg)
-  auto fn = pf ? k_apply_tiles<T, kApplyBlock, CPT, R, G, true> : k_apply_tiles<T, kApplyBlock, CPT, R, G, false>;
-  hipLaunchKernelGGL(fn, dim3(s->nslots[ri][map]), dim3(kApplyBlock), 0, st, d, x, y, accumulate);
+  // static LDS (lds_bytes_rg).  Where it leaves room for one workgroup per
+  // CU only (FP64 with four staged rows: 132 KB), the staged-after form runs
+  // 1024 threads with half the columns each, so a CU still holds 16 waves
+  // (pf 6p6n FP64 R4 G4: 54.8 -> 46.3 ms); every column keeps its order of
+  // accumulation, so the result is the same.  The PF form stays at 512: its
+  // batch loaded ahead spills at 1024 threads (47.2 -> 58.1 ms)
+  constexpr int S = R > G ? R : G;
+  constexpr bool kWide = (size_t)kWbBytes * G + (size_t)S * kMaxCols * sizeof(T) > kLdsLimit / 2 && CPT >= 2;
+  constexpr int B = kWide ? 2 * kApplyBlock : kApplyBlock;
+  constexpr int C = kWide ? CPT / 2 : CPT;
+  if (pf)
+    hipLaunchKernelGGL((k_apply_tiles<T, kApplyBlock, CPT, R, G, true>), dim3(s->nslots[ri][map]), dim3(kApplyBlock),
+                       0, st, d, x, y, accumulate);
+  else
+    hipLaunchKernelGGL((k_apply_tiles<T, B, C, R, G, false>), dim3(s->nslots[ri][map]), dim3(B), 0, st, d, x, y,
+                       accumulate);
   HIP_OK(hipGetLastError());
 }
 
