@@ -151,8 +151,9 @@ __device__ __forceinline__ u32x4 buf_u32x4(__amdgpu_buffer_rsrc_t r, uint32_t vo
 // load ([e/4][column][4], qstep bytes per quad row).  The next quad (the
 // rest of this column, or the next column's first) is in flight while the
 // current one is applied; load(pk) reads one entry's operands, apply(q, e) adds
-// its products.
-template <int CPT, typename Load, typename Apply>
+// its products.  PAIRS issues a quad as two pairs (half the registers for
+// operands in flight).
+template <int CPT, bool PAIRS = false, typename Load, typename Apply>
 __device__ __forceinline__ void walk_quads(__amdgpu_buffer_rsrc_t rpk, const uint32_t (&cl)[CPT], const int (&Eq)[CPT],
                                            uint32_t qstep, Load&& load, Apply&& apply) {
   u32x4 pq = {0u, 0u, 0u, 0u};
@@ -171,11 +172,22 @@ __device__ __forceinline__ void walk_quads(__amdgpu_buffer_rsrc_t rpk, const uin
       // issued together, then its products; slots past Ee hold padding
       // entries (a zero coefficient row), so a whole pair or quad is applied
       if (Ee - e > 2) {
-        const auto l0 = load(cur.x), l1 = load(cur.y), l2 = load(cur.z), l3 = load(cur.w);
-        apply(q, l0);
-        apply(q, l1);
-        apply(q, l2);
-        apply(q, l3);
+        if constexpr (PAIRS) {
+          {
+            const auto l0 = load(cur.x), l1 = load(cur.y);
+            apply(q, l0);
+            apply(q, l1);
+          }
+          const auto l2 = load(cur.z), l3 = load(cur.w);
+          apply(q, l2);
+          apply(q, l3);
+        } else {
+          const auto l0 = load(cur.x), l1 = load(cur.y), l2 = load(cur.z), l3 = load(cur.w);
+          apply(q, l0);
+          apply(q, l1);
+          apply(q, l2);
+          apply(q, l3);
+        }
       } else {
         const auto l0 = load(cur.x), l1 = load(cur.y);
         apply(q, l0);
@@ -493,7 +505,8 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
             }
           }
         }
-        walk_quads<CPT>(
+        // 1024-thread PF launches (FP64) issue pairs: quads spill there
+        walk_quads<CPT, (BLOCK > 512 && PF)>(
             rpk, cl, Eq, (uint32_t)L * 16u,
             [&](uint32_t pk) {
               return PnEntry<T, G>{load_col(pk & 0x1ffffu),
@@ -2838,18 +2851,19 @@ void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumu
   d.tpp_val32 = s->tpp_val32[ri].p;
   d.tpp_val64 = s->tpp_val64[ri].p;
   // static LDS (lds_bytes_rg).  Where it leaves room for one workgroup per
-  // CU only (FP64 with four staged rows: 132 KB), the staged-after form runs
-  // 1024 threads with half the columns each, so a CU still holds 16 waves
-  // (pf 6p6n FP64 R4 G4: 54.8 -> 46.3 ms); every column keeps its order of
-  // accumulation, so the result is the same.  The PF form stays at 512: its
-  // batch loaded ahead spills at 1024 threads (47.2 -> 58.1 ms)
+  // CU only (FP64 with four staged rows: 132 KB), the workgroup has 1024
+  // threads with half the columns each, so a CU still holds 16 waves; every
+  // column keeps its order of accumulation, so the result is the same.  The
+  // PF form then walks a quad as two pairs (k_apply_tiles: BLOCK > 512), which
+  // keeps its batch loaded ahead in registers (pf 6p6n FP64 R4 G4 PF: 47.2 ms
+  // at 512 threads, 58.1 at 1024 with quads, 42.7 with pairs)
   constexpr int S = R > G ? R : G;
   constexpr bool kWide = (size_t)kWbBytes * G + (size_t)S * kMaxCols * sizeof(T) > kLdsLimit / 2 && CPT >= 2;
   constexpr int B = kWide ? 2 * kApplyBlock : kApplyBlock;
   constexpr int C = kWide ? CPT / 2 : CPT;
   if (pf)
-    hipLaunchKernelGGL((k_apply_tiles<T, kApplyBlock, CPT, R, G, true>), dim3(s->nslots[ri][map]), dim3(kApplyBlock),
-                       0, st, d, x, y, accumulate);
+    hipLaunchKernelGGL((k_apply_tiles<T, B, C, R, G, true>), dim3(s->nslots[ri][map]), dim3(B), 0, st, d, x, y,
+                       accumulate);
   else
     hipLaunchKernelGGL((k_apply_tiles<T, B, C, R, G, false>), dim3(s->nslots[ri][map]), dim3(B), 0, st, d, x, y,
                        accumulate);
