@@ -505,8 +505,9 @@ __global__ __launch_bounds__(BLOCK, PF ? 4 : 1) void k_apply_tiles(DevTables tb,
             }
           }
         }
-        // 1024-thread PF launches (FP64) issue pairs: quads spill there
-        walk_quads<CPT, (BLOCK > 512 && PF)>(
+        // the PF forms issue pairs: the batch loaded ahead holds the registers
+        // a quad's operands would take (FP64 at 1024 threads spills with quads)
+        walk_quads<CPT, PF>(
             rpk, cl, Eq, (uint32_t)L * 16u,
             [&](uint32_t pk) {
               return PnEntry<T, G>{load_col(pk & 0x1ffffu),
@@ -2854,9 +2855,9 @@ void launch_variant(bs_system* s, int map, bool pf, const T* x, T* y, int accumu
   // CU only (FP64 with four staged rows: 132 KB), the workgroup has 1024
   // threads with half the columns each, so a CU still holds 16 waves; every
   // column keeps its order of accumulation, so the result is the same.  The
-  // PF form then walks a quad as two pairs (k_apply_tiles: BLOCK > 512), which
-  // keeps its batch loaded ahead in registers (pf 6p6n FP64 R4 G4 PF: 47.2 ms
-  // at 512 threads, 58.1 at 1024 with quads, 42.7 with pairs)
+  // PF forms walk a quad as two pairs (k_apply_tiles), which keeps the batch
+  // loaded ahead in registers (pf 6p6n FP64 R4 G4 PF: 47.2 ms at 512 threads,
+  // 58.1 at 1024 with quads, 42.7 with pairs)
   constexpr int S = R > G ? R : G;
   constexpr bool kWide = (size_t)kWbBytes * G + (size_t)S * kMaxCols * sizeof(T) > kLdsLimit / 2 && CPT >= 2;
   constexpr int B = kWide ? 2 * kApplyBlock : kApplyBlock;
